@@ -1,0 +1,163 @@
+"""Test infrastructure only: ctypes bridges to the CPU oracle (oracle/_build/liboracle.so)
+and, in the build container, to the reference renderer compiled from its own sources
+(oracle/_ref/libdfpsr_ref.so).  Both take the same scene_desc (oracle/scene_desc.h).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module,
+and only as the checker or the timed CPU baseline -- never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libdfpsr_ref.so")
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("has_color", C.c_int32), ("has_depth", C.c_int32),
+        ("initial_color", C.c_void_p), ("initial_depth", C.c_void_p),
+        ("cam_location", C.c_float * 12),
+        ("perspective", C.c_int32),
+        ("width_slope", C.c_float), ("near_clip", C.c_float), ("far_clip", C.c_float),
+        ("filter", C.c_int32),
+        ("point_count", C.c_int32), ("points", C.c_void_p),
+        ("part_count", C.c_int32),
+        ("part_polygon_start", C.c_void_p), ("part_diffuse", C.c_void_p), ("part_light", C.c_void_p),
+        ("indices", C.c_void_p), ("colors", C.c_void_p), ("texcoords", C.c_void_p),
+        ("texture_count", C.c_int32), ("texture_dims", C.c_void_p), ("texture_level0", C.c_void_p),
+        ("instance_count", C.c_int32), ("instance_transforms", C.c_void_p),
+        ("threaded", C.c_int32),
+    ]
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def make_desc(scene, threaded: bool = True) -> Tuple[SceneDesc, list]:
+    """Build a scene_desc for `scene`; the returned list keeps the arrays alive."""
+    keep = []
+
+    def arr(x, dt):
+        a = np.ascontiguousarray(x, dtype=dt)
+        keep.append(a)
+        return a
+
+    d = SceneDesc()
+    d.width, d.height = scene.width, scene.height
+    d.has_color, d.has_depth = int(scene.has_color), int(scene.has_depth)
+    d.initial_color = _ptr(arr(scene.initial_color, np.uint32)) if scene.initial_color is not None else None
+    d.initial_depth = _ptr(arr(scene.initial_depth, np.float32)) if scene.initial_depth is not None else None
+    d.cam_location[:] = [float(v) for v in np.asarray(scene.cam_location, dtype=np.float32)]
+    d.perspective = int(scene.perspective)
+    d.width_slope, d.near_clip, d.far_clip = scene.width_slope, scene.near_clip, scene.far_clip
+    d.filter = scene.filter
+    d.point_count = scene.points.shape[0]
+    d.points = _ptr(arr(scene.points, np.float32))
+    d.part_count = len(scene.parts)
+    starts = [p.polygon_start for p in scene.parts] + [scene.parts[-1].polygon_end if scene.parts else 0]
+    d.part_polygon_start = _ptr(arr(starts, np.int32))
+    d.part_diffuse = _ptr(arr([p.diffuse for p in scene.parts] or [0], np.int32))
+    d.part_light = _ptr(arr([p.light for p in scene.parts] or [0], np.int32))
+    d.indices = _ptr(arr(scene.indices, np.int32))
+    d.colors = _ptr(arr(scene.colors, np.float32))
+    d.texcoords = _ptr(arr(scene.texcoords, np.float32))
+    d.texture_count = len(scene.textures)
+    dims = []
+    ptrs = (C.c_void_p * max(1, len(scene.textures)))()
+    for i, t in enumerate(scene.textures):
+        dims += [t.width, t.height, t.resolutions]
+        lv = arr(t.level0, np.uint32)
+        ptrs[i] = lv.ctypes.data
+    keep.append(ptrs)
+    d.texture_dims = _ptr(arr(dims or [0], np.int32))
+    d.texture_level0 = C.cast(ptrs, C.c_void_p)
+    d.instance_count = scene.instances.shape[0]
+    d.instance_transforms = _ptr(arr(scene.instances, np.float32))
+    d.threaded = int(threaded)
+    return d, keep
+
+
+_libs = {}
+
+
+def _load(path: str, name: str):
+    if name not in _libs:
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built")
+        lib = C.CDLL(path)
+        _libs[name] = lib
+    return _libs[name]
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def load_ref():
+    lib = _load(REF_SO, "ref")
+    lib.ref_render_scene.argtypes = [C.POINTER(SceneDesc), C.c_void_p, C.c_void_p]
+    lib.ref_time_scene.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    return lib
+
+
+def load_oracle():
+    lib = _load(ORACLE_SO, "oracle")
+    lib.oracle_render_scene.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p]
+    lib.oracle_render_scene.restype = C.c_int32
+    lib.oracle_time_scene.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    lib.oracle_time_scene.restype = C.c_int32
+    return lib
+
+
+def _outputs(scene):
+    color = np.zeros((scene.height, scene.width), dtype=np.uint32)
+    depth = np.zeros((scene.height, scene.width), dtype=np.float32)
+    return color, depth
+
+
+def ref_render(scene, threaded: bool = True):
+    lib = load_ref()
+    d, keep = make_desc(scene, threaded)
+    color, depth = _outputs(scene)
+    rc = lib.ref_render_scene(C.byref(d), color.ctypes.data_as(C.c_void_p), depth.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError("reference renderer failed")
+    return color, depth
+
+
+def oracle_render(scene, threads: int = 1):
+    """Render with the oracle; threads > 1 splits rows into 12 slices like CommandQueue::execute."""
+    lib = load_oracle()
+    d, keep = make_desc(scene, threads > 1)
+    color, depth = _outputs(scene)
+    rc = lib.oracle_render_scene(C.byref(d), threads, color.ctypes.data_as(C.c_void_p), depth.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"oracle failed with {rc}")
+    return color, depth
+
+
+def oracle_time(scene, threads: int, warmup: int, iterations: int) -> float:
+    lib = load_oracle()
+    d, keep = make_desc(scene, threads > 1)
+    out = C.c_double()
+    rc = lib.oracle_time_scene(C.byref(d), threads, warmup, iterations, C.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle timing failed with {rc}")
+    return out.value
+
+
+def ref_time(scene, warmup: int, iterations: int) -> float:
+    lib = load_ref()
+    d, keep = make_desc(scene, True)
+    out = C.c_double()
+    if lib.ref_time_scene(C.byref(d), warmup, iterations, C.byref(out)) != 0:
+        raise RuntimeError("reference timing failed")
+    return out.value

@@ -1,0 +1,49 @@
+/*
+ * CPU oracle for the polygon rendering path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A scalar C++14 restatement of the reference renderer's arithmetic, used by
+ * tests/ as the checker of the GPU library and by bench.py as a CPU baseline.
+ * The product path (libdfpsr_hip.so) never links, loads or calls it.
+ * Each function cites the reference file:line it restates (paths relative to
+ * the reference's Source/ folder).
+ */
+#ifndef DFPSR_ORACLE_H
+#define DFPSR_ORACLE_H
+
+#include <stdint.h>
+#include "scene_desc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Render one frame.  threads <= 1: every command over the whole canvas in order
+ * (model_render, modelAPI.cpp:197-201).  threads > 1: 12 row slices like
+ * CommandQueue::execute (renderCore.cpp:449-480), shared by min(threads, 12)
+ * worker threads.  Both give identical images.  Returns 0 on success. */
+int32_t oracle_render_scene(const scene_desc *scene, int32_t threads, uint32_t *out_color, float *out_depth);
+
+/* Median seconds of the begin..end frame over `iterations` (after `warmup`). */
+int32_t oracle_time_scene(const scene_desc *scene, int32_t threads, int32_t warmup, int32_t iterations, double *out_seconds);
+
+/* Stage-level entry points used to pin the restatement piece by piece. */
+void oracle_reciprocal(int32_t n, const float *in, float *out);
+void oracle_rasterize(const int64_t *corners, int32_t left, int32_t top, int32_t width, int32_t height, int32_t *out_rows);
+int32_t oracle_texture_pyramid(int32_t w, int32_t h, int32_t res, const uint32_t *level0, uint32_t *out_buffer,
+                               int32_t capacity, int32_t *out_log2w, int32_t *out_log2h, int32_t *out_max_mip);
+int32_t oracle_sample_quads(int32_t w, int32_t h, int32_t res, const uint32_t *level0, int32_t quad_count,
+                            const float *u, const float *v, uint32_t *out_color, uint32_t *out_mip);
+void oracle_project(const scene_desc *scene, const float *transform, int32_t n, const float *points,
+                    float *out_cs, float *out_is, int64_t *out_flat);
+
+/* Draw-command list of a frame, as renderer_end would execute it: per command the
+ * source triangle (instance-major, then polygon fan order), the three projected
+ * corners (flat sub-pixel x,y) and the pixel bound.  Returns the command count
+ * (or the required capacity if larger than `capacity`). */
+int32_t oracle_commands(const scene_desc *scene, int32_t capacity, int32_t *out_triangle, int64_t *out_flat, int32_t *out_bound);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,242 @@
+// Test infrastructure only.  A small driver of the reference renderer's own
+// public API, compiled against the reference sources where they lie (see
+// Makefile.ref).  It turns a scene_desc into exactly the calls an application
+// makes (Source/SDK/terrain/main.cpp:120-176 for model building,
+// :381-421 for the frame), and exposes a few of the reference's internal
+// functions so the oracle's restatement can be pinned piece by piece.
+// Nothing here is shipped; the GPU box never runs it for parity.
+
+#include "DFPSR/includeFramework.h"
+#include "DFPSR/implementation/render/ITriangle2D.h"
+#include "DFPSR/implementation/render/renderCore.h"
+#include "scene_desc.h"
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+using namespace dsr;
+
+static Transform3D makeTransform(const float *t) {
+	return Transform3D(FVector3D(t[0], t[1], t[2]),
+	                   FMatrix3x3(FVector3D(t[3], t[4], t[5]), FVector3D(t[6], t[7], t[8]), FVector3D(t[9], t[10], t[11])));
+}
+
+static Camera makeCamera(const scene_desc *s) {
+	Transform3D location = makeTransform(s->cam_location);
+	if (s->perspective) {
+		return Camera::createPerspective(location, float(s->width), float(s->height), s->width_slope, s->near_clip, s->far_clip);
+	} else {
+		return Camera::createOrthogonal(location, float(s->width), float(s->height), s->width_slope);
+	}
+}
+
+static TextureRgbaU8 makeTexture(int32_t w, int32_t h, int32_t res, const uint32_t *level0) {
+	TextureRgbaU8 tex = texture_create_RgbaU8(w, h, res);
+	int32_t tw = texture_getMaxWidth(tex);
+	int32_t th = texture_getMaxHeight(tex);
+	for (int32_t y = 0; y < th; y++) {
+		SafePointer<uint32_t> row = texture_getSafePointer(tex, 0, y);
+		for (int32_t x = 0; x < tw; x++) {
+			row[x] = level0[(y % h) * w + (x % w)];
+		}
+	}
+	texture_generatePyramid(tex);
+	return tex;
+}
+
+struct BuiltScene {
+	Model model;
+	std::vector<TextureRgbaU8> textures;
+	Camera camera;
+	OrderedImageRgbaU8 color;
+	AlignedImageF32 depth;
+};
+
+static void buildScene(const scene_desc *s, BuiltScene &b) {
+	for (int32_t t = 0; t < s->texture_count; t++) {
+		b.textures.push_back(makeTexture(s->texture_dims[t * 3], s->texture_dims[t * 3 + 1], s->texture_dims[t * 3 + 2], s->texture_level0[t]));
+	}
+	b.model = model_create();
+	model_setFilter(b.model, s->filter ? Filter::Alpha : Filter::Solid);
+	for (int32_t p = 0; p < s->point_count; p++) {
+		model_addPoint(b.model, FVector3D(s->points[p * 3], s->points[p * 3 + 1], s->points[p * 3 + 2]));
+	}
+	for (int32_t part = 0; part < s->part_count; part++) {
+		int32_t partIndex = model_addEmptyPart(b.model, U"part");
+		if (s->part_diffuse && s->part_diffuse[part] >= 0) model_setDiffuseMap(b.model, partIndex, b.textures[s->part_diffuse[part]]);
+		if (s->part_light && s->part_light[part] >= 0) model_setLightMap(b.model, partIndex, b.textures[s->part_light[part]]);
+		for (int32_t poly = s->part_polygon_start[part]; poly < s->part_polygon_start[part + 1]; poly++) {
+			const int32_t *idx = s->indices + poly * 4;
+			int32_t polyIndex;
+			int32_t corners;
+			if (idx[3] < 0) {
+				polyIndex = model_addTriangle(b.model, partIndex, idx[0], idx[1], idx[2]);
+				corners = 3;
+			} else {
+				polyIndex = model_addQuad(b.model, partIndex, idx[0], idx[1], idx[2], idx[3]);
+				corners = 4;
+			}
+			for (int32_t c = 0; c < corners; c++) {
+				const float *col = s->colors + poly * 16 + c * 4;
+				const float *tc = s->texcoords + poly * 16 + c * 4;
+				model_setVertexColor(b.model, partIndex, polyIndex, c, FVector4D(col[0], col[1], col[2], col[3]));
+				model_setTexCoord(b.model, partIndex, polyIndex, c, FVector4D(tc[0], tc[1], tc[2], tc[3]));
+			}
+		}
+	}
+	b.camera = makeCamera(s);
+	if (s->has_color) {
+		b.color = image_create_RgbaU8(s->width, s->height);
+	}
+	if (s->has_depth) {
+		b.depth = image_create_F32(s->width, s->height);
+	}
+}
+
+static void loadCanvas(const scene_desc *s, BuiltScene &b) {
+	for (int32_t y = 0; y < s->height; y++) {
+		if (s->has_color) {
+			SafePointer<uint32_t> row = image_getSafePointer(b.color, y);
+			for (int32_t x = 0; x < s->width; x++) row[x] = s->initial_color ? s->initial_color[y * s->width + x] : 0u;
+		}
+		if (s->has_depth) {
+			SafePointer<float> row = image_getSafePointer(b.depth, y);
+			for (int32_t x = 0; x < s->width; x++) row[x] = s->initial_depth ? s->initial_depth[y * s->width + x] : 0.0f;
+		}
+	}
+}
+
+static void renderFrame(const scene_desc *s, BuiltScene &b) {
+	if (s->threaded) {
+		Renderer renderer = renderer_create();
+		ImageRgbaU8 color = b.color;
+		ImageF32 depth = b.depth;
+		renderer_begin(renderer, color, depth);
+		for (int32_t i = 0; i < s->instance_count; i++) {
+			renderer_giveTask(renderer, b.model, makeTransform(s->instance_transforms + i * 12), b.camera);
+		}
+		renderer_end(renderer);
+	} else {
+		ImageRgbaU8 color = b.color;
+		ImageF32 depth = b.depth;
+		for (int32_t i = 0; i < s->instance_count; i++) {
+			model_render(b.model, makeTransform(s->instance_transforms + i * 12), color, depth, b.camera);
+		}
+	}
+}
+
+static void readCanvas(const scene_desc *s, BuiltScene &b, uint32_t *outColor, float *outDepth) {
+	for (int32_t y = 0; y < s->height; y++) {
+		if (s->has_color && outColor) {
+			SafePointer<uint32_t> row = image_getSafePointer(b.color, y);
+			for (int32_t x = 0; x < s->width; x++) outColor[y * s->width + x] = row[x];
+		}
+		if (s->has_depth && outDepth) {
+			SafePointer<float> row = image_getSafePointer(b.depth, y);
+			for (int32_t x = 0; x < s->width; x++) outDepth[y * s->width + x] = row[x];
+		}
+	}
+}
+
+extern "C" {
+
+int ref_render_scene(const scene_desc *s, uint32_t *outColor, float *outDepth) {
+	try {
+		BuiltScene b;
+		buildScene(s, b);
+		loadCanvas(s, b);
+		renderFrame(s, b);
+		readCanvas(s, b, outColor, outDepth);
+		return 0;
+	} catch (...) {
+		return -1;
+	}
+}
+
+// Median wall time of renderer_begin..renderer_end over `iterations` frames,
+// with the canvas reloaded (untimed) before each.
+int ref_time_scene(const scene_desc *s, int32_t warmup, int32_t iterations, double *outSeconds) {
+	try {
+		BuiltScene b;
+		buildScene(s, b);
+		std::vector<double> times;
+		for (int32_t i = 0; i < warmup + iterations; i++) {
+			loadCanvas(s, b);
+			auto t0 = std::chrono::steady_clock::now();
+			renderFrame(s, b);
+			auto t1 = std::chrono::steady_clock::now();
+			if (i >= warmup) times.push_back(std::chrono::duration<double>(t1 - t0).count());
+		}
+		std::sort(times.begin(), times.end());
+		*outSeconds = times.empty() ? 0.0 : times[times.size() / 2];
+		return 0;
+	} catch (...) {
+		return -1;
+	}
+}
+
+// The full pyramid buffer (smallest level first) after texture_generatePyramid.
+int ref_texture_pyramid(int32_t w, int32_t h, int32_t res, const uint32_t *level0, uint32_t *outBuffer, int32_t capacity,
+                        int32_t *outLog2w, int32_t *outLog2h, int32_t *outMaxMip) {
+	TextureRgbaU8 tex = makeTexture(w, h, res, level0);
+	int32_t levels = texture_getMipLevelCount(tex);
+	int32_t total = 0;
+	for (int32_t l = 0; l < levels; l++) total += texture_getWidth(tex, l) * texture_getHeight(tex, l);
+	if (total > capacity) return -1;
+	SafePointer<uint32_t> data = tex.impl_buffer.getSafe<uint32_t>("pyramid");
+	for (int32_t i = 0; i < total; i++) outBuffer[i] = data[i];
+	*outLog2w = tex.impl_log2width;
+	*outLog2h = tex.impl_log2height;
+	*outMaxMip = tex.impl_maxMipLevel;
+	return total;
+}
+
+// Mip selection + bilinear sampling of 2x2 quads, as the "with mipmap" shaders
+// call them (shaderMethods.h:59-83).  u, v: quadCount x 4 lanes.
+int ref_sample_quads(int32_t w, int32_t h, int32_t res, const uint32_t *level0, int32_t quadCount,
+                     const float *u, const float *v, uint32_t *outColor, uint32_t *outMip) {
+	TextureRgbaU8 tex = makeTexture(w, h, res, level0);
+	for (int32_t q = 0; q < quadCount; q++) {
+		F32x4 vu(u[q * 4], u[q * 4 + 1], u[q * 4 + 2], u[q * 4 + 3]);
+		F32x4 vv(v[q * 4], v[q * 4 + 1], v[q * 4 + 2], v[q * 4 + 3]);
+		uint32_t mip = texture_getMipLevelIndex<F32x4>(tex, vu, vv);
+		U32x4 c = texture_sample_bilinear<false, false, true, false>(tex, vu, vv, mip);
+		UVector4D cv = c.get();
+		outColor[q * 4] = cv.x; outColor[q * 4 + 1] = cv.y; outColor[q * 4 + 2] = cv.z; outColor[q * 4 + 3] = cv.w;
+		outMip[q] = mip;
+	}
+	return 0;
+}
+
+// 1 / x as the perspective filler computes it (simd.h:4045-4068).
+void ref_reciprocal(int32_t n, const float *in, float *out) {
+	for (int32_t i = 0; i + 3 < n; i += 4) {
+		FVector4D r = reciprocal(F32x4(in[i], in[i + 1], in[i + 2], in[i + 3])).get();
+		out[i] = r.x; out[i + 1] = r.y; out[i + 2] = r.z; out[i + 3] = r.w;
+	}
+}
+
+// Row intervals of one triangle in sub-pixel coordinates (ITriangle2D.cpp:150-169).
+void ref_rasterize(const int64_t *corners, int32_t left, int32_t top, int32_t width, int32_t height, int32_t *outRows) {
+	std::vector<RowInterval> rows(height);
+	rasterizeTriangle(LVector2D(corners[0], corners[1]), LVector2D(corners[2], corners[3]), LVector2D(corners[4], corners[5]), rows.data(), IRect(left, top, width, height));
+	for (int32_t r = 0; r < height; r++) {
+		outRows[r * 2] = rows[r].left;
+		outRows[r * 2 + 1] = rows[r].right;
+	}
+}
+
+// Camera::worldToScreen of points (Camera.h:157-187): cs xyz, is xy, flat xy.
+void ref_project(const scene_desc *s, const float *transform, int32_t n, const float *points, float *outCs, float *outIs, int64_t *outFlat) {
+	Camera camera = makeCamera(s);
+	Transform3D t = makeTransform(transform);
+	for (int32_t i = 0; i < n; i++) {
+		ProjectedPoint p = camera.worldToScreen(t.transformPoint(FVector3D(points[i * 3], points[i * 3 + 1], points[i * 3 + 2])));
+		outCs[i * 3] = p.cs.x; outCs[i * 3 + 1] = p.cs.y; outCs[i * 3 + 2] = p.cs.z;
+		outIs[i * 2] = p.is.x; outIs[i * 2 + 1] = p.is.y;
+		outFlat[i * 2] = p.flat.x; outFlat[i * 2 + 1] = p.flat.y;
+	}
+}
+
+}
