@@ -114,7 +114,61 @@ def load_oracle():
     lib.oracle_render_scene.restype = C.c_int32
     lib.oracle_time_scene.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     lib.oracle_time_scene.restype = C.c_int32
+    lib.oracle_texture_query.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
+    lib.oracle_texture_query.restype = C.c_uint32
+    lib.oracle_mix_colors.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    lib.oracle_mix_colors.restype = C.c_uint32
+    lib.oracle_pack_saturated.argtypes = [C.c_float] * 4
+    lib.oracle_pack_saturated.restype = C.c_uint32
+    lib.oracle_reciprocal.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
+    lib.oracle_rasterize.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+    lib.oracle_texture_pyramid.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.oracle_texture_pyramid.restype = C.c_int32
+    lib.oracle_sample_quads.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]
+    lib.oracle_project.argtypes = [C.POINTER(SceneDesc), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.oracle_commands.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.oracle_commands.restype = C.c_int32
     return lib
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def oracle_rasterize(corners, left, top, width, height):
+    lib = load_oracle()
+    c = np.ascontiguousarray(corners, dtype=np.int64)
+    rows = np.zeros((height, 2), dtype=np.int32)
+    lib.oracle_rasterize(ptr(c), left, top, width, height, ptr(rows))
+    return rows
+
+
+def oracle_commands(scene):
+    """Draw commands of a frame: (source triangle, flat corners (n,3,2), pixel bound (n,4))."""
+    lib = load_oracle()
+    d, keep = make_desc(scene, False)
+    n = lib.oracle_commands(C.byref(d), 0, None, None, None)
+    tri = np.zeros(n, dtype=np.int32)
+    flat = np.zeros((n, 3, 2), dtype=np.int64)
+    bound = np.zeros((n, 4), dtype=np.int32)
+    lib.oracle_commands(C.byref(d), n, ptr(tri), ptr(flat), ptr(bound))
+    return tri, flat, bound
+
+
+def oracle_project(scene, transform, points):
+    lib = load_oracle()
+    d, keep = make_desc(scene, False)
+    p = np.ascontiguousarray(points, dtype=np.float32)
+    t = np.ascontiguousarray(transform, dtype=np.float32)
+    n = p.shape[0]
+    cs = np.zeros((n, 3), np.float32)
+    is_ = np.zeros((n, 2), np.float32)
+    flat = np.zeros((n, 2), np.int64)
+    lib.oracle_project(C.byref(d), ptr(t), n, ptr(p), ptr(cs), ptr(is_), ptr(flat))
+    return cs, is_, flat
 
 
 def _outputs(scene):

@@ -417,13 +417,13 @@ inline uint32_t pixelOffset(const Texture &t, uint32_t x, uint32_t y, uint32_t m
 
 // texture_create_RgbaU8 + the Texture constructor (textureAPI.cpp:65-78, Texture.h:66-101),
 // level 0 filled from `level0`, then texture_generatePyramid's box filter (textureAPI.cpp:44-87).
-Texture makeTexture(int32_t w, int32_t h, int32_t resolutions, const uint32_t *level0) {
+// The TextureRgbaU8 constructor's layout (Texture.h:66-101): levels stored smallest first.
+Texture layoutTexture(uint32_t log2w, uint32_t log2h, uint32_t requestedMaxMip) {
 	Texture t;
-	t.log2w = log2Ceil((uint32_t)w);
-	t.log2h = log2Ceil((uint32_t)h);
-	uint32_t requested = (uint32_t)(resolutions - 1);
-	if (requested > 15u) requested = 15u;
-	t.maxMip = std::min(std::min(t.log2w, t.log2h), requested);
+	t.log2w = log2w;
+	t.log2h = log2h;
+	if (requestedMaxMip > 15u) requestedMaxMip = 15u;
+	t.maxMip = std::min(std::min(t.log2w, t.log2h), requestedMaxMip);
 	uint32_t highest = 1u << (t.log2w + t.log2h);
 	uint64_t pixelCount = 0;
 	uint32_t levelCount = highest;
@@ -440,10 +440,11 @@ Texture makeTexture(int32_t w, int32_t h, int32_t resolutions, const uint32_t *l
 	t.floatMaxWidth = float(1u << t.log2w);
 	t.floatMaxHeight = float(1u << t.log2h);
 	t.buffer.assign((size_t)pixelCount, 0u);
-	uint32_t tw = 1u << t.log2w, th = 1u << t.log2h;
-	for (uint32_t y = 0; y < th; y++)
-		for (uint32_t x = 0; x < tw; x++)
-			t.buffer[t.startOffset + y * tw + x] = level0[(y % (uint32_t)h) * (uint32_t)w + (x % (uint32_t)w)];
+	return t;
+}
+
+// box filter of texture_generatePyramid / downsample (textureAPI.cpp:44-87)
+void generatePyramid(Texture &t) {
 	for (uint32_t level = 1; level <= t.maxMip; level++) {
 		uint32_t lw = levelWidth(t, level), lh = levelHeight(t, level);
 		for (uint32_t y = 0; y < lh; y++) {
@@ -461,7 +462,26 @@ Texture makeTexture(int32_t w, int32_t h, int32_t resolutions, const uint32_t *l
 			}
 		}
 	}
+}
+
+// texture_create_RgbaU8 (textureAPI.cpp:65-78), level 0 filled from `level0`, then the pyramid.
+Texture makeTexture(int32_t w, int32_t h, int32_t resolutions, const uint32_t *level0) {
+	Texture t = layoutTexture(log2Ceil((uint32_t)w), log2Ceil((uint32_t)h), (uint32_t)(resolutions - 1));
+	uint32_t tw = 1u << t.log2w, th = 1u << t.log2h;
+	for (uint32_t y = 0; y < th; y++)
+		for (uint32_t x = 0; x < tw; x++)
+			t.buffer[t.startOffset + y * tw + x] = level0[(y % (uint32_t)h) * (uint32_t)w + (x % (uint32_t)w)];
+	generatePyramid(t);
 	return t;
+}
+
+// texture_sample_nearest (textureAPI.h:232-244); not used by the shaders, kept to pin the
+// wrap offset and truncation that the bilinear sampler shares.
+uint32_t sampleNearest(const Texture &t, float u, float v, uint32_t mip) {
+	uint32_t scaleU = (1u << t.log2w) >> mip, scaleV = (1u << t.log2h) >> mip;
+	uint32_t x = truncU32((u + 256.0f) * fromU32(scaleU));
+	uint32_t y = truncU32((v + 256.0f) * fromU32(scaleV));
+	return t.buffer[pixelOffset(t, x, y, mip)];
 }
 
 // texture_getMipLevelIndex (textureAPI.h:472-495) from quad lanes 0, 1 and 2.
@@ -1119,6 +1139,29 @@ int32_t oracle_time_scene(const scene_desc *s, int32_t threads, int32_t warmup, 
 	*outSeconds = times.empty() ? 0.0 : times[times.size() / 2];
 	return 0;
 }
+
+uint32_t oracle_texture_query(int32_t log2w, int32_t log2h, int32_t maxMip, int32_t writeCount, const uint32_t *writes,
+                              int32_t op, uint32_t x, uint32_t y, uint32_t mip, float u, float v) {
+	Texture t = layoutTexture((uint32_t)log2w, (uint32_t)log2h, (uint32_t)maxMip);
+	for (int32_t i = 0; i < writeCount; i++) {
+		const uint32_t *wr = writes + i * 4;
+		t.buffer[pixelOffset(t, wr[0], wr[1], wr[2])] = wr[3];
+	}
+	switch (op) {
+	case 0: return t.startOffset;
+	case 1: return t.maxLevelMask;
+	case 2: return pixelOffsetToLayer(t, mip);
+	case 3: return pixelOffset(t, x, y, mip);
+	case 4: return t.buffer[pixelOffset(t, x, y, mip)];
+	case 5: return sampleNearest(t, u, v, mip);
+	case 6: return t.maxMip;
+	default: return 0xFFFFFFFFu;
+	}
+}
+
+uint32_t oracle_mix_colors(uint32_t a, uint32_t b, uint32_t weight) { return mixColors(a, b, weight); }
+
+uint32_t oracle_pack_saturated(float r, float g, float b, float a) { return packSaturated(r, g, b, a); }
 
 void oracle_reciprocal(int32_t n, const float *in, float *out) {
 	for (int32_t i = 0; i < n; i++) out[i] = reciprocalLane(in[i]);

@@ -28,6 +28,15 @@ int32_t oracle_time_scene(const scene_desc *scene, int32_t threads, int32_t warm
 
 /* Stage-level entry points used to pin the restatement piece by piece. */
 void oracle_reciprocal(int32_t n, const float *in, float *out);
+/* Texture layout probes on a TextureRgbaU8(log2w, log2h, maxMip) after `writes` (x, y, mip, value)
+ * texture_writePixel calls.  op: 0 startOffset, 1 maxLevelMask, 2 layer offset(mip),
+ * 3 pixel offset(x, y, mip), 4 readPixel(x, y, mip), 5 sample_nearest(u, v, mip), 6 max mip. */
+uint32_t oracle_texture_query(int32_t log2w, int32_t log2h, int32_t max_mip, int32_t write_count, const uint32_t *writes,
+                              int32_t op, uint32_t x, uint32_t y, uint32_t mip, float u, float v);
+/* texture_interpolate_color_linear of one pixel pair (textureAPI.h:302-315). */
+uint32_t oracle_mix_colors(uint32_t a, uint32_t b, uint32_t weight);
+/* packOrder_floatToSaturatedByte for RGBA (PackOrder.h:191-212). */
+uint32_t oracle_pack_saturated(float r, float g, float b, float a);
 void oracle_rasterize(const int64_t *corners, int32_t left, int32_t top, int32_t width, int32_t height, int32_t *out_rows);
 int32_t oracle_texture_pyramid(int32_t w, int32_t h, int32_t res, const uint32_t *level0, uint32_t *out_buffer,
                                int32_t capacity, int32_t *out_log2w, int32_t *out_log2h, int32_t *out_max_mip);

@@ -1,0 +1,128 @@
+"""Fixed, seeded scenes whose reference renders are committed under tests/golden/.
+
+The list covers the edge cases of the path: clipping against every frustum plane, near-plane
+crossings, odd canvas sizes (the last odd row, fillerTemplates.h:287-289), single-pixel
+canvases, degenerate and zero-area triangles, shared edges, textures with negative UVs and
+every mip level, lightmaps, the Alpha filter, orthogonal cameras, missing colour or depth
+canvases, initial canvas content and several instances of one model.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from dfpsr_amd import scenes as S
+
+
+def variant(seed: int) -> S.Scene:
+    """Random small scene; the kind of edge case is picked by seed % 7."""
+    rng = S.Rng(1000 + seed)
+
+    def r():
+        return float(rng.uniform(1)[0])
+
+    w = int(rng.integers(1, 1, 160)[0])
+    h = int(rng.integers(1, 1, 120)[0])
+    n = int(rng.integers(1, 1, 300)[0])
+    sc = S.cube_soup(width=w, height=h, triangles=n, seed=seed, spread=0.1 + 1.5 * r(),
+                     cam_distance=0.5 + 3 * r(), width_slope=0.3 + 2 * r(),
+                     far_clip=[100.0, 2.5, float("inf")][seed % 3], name=f"variant{seed}")
+    kind = seed % 7
+    if kind == 1:
+        sc.textures = [S.noise_checker_texture(64, seed, 7, check=8)]
+        sc.parts = [S.Part(0, n, diffuse=0)]
+        sc.texcoords[:, :3, :2] = rng.uniform(n * 6, -2, 3).reshape(n, 3, 2)
+    elif kind == 2:
+        sc.textures = [S.noise_checker_texture(32, seed, 6, check=4), S.noise_checker_texture(16, seed + 1, 1, check=2)]
+        sc.parts = [S.Part(0, n // 2, diffuse=0, light=1), S.Part(n // 2, n, light=1)]
+        sc.texcoords[:, :3, :] = rng.uniform(n * 12, -1, 2).reshape(n, 3, 4)
+        sc.filter = 1
+        sc.colors[:, :, 3] = rng.uniform(n * 4).reshape(n, 4)
+    elif kind == 3:
+        sc.perspective = False
+        sc.width_slope = 0.5 + 2 * r()
+    elif kind == 4:
+        if seed % 2:
+            sc.has_color = False
+        else:
+            sc.has_depth = False
+    elif kind == 5:
+        sc.colors[:] = 1.0
+        sc.textures = [S.noise_checker_texture(128, seed, 8, check=16)]
+        sc.parts = [S.Part(0, n, diffuse=0)]
+        sc.texcoords[:, :3, :2] = rng.uniform(n * 6, -1, 2).reshape(n, 3, 2)
+    elif kind == 6:
+        sc.initial_color = S.Rng(seed).u64(w * h).astype(np.uint32).reshape(h, w)
+        sc.initial_depth = S.Rng(seed + 9).uniform(w * h, 0, 0.5).reshape(h, w)
+        sc.cam_location = S.look_transform((0.3, 0.2, -2.0), 0.2, 0.1)
+        t2 = S.identity_transform()
+        t2[0] = 0.3
+        sc.instances = np.stack([S.identity_transform(), t2])
+    return sc
+
+
+def micro_scenes():
+    """Hand-placed triangles in pixel coordinates."""
+    out = []
+    # Two triangles sharing an edge in opposite directions (watertight tie rule).
+    out.append(S.single_triangles(32, 24, [[(4.3, 3.1), (27.7, 5.2), (6.1, 20.4)], [(27.7, 5.2), (25.9, 21.8), (6.1, 20.4)]],
+                                  name="shared_edge"))
+    # Zero-area and sub-pixel triangles, and one with two equal corners.
+    out.append(S.single_triangles(16, 16, [[(2.0, 2.0), (9.0, 2.0), (5.0, 2.0)], [(3.2, 3.2), (3.4, 3.3), (3.3, 3.5)],
+                                           [(5.0, 5.0), (5.0, 5.0), (12.0, 11.0)], [(1.5, 9.5), (14.5, 10.5), (2.5, 15.5)]],
+                                  name="degenerate"))
+    # Odd canvas: the last row pair has no lower row.
+    out.append(S.single_triangles(31, 17, [[(1.0, 1.0), (30.0, 3.0), (2.0, 16.9)], [(30.0, 3.0), (29.0, 16.9), (2.0, 16.9)]],
+                                  name="odd_canvas"))
+    # Nearly horizontal edges with long row spans.
+    out.append(S.single_triangles(96, 20, [[(0.5, 2.0), (95.5, 3.0), (48.0, 18.0)], [(10.0, 8.2), (90.0, 8.4), (12.0, 8.9)]],
+                                  name="long_spans"))
+    # A 1x1 canvas.
+    out.append(S.single_triangles(1, 1, [[(-1.0, -1.0), (2.0, -1.0), (-1.0, 2.0)]], name="one_pixel"))
+    # Depth ordering: three overlapping triangles at different and equal depths (ties lose).
+    sc = S.single_triangles(24, 24, [[(2, 2), (22, 4), (4, 21)], [(3, 3), (21, 5), (5, 20)], [(1, 1), (23, 3), (3, 22)]],
+                            depth=2.0, name="depth_ties")
+    out.append(sc)
+    # Textured micro triangles covering mip selection and wrapping.
+    out.append(S.single_triangles(40, 40, [[(1, 1), (39, 2), (2, 38)], [(39, 2), (38, 39), (2, 38)]], textured=True,
+                                  name="textured_micro"))
+    return out
+
+
+def clip_scenes():
+    """Triangles crossing the near plane and the clip frustum's sides."""
+    out = []
+    sc = S.cube_soup(width=64, height=48, triangles=200, seed=77, spread=2.0, cam_distance=0.3, name="near_crossing")
+    out.append(sc)
+    sc = S.cube_soup(width=80, height=40, triangles=150, seed=78, spread=3.0, cam_distance=1.0, width_slope=0.2,
+                     name="side_clipping")
+    out.append(sc)
+    sc = S.terrain(width=96, height=54, grid=24, texture_size=64, resolutions=7, cam_height=3.0, cam_back=-2.0,
+                   name="terrain_small")
+    out.append(sc)
+    return out
+
+
+def id_color(scene: S.Scene) -> S.Scene:
+    """Same geometry with one constant colour per polygon and no textures: selects the
+    "Single color" shader (RgbaMultiply.h:170-171), so the colour buffer is a triangle-id map
+    that does not depend on the CPU's reciprocal estimate."""
+    import copy
+    sc = copy.copy(scene)
+    m = scene.polygon_count
+    ids = np.arange(1, m + 1, dtype=np.int64)
+    cols = np.ones((m, 4, 4), dtype=np.float32)
+    cols[:, :, 0] = ((ids & 255) / 255.0).astype(np.float32)[:, None]
+    cols[:, :, 1] = (((ids >> 8) & 255) / 255.0).astype(np.float32)[:, None]
+    cols[:, :, 2] = (((ids >> 16) & 255) / 255.0).astype(np.float32)[:, None]
+    sc.colors = cols
+    sc.textures = []
+    sc.parts = [S.Part(p.polygon_start, p.polygon_end) for p in scene.parts]
+    sc.filter = 0
+    sc.name = scene.name + "_id"
+    return sc
+
+
+def fixture_scenes():
+    out = micro_scenes() + clip_scenes() + [variant(s) for s in range(40)]
+    out += [id_color(s) for s in clip_scenes()]
+    return out
