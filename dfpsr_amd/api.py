@@ -1,0 +1,252 @@
+"""Python mirror of the reference's rendererAPI / modelAPI / textureAPI for the polygon path,
+over the C ABI of libdfpsr_hip.so.  Names and argument meaning follow the reference
+(api/rendererAPI.h, api/modelAPI.h, api/textureAPI.h); errors raise DfpsrError where the
+reference calls throwError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from ._lib import Camera, DfpsrError, Image, check, lib
+
+Filter_Solid = 0
+Filter_Alpha = 1
+
+
+class Texture:
+    """TextureRgbaU8 resident on the device."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().dfpsr_hip_texture_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        vals = [C.c_int32() for _ in range(4)]
+        check(lib().dfpsr_hip_texture_info(self._h, *[C.byref(v) for v in vals]))
+        return tuple(v.value for v in vals)  # log2w, log2h, max_mip, texel_count
+
+    def read(self) -> np.ndarray:
+        n = self.info()[3]
+        out = np.zeros(n, dtype=np.uint32)
+        check(lib().dfpsr_hip_texture_read(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+def texture_create_RgbaU8(width: int, height: int, resolutions: int) -> Texture:
+    h = C.c_void_p()
+    check(lib().dfpsr_hip_texture_create(C.byref(h), width, height, resolutions))
+    return Texture(h)
+
+
+def texture_write_level0(texture: Texture, pixels: np.ndarray) -> None:
+    p = np.ascontiguousarray(pixels, dtype=np.uint32)
+    check(lib().dfpsr_hip_texture_write_level0(texture.handle, p.ctypes.data_as(C.c_void_p), p.shape[1]))
+
+
+def texture_generatePyramid(texture: Texture) -> None:
+    check(lib().dfpsr_hip_texture_generate_pyramid(texture.handle))
+
+
+class Model:
+    def __init__(self, handle):
+        self._h = handle
+        self._keep = []  # textures referenced by parts must outlive the model's frames
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().dfpsr_hip_model_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+
+def model_create(filter: int = Filter_Solid) -> Model:
+    h = C.c_void_p()
+    check(lib().dfpsr_hip_model_create(C.byref(h), filter))
+    return Model(h)
+
+
+def model_setPoints(model: Model, points: np.ndarray) -> None:
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    check(lib().dfpsr_hip_model_set_points(model.handle, p.shape[0], p.ctypes.data_as(C.c_void_p)))
+
+
+def model_addPart(model: Model, indices, colors, texcoords, diffuse: Optional[Texture] = None,
+                  light: Optional[Texture] = None) -> None:
+    ix = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, 4)
+    co = np.ascontiguousarray(colors, dtype=np.float32).reshape(-1, 16)
+    tc = np.ascontiguousarray(texcoords, dtype=np.float32).reshape(-1, 16)
+    check(lib().dfpsr_hip_model_add_part(model.handle, ix.shape[0], ix.ctypes.data_as(C.c_void_p),
+                                         co.ctypes.data_as(C.c_void_p), tc.ctypes.data_as(C.c_void_p),
+                                         diffuse.handle if diffuse else None, light.handle if light else None))
+    model._keep += [t for t in (diffuse, light) if t is not None]
+
+
+def model_getBoundingBox(model: Model):
+    mn = np.zeros(3, np.float32)
+    mx = np.zeros(3, np.float32)
+    check(lib().dfpsr_hip_model_bound(model.handle, mn.ctypes.data_as(C.c_void_p), mx.ctypes.data_as(C.c_void_p)))
+    return mn, mx
+
+
+def camera_create(perspective: bool, location, image_width: float, image_height: float, width_slope: float,
+                  near_clip: float = 0.01, far_clip: float = 1000.0) -> Camera:
+    """Camera::createPerspective (perspective=True) or Camera::createOrthogonal."""
+    c = Camera()
+    c.perspective = int(perspective)
+    c.location[:] = [float(v) for v in np.asarray(location, dtype=np.float32)]
+    c.image_width, c.image_height = float(image_width), float(image_height)
+    c.width_slope, c.near_clip, c.far_clip = float(width_slope), float(near_clip), float(far_clip)
+    return c
+
+
+class Renderer:
+    def __init__(self, handle):
+        self._h = handle
+        self._images = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().dfpsr_hip_renderer_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stats(self):
+        out = np.zeros(4, np.int64)
+        check(lib().dfpsr_hip_renderer_stats(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+def renderer_create() -> Renderer:
+    h = C.c_void_p()
+    check(lib().dfpsr_hip_renderer_create(C.byref(h)))
+    return Renderer(h)
+
+
+def host_image(array: Optional[np.ndarray]) -> Image:
+    """Canvas over a host numpy array (H, W) of uint32 or float32, copied at begin/end."""
+    im = Image()
+    if array is None:
+        return im
+    assert array.flags["C_CONTIGUOUS"] and array.itemsize == 4
+    im.pixels = array.ctypes.data
+    im.height, im.width = array.shape
+    im.stride_bytes = array.strides[0]
+    im.on_device = 0
+    return im
+
+
+def device_image(tensor) -> Image:
+    """Canvas over a device tensor (H, W) of 4-byte elements (torch, on the GPU), drawn in place."""
+    im = Image()
+    if tensor is None:
+        return im
+    im.pixels = tensor.data_ptr()
+    im.height, im.width = tensor.shape
+    im.stride_bytes = tensor.stride(0) * tensor.element_size()
+    im.on_device = 1
+    return im
+
+
+def renderer_begin(renderer: Renderer, color: Image, depth: Image) -> None:
+    renderer._images = (color, depth)
+    check(lib().dfpsr_hip_renderer_begin(renderer.handle, C.byref(color), C.byref(depth)))
+
+
+def renderer_giveTask(renderer: Renderer, model: Model, transform, camera: Camera) -> None:
+    t = np.ascontiguousarray(transform, dtype=np.float32)
+    check(lib().dfpsr_hip_renderer_give_model(renderer.handle, model.handle, t.ctypes.data_as(C.c_void_p), C.byref(camera)))
+
+
+def renderer_giveTask_triangles(renderer: Renderer, camera_space, colors, texcoords, diffuse: Optional[Texture],
+                                light: Optional[Texture], filter: int, camera: Camera) -> None:
+    cs = np.ascontiguousarray(camera_space, dtype=np.float32).reshape(-1, 9)
+    co = np.ascontiguousarray(colors, dtype=np.float32).reshape(-1, 12)
+    tc = np.ascontiguousarray(texcoords, dtype=np.float32).reshape(-1, 12)
+    check(lib().dfpsr_hip_renderer_give_triangles(renderer.handle, cs.shape[0], cs.ctypes.data_as(C.c_void_p),
+                                                  co.ctypes.data_as(C.c_void_p), tc.ctypes.data_as(C.c_void_p),
+                                                  diffuse.handle if diffuse else None, light.handle if light else None,
+                                                  filter, C.byref(camera)))
+
+
+def renderer_takesTriangles(renderer: Renderer) -> bool:
+    v = C.c_int32()
+    check(lib().dfpsr_hip_renderer_takes_triangles(renderer.handle, C.byref(v)))
+    return bool(v.value)
+
+
+def renderer_end(renderer: Renderer) -> None:
+    check(lib().dfpsr_hip_renderer_end(renderer.handle))
+
+
+# ---------------------------------------------------------------------------------------
+# Scenes (dfpsr_amd.scenes) on the GPU
+# ---------------------------------------------------------------------------------------
+
+class GpuScene:
+    """A scene's textures and model uploaded once, ready for frames."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        self.textures = []
+        for t in scene.textures:
+            tex = texture_create_RgbaU8(t.width, t.height, t.resolutions)
+            texture_write_level0(tex, _tile_level0(t))
+            texture_generatePyramid(tex)
+            self.textures.append(tex)
+        self.model = model_create(scene.filter)
+        model_setPoints(self.model, scene.points)
+        for p in scene.parts:
+            sl = slice(p.polygon_start, p.polygon_end)
+            model_addPart(self.model, scene.indices[sl], scene.colors[sl], scene.texcoords[sl],
+                          self.textures[p.diffuse] if p.diffuse >= 0 else None,
+                          self.textures[p.light] if p.light >= 0 else None)
+        self.camera = camera_create(scene.perspective, scene.cam_location, scene.width, scene.height, scene.width_slope,
+                                    scene.near_clip, scene.far_clip)
+
+    def give(self, renderer: Renderer) -> None:
+        for t in self.scene.instances:
+            renderer_giveTask(renderer, self.model, t, self.camera)
+
+
+def _tile_level0(t) -> np.ndarray:
+    """Level 0 at the texture's power-of-two size, repeating the given pixels (as the reference
+    driver fills texture_getSafePointer rows)."""
+    w = 1 << max(0, int(t.width - 1).bit_length())
+    h = 1 << max(0, int(t.height - 1).bit_length())
+    if (w, h) == (t.width, t.height):
+        return np.ascontiguousarray(t.level0, dtype=np.uint32)
+    ys = np.arange(h) % t.height
+    xs = np.arange(w) % t.width
+    return np.ascontiguousarray(t.level0[ys][:, xs], dtype=np.uint32)
+
+
+def render_scene(scene, renderer: Optional[Renderer] = None, gpu_scene: Optional[GpuScene] = None):
+    """One frame of `scene` on host canvases; returns (color, depth) as numpy arrays."""
+    r = renderer or renderer_create()
+    g = gpu_scene or GpuScene(scene)
+    color = np.zeros((scene.height, scene.width), np.uint32) if scene.initial_color is None else scene.initial_color.astype(np.uint32).copy()
+    depth = np.zeros((scene.height, scene.width), np.float32) if scene.initial_depth is None else scene.initial_depth.astype(np.float32).copy()
+    renderer_begin(r, host_image(color if scene.has_color else None), host_image(depth if scene.has_depth else None))
+    g.give(r)
+    renderer_end(r)
+    return color, depth
+
+
+__all__ = [n for n in dir() if not n.startswith("_")] + ["DfpsrError"]

@@ -1,0 +1,381 @@
+// C ABI of libdfpsr_hip.so: textures, models and renderers (include/dfpsr_hip.h).
+//
+// Host side of the polygon path.  renderer_giveTask only records tasks (and culls whole
+// models against the camera, as model_render_threaded does, modelAPI.cpp:227); all per-point,
+// per-triangle and per-pixel work runs in the kernels of kernels.hip at renderer_end.
+#include "../../include/dfpsr_hip.h"
+#include "frame.h"
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace dfpsr;
+
+namespace dfpsr {
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string &message) {
+	g_lastError = message;
+	return code;
+}
+
+hipError_t DevBuf::ensure(size_t need) {
+	if (need <= bytes && ptr != nullptr) return hipSuccess;
+	release();
+	size_t n = need < 256 ? 256 : need;
+	hipError_t e = hipMalloc(&ptr, n);
+	if (e == hipSuccess) bytes = n; else ptr = nullptr;
+	return e;
+}
+
+void DevBuf::release() {
+	if (ptr) (void)hipFree(ptr);
+	ptr = nullptr;
+	bytes = 0;
+}
+}  // namespace dfpsr
+
+// ---------------------------------------------------------------------------------------
+// Textures
+// ---------------------------------------------------------------------------------------
+
+struct dfpsr_hip_texture {
+	TextureDesc desc;  // desc.texels is the device pyramid
+	DevBuf buffer;
+};
+
+static uint32_t log2Ceil(uint32_t size) {  // findLog2Size (textureAPI.cpp:31-41)
+	for (uint32_t l = 0; l < 15; l++)
+		if ((1u << l) >= size) return l;
+	return 15;
+}
+
+extern "C" const char *dfpsr_hip_last_error(void) { return g_lastError.c_str(); }
+
+extern "C" const char *dfpsr_hip_version(void) { return "dfpsr_hip 0.1 gfx950"; }
+
+extern "C" int dfpsr_hip_texture_create(dfpsr_hip_texture **out, int32_t width, int32_t height, int32_t resolutions) {
+	if (!out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "texture_create: null output");
+	*out = nullptr;
+	if (resolutions < 1) return fail(DFPSR_HIP_ERROR_ARGUMENT, "texture_create: a texture needs at least one resolution");
+	if (width < 1 || height < 1 || width > 32768 || height > 32768)
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, "texture_create: dimensions must be 1..32768");
+	std::unique_ptr<dfpsr_hip_texture> t(new (std::nothrow) dfpsr_hip_texture());
+	if (!t) return fail(DFPSR_HIP_ERROR_MEMORY, "texture_create: out of host memory");
+	t->desc = textureLayout(log2Ceil((uint32_t)width), log2Ceil((uint32_t)height), (uint32_t)(resolutions - 1));
+	size_t bytes = (size_t)t->desc.texelCount * 4;
+	if (t->buffer.ensure(bytes) != hipSuccess) return fail(DFPSR_HIP_ERROR_MEMORY, "texture_create: device allocation failed");
+	HIP_TRY(hipMemset(t->buffer.ptr, 0, bytes));
+	t->desc.texels = (const uint32_t *)t->buffer.ptr;
+	*out = t.release();
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_texture_write_level0(dfpsr_hip_texture *t, const uint32_t *pixels, int32_t stride) {
+	if (!t || !pixels) return fail(DFPSR_HIP_ERROR_ARGUMENT, "texture_write_level0: null argument");
+	uint32_t w = 1u << t->desc.log2w, h = 1u << t->desc.log2h;
+	if (stride < (int32_t)w) return fail(DFPSR_HIP_ERROR_ARGUMENT, "texture_write_level0: stride below the width");
+	uint32_t *dst = (uint32_t *)t->buffer.ptr + t->desc.startOffset;
+	HIP_TRY(hipMemcpy2D(dst, w * 4, pixels, (size_t)stride * 4, w * 4, h, hipMemcpyHostToDevice));
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_texture_generate_pyramid(dfpsr_hip_texture *t) {
+	if (!t) return fail(DFPSR_HIP_ERROR_ARGUMENT, "texture_generate_pyramid: null texture");
+	for (uint32_t level = 1; level <= t->desc.maxMip; level++) {
+		uint32_t n = (1u << (t->desc.log2w - level)) * (1u << (t->desc.log2h - level));
+		hipLaunchKernelGGL(k_downsample, dim3((n + 255) / 256), dim3(256), 0, 0, (uint32_t *)t->buffer.ptr, t->desc, level);
+		HIP_TRY(hipGetLastError());
+	}
+	HIP_TRY(hipDeviceSynchronize());
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_texture_info(const dfpsr_hip_texture *t, int32_t *log2w, int32_t *log2h, int32_t *maxMip, int32_t *texels) {
+	if (!t) return fail(DFPSR_HIP_ERROR_ARGUMENT, "texture_info: null texture");
+	if (log2w) *log2w = (int32_t)t->desc.log2w;
+	if (log2h) *log2h = (int32_t)t->desc.log2h;
+	if (maxMip) *maxMip = (int32_t)t->desc.maxMip;
+	if (texels) *texels = (int32_t)t->desc.texelCount;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_texture_read(const dfpsr_hip_texture *t, uint32_t *out) {
+	if (!t || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "texture_read: null argument");
+	HIP_TRY(hipMemcpy(out, t->buffer.ptr, (size_t)t->desc.texelCount * 4, hipMemcpyDeviceToHost));
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_texture_device_pointer(const dfpsr_hip_texture *t, uint32_t **out) {
+	if (!t || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "texture_device_pointer: null argument");
+	*out = (uint32_t *)t->buffer.ptr;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" void dfpsr_hip_texture_destroy(dfpsr_hip_texture *t) { delete t; }
+
+// ---------------------------------------------------------------------------------------
+// Models
+// ---------------------------------------------------------------------------------------
+
+struct dfpsr_hip_model {
+	int32_t filter = 0;
+	std::vector<float> points;
+	f3 minBound = make3(0.0f, 0.0f, 0.0f), maxBound = make3(0.0f, 0.0f, 0.0f);
+	std::vector<int32_t> indices;
+	std::vector<float> colors, texcoords;
+	std::vector<int32_t> fanPolygon, polygonPart;
+	std::vector<const TextureDesc *> partDiffuse, partLight;
+	DevBuf dPoints, dIndices, dColors, dTexcoords, dFanPolygon, dPolygonPart;
+	bool dirty = true;
+
+	ModelView view() const {
+		ModelView v;
+		v.points = dPoints.as<float>();
+		v.indices = dIndices.as<int32_t>();
+		v.colors = dColors.as<float>();
+		v.texcoords = dTexcoords.as<float>();
+		v.fanPolygon = dFanPolygon.as<int32_t>();
+		v.polygonPart = dPolygonPart.as<int32_t>();
+		v.pointCount = (int32_t)(points.size() / 3);
+		v.fanCount = (int32_t)fanPolygon.size();
+		v.partCount = (int32_t)partDiffuse.size();
+		v.filter = filter;
+		v.partDiffuse = partDiffuse.data();
+		v.partLight = partLight.data();
+		return v;
+	}
+
+	int upload() {
+		if (!dirty) return DFPSR_HIP_OK;
+		struct { DevBuf *buf; const void *src; size_t bytes; } items[] = {
+			{&dPoints, points.data(), points.size() * 4}, {&dIndices, indices.data(), indices.size() * 4},
+			{&dColors, colors.data(), colors.size() * 4}, {&dTexcoords, texcoords.data(), texcoords.size() * 4},
+			{&dFanPolygon, fanPolygon.data(), fanPolygon.size() * 4}, {&dPolygonPart, polygonPart.data(), polygonPart.size() * 4}};
+		for (auto &it : items) {
+			if (it.buf->ensure(it.bytes) != hipSuccess) return fail(DFPSR_HIP_ERROR_MEMORY, "model upload: device allocation failed");
+			if (it.bytes) HIP_TRY(hipMemcpy(it.buf->ptr, it.src, it.bytes, hipMemcpyHostToDevice));
+		}
+		dirty = false;
+		return DFPSR_HIP_OK;
+	}
+};
+
+extern "C" int dfpsr_hip_model_create(dfpsr_hip_model **out, int32_t filter) {
+	if (!out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_create: null output");
+	if (filter != DFPSR_HIP_FILTER_SOLID && filter != DFPSR_HIP_FILTER_ALPHA) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_create: unknown filter");
+	*out = new (std::nothrow) dfpsr_hip_model();
+	if (!*out) return fail(DFPSR_HIP_ERROR_MEMORY, "model_create: out of host memory");
+	(*out)->filter = filter;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_model_set_points(dfpsr_hip_model *m, int32_t count, const float *xyz) {
+	if (!m || count < 0 || (count > 0 && !xyz)) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_set_points: bad argument");
+	m->points.assign(xyz, xyz + (size_t)count * 3);
+	// ModelImpl::expandBound from a bound that starts at the origin (Model.cpp:281-311).
+	f3 mn = make3(0.0f, 0.0f, 0.0f), mx = make3(0.0f, 0.0f, 0.0f);
+	for (int32_t i = 0; i < count; i++) {
+		const float *p = xyz + (size_t)i * 3;
+		if (mn.x > p[0]) mn.x = p[0];
+		if (mn.y > p[1]) mn.y = p[1];
+		if (mn.z > p[2]) mn.z = p[2];
+		if (mx.x < p[0]) mx.x = p[0];
+		if (mx.y < p[1]) mx.y = p[1];
+		if (mx.z < p[2]) mx.z = p[2];
+	}
+	m->minBound = mn;
+	m->maxBound = mx;
+	m->dirty = true;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_model_add_part(dfpsr_hip_model *m, int32_t count, const int32_t *indices, const float *colors,
+                                        const float *texcoords, const dfpsr_hip_texture *diffuse, const dfpsr_hip_texture *light) {
+	if (!m || count < 0 || (count > 0 && (!indices || !colors || !texcoords)))
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_add_part: bad argument");
+	int32_t points = (int32_t)(m->points.size() / 3);
+	for (int32_t i = 0; i < count; i++) {
+		const int32_t *ix = indices + (size_t)i * 4;
+		int32_t corners = ix[3] < 0 ? 3 : 4;
+		for (int32_t c = 0; c < corners; c++)
+			if (ix[c] < 0 || ix[c] >= points) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_add_part: point index out of range");
+	}
+	int32_t part = (int32_t)m->partDiffuse.size();
+	int32_t polyBase = (int32_t)(m->indices.size() / 4);
+	m->indices.insert(m->indices.end(), indices, indices + (size_t)count * 4);
+	m->colors.insert(m->colors.end(), colors, colors + (size_t)count * 16);
+	m->texcoords.insert(m->texcoords.end(), texcoords, texcoords + (size_t)count * 16);
+	for (int32_t i = 0; i < count; i++) {
+		int32_t poly = polyBase + i;
+		m->polygonPart.push_back(part);
+		m->fanPolygon.push_back(poly * 2);
+		if (indices[(size_t)i * 4 + 3] >= 0) m->fanPolygon.push_back(poly * 2 + 1);
+	}
+	m->partDiffuse.push_back(diffuse ? &diffuse->desc : nullptr);
+	m->partLight.push_back(light ? &light->desc : nullptr);
+	m->dirty = true;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_model_bound(const dfpsr_hip_model *m, float mn[3], float mx[3]) {
+	if (!m || !mn || !mx) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_bound: null argument");
+	mn[0] = m->minBound.x; mn[1] = m->minBound.y; mn[2] = m->minBound.z;
+	mx[0] = m->maxBound.x; mx[1] = m->maxBound.y; mx[2] = m->maxBound.z;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" void dfpsr_hip_model_destroy(dfpsr_hip_model *m) { delete m; }
+
+// ---------------------------------------------------------------------------------------
+// Renderer
+// ---------------------------------------------------------------------------------------
+
+struct dfpsr_hip_renderer {
+	Frame frame;
+	bool receiving = false;
+	dfpsr_hip_image color{}, depth{};
+	int32_t width = 0, height = 0;
+	DevBuf colorStage, depthStage;
+	std::vector<std::unique_ptr<dfpsr_hip_model>> triangleModels;  // giveTask_triangle batches of this frame
+};
+
+static CameraParams cameraFrom(const dfpsr_hip_camera *c) {
+	return deriveCamera(c->perspective, c->location, c->image_width, c->image_height, c->width_slope, c->near_clip, c->far_clip);
+}
+
+static int checkCamera(const dfpsr_hip_camera *c) {
+	if (!c) return fail(DFPSR_HIP_ERROR_ARGUMENT, "null camera");
+	if (!(c->image_width > 0.0f) || !(c->image_height > 0.0f) || c->width_slope == 0.0f)
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, "camera needs a positive image size and a non-zero slope");
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_create(dfpsr_hip_renderer **out) {
+	if (!out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_create: null output");
+	std::unique_ptr<dfpsr_hip_renderer> r(new (std::nothrow) dfpsr_hip_renderer());
+	if (!r) return fail(DFPSR_HIP_ERROR_MEMORY, "renderer_create: out of host memory");
+	int rc = r->frame.init();
+	if (rc != DFPSR_HIP_OK) return rc;
+	*out = r.release();
+	return DFPSR_HIP_OK;
+}
+
+extern "C" void dfpsr_hip_renderer_destroy(dfpsr_hip_renderer *r) { delete r; }
+
+static bool hasImage(const dfpsr_hip_image *im) { return im && im->pixels; }
+
+extern "C" int dfpsr_hip_renderer_begin(dfpsr_hip_renderer *r, const dfpsr_hip_image *color, const dfpsr_hip_image *depth) {
+	if (!r) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_begin: null renderer");
+	if (r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "Called renderer_begin on the same renderer twice without ending the previous batch!");
+	bool hc = hasImage(color), hd = hasImage(depth);
+	for (const dfpsr_hip_image *im : {color, depth}) {
+		if (hasImage(im) && (im->width < 1 || im->height < 1 || im->stride_bytes < im->width * 4 || (im->stride_bytes & 3)))
+			return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_begin: bad canvas size or stride");
+	}
+	if (hc && hd && (color->width != depth->width || color->height != depth->height))
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_begin: colour and depth canvases differ in size");
+	r->color = hc ? *color : dfpsr_hip_image{};
+	r->depth = hd ? *depth : dfpsr_hip_image{};
+	r->width = hc ? color->width : (hd ? depth->width : 0);
+	r->height = hc ? color->height : (hd ? depth->height : 0);
+	// Host canvases are staged on the device for the frame.
+	if (hc && !color->on_device) {
+		size_t bytes = (size_t)color->stride_bytes * color->height;
+		if (r->colorStage.ensure(bytes) != hipSuccess) return fail(DFPSR_HIP_ERROR_MEMORY, "renderer_begin: device allocation failed");
+		HIP_TRY(hipMemcpy(r->colorStage.ptr, color->pixels, bytes, hipMemcpyHostToDevice));
+	}
+	if (hd && !depth->on_device) {
+		size_t bytes = (size_t)depth->stride_bytes * depth->height;
+		if (r->depthStage.ensure(bytes) != hipSuccess) return fail(DFPSR_HIP_ERROR_MEMORY, "renderer_begin: device allocation failed");
+		HIP_TRY(hipMemcpy(r->depthStage.ptr, depth->pixels, bytes, hipMemcpyHostToDevice));
+	}
+	CanvasHost cv;
+	cv.color = hc ? (uint32_t *)(color->on_device ? color->pixels : r->colorStage.ptr) : nullptr;
+	cv.depth = hd ? (float *)(depth->on_device ? depth->pixels : r->depthStage.ptr) : nullptr;
+	cv.width = r->width;
+	cv.height = r->height;
+	cv.colorStride = hc ? color->stride_bytes / 4 : 0;
+	cv.depthStride = hd ? depth->stride_bytes / 4 : 0;
+	r->frame.reset();
+	r->frame.addCanvas(cv);
+	r->triangleModels.clear();
+	r->receiving = true;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_give_model(dfpsr_hip_renderer *r, const dfpsr_hip_model *m, const float transform[12],
+                                             const dfpsr_hip_camera *camera) {
+	if (!r || !m || !transform) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_give_model: null argument");
+	if (!r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "Cannot call renderer_giveTask before renderer_begin!");
+	int rc = checkCamera(camera);
+	if (rc) return rc;
+	CameraParams cam = cameraFrom(camera);
+	// Camera::isBoxSeen on the model bound (modelAPI.cpp:227).
+	if (boxSeen(cam, m->minBound, m->maxBound, transformFrom(transform)) == 0) return DFPSR_HIP_OK;
+	rc = const_cast<dfpsr_hip_model *>(m)->upload();
+	if (rc) return rc;
+	return r->frame.addTask(m->view(), transform, cam, 0, false);
+}
+
+extern "C" int dfpsr_hip_renderer_give_triangles(dfpsr_hip_renderer *r, int32_t count, const float *cs, const float *colors,
+                                                 const float *texcoords, const dfpsr_hip_texture *diffuse,
+                                                 const dfpsr_hip_texture *light, int32_t filter, const dfpsr_hip_camera *camera) {
+	if (!r || count < 0 || (count > 0 && (!cs || !colors || !texcoords)))
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_give_triangles: bad argument");
+	if (!r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "Cannot call renderer_giveTask_triangle before renderer_begin!");
+	if (filter != DFPSR_HIP_FILTER_SOLID && filter != DFPSR_HIP_FILTER_ALPHA) return fail(DFPSR_HIP_ERROR_ARGUMENT, "unknown filter");
+	int rc = checkCamera(camera);
+	if (rc) return rc;
+	if (count == 0) return DFPSR_HIP_OK;
+	std::unique_ptr<dfpsr_hip_model> m(new (std::nothrow) dfpsr_hip_model());
+	if (!m) return fail(DFPSR_HIP_ERROR_MEMORY, "renderer_give_triangles: out of host memory");
+	m->filter = filter;
+	m->points.assign(cs, cs + (size_t)count * 9);
+	std::vector<int32_t> idx((size_t)count * 4);
+	std::vector<float> col((size_t)count * 16, 0.0f), tex((size_t)count * 16, 0.0f);
+	for (int32_t t = 0; t < count; t++) {
+		idx[(size_t)t * 4 + 0] = t * 3; idx[(size_t)t * 4 + 1] = t * 3 + 1; idx[(size_t)t * 4 + 2] = t * 3 + 2; idx[(size_t)t * 4 + 3] = -1;
+		std::memcpy(&col[(size_t)t * 16], colors + (size_t)t * 12, 12 * sizeof(float));
+		std::memcpy(&tex[(size_t)t * 16], texcoords + (size_t)t * 12, 12 * sizeof(float));
+	}
+	rc = dfpsr_hip_model_add_part(m.get(), count, idx.data(), col.data(), tex.data(), diffuse, light);
+	if (rc) return rc;
+	rc = m->upload();
+	if (rc) return rc;
+	static const float identity[12] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+	rc = r->frame.addTask(m->view(), identity, cameraFrom(camera), 0, true);
+	if (rc) return rc;
+	r->triangleModels.push_back(std::move(m));
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_takes_triangles(const dfpsr_hip_renderer *r, int32_t *out) {
+	if (!r || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_takes_triangles: null argument");
+	*out = r->receiving ? 1 : 0;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_end(dfpsr_hip_renderer *r) {
+	if (!r) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_end: null renderer");
+	if (!r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "Called renderer_end without renderer_begin!");
+	r->receiving = false;
+	int rc = r->frame.render();
+	if (rc) return rc;
+	if (hasImage(&r->color) && !r->color.on_device)
+		HIP_TRY(hipMemcpy(r->color.pixels, r->colorStage.ptr, (size_t)r->color.stride_bytes * r->color.height, hipMemcpyDeviceToHost));
+	if (hasImage(&r->depth) && !r->depth.on_device)
+		HIP_TRY(hipMemcpy(r->depth.pixels, r->depthStage.ptr, (size_t)r->depth.stride_bytes * r->depth.height, hipMemcpyDeviceToHost));
+	r->triangleModels.clear();
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_stats(const dfpsr_hip_renderer *r, int64_t out[4]) {
+	if (!r || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_stats: null argument");
+	for (int i = 0; i < 4; i++) out[i] = r->frame.stats[i];
+	return DFPSR_HIP_OK;
+}

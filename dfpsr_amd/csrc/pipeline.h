@@ -1,0 +1,101 @@
+// Device-side data of one frame of the polygon path and the kernel entry points.
+#pragma once
+
+#include "shading.h"
+
+namespace dfpsr {
+
+// Screen tiles: 16x16 pixels, one 64-lane wavefront per tile and one lane per 2x2 quad.
+// Both sides are even, so a row pair and a quad never straddle two tiles.
+constexpr int32_t TILE_W = 16;
+constexpr int32_t TILE_H = 16;
+constexpr int32_t TILE_LANES = (TILE_W / 2) * (TILE_H / 2);
+static_assert(TILE_LANES == 64, "one wavefront per tile");
+// Commands a tile sorts in LDS at once; longer lists are processed in ordered rounds.
+constexpr int32_t TILE_SORT_CAP = 1024;
+
+// One renderer_giveTask(model, transform, camera) call, as the kernels see it.
+struct TaskDev {
+	const float *points;       // pointCount x 3
+	const int32_t *indices;    // polygons x 4
+	const float *colors;       // polygons x 16
+	const float *texcoords;    // polygons x 16
+	const int32_t *fanPolygon; // fan triangle -> polygon * 2 + k
+	const int32_t *polygonPart;
+	const int32_t *partTexture;  // frame texture slot per part: diffuse, light (-1 = none)
+	int32_t pointCount, fanCount;
+	int32_t pointBase, fanBase;  // offsets into the frame's projected points / fan numbering
+	int32_t filter;              // 0 Solid, 1 Alpha
+	int32_t canvas;              // canvas slot (the batched form renders many canvases)
+	int32_t cameraSpacePoints;   // 1: points are already in camera space (renderer_giveTask_triangle)
+	float transform[12];
+	CameraParams camera;
+};
+
+// A colour and depth canvas pair.  Either pointer may be null.
+struct CanvasDev {
+	uint32_t *color;
+	float *depth;
+	int32_t width, height;
+	int32_t colorStride, depthStride;  // in elements
+	int32_t tileBase;                  // first tile of this canvas in the frame's tile numbering
+	int32_t tilesX, tilesY;
+};
+
+// One draw command: a (sub-)triangle that passed culling, clipping and the front-face test.
+struct Command {
+	int64_t fx[3], fy[3];  // corners in sub-pixels (256 per pixel)
+	f3 start, dx, dy;      // weight planes
+	int32_t task, fan;     // source: task and fan triangle within the task's model
+	int32_t canvas, flags; // canvas slot; bit 0 = affine (orthogonal camera)
+};
+
+struct FrameDev {
+	const TaskDev *tasks;
+	int32_t taskCount;
+	const int32_t *taskFanEnd;    // inclusive prefix of fan triangles per task
+	const int32_t *taskPointEnd;  // inclusive prefix of points per task
+	int32_t fanTotal, pointTotal;
+	const CanvasDev *canvases;
+	int32_t canvasCount;
+	const int32_t *canvasTileEnd; // inclusive prefix of tiles per canvas
+	int32_t tileTotal;
+	const TextureDesc *textures;
+	Projected *projected;         // pointTotal
+	int32_t *fanCount;            // fanTotal: commands made by each fan triangle
+	int32_t *blockSums;           // scan scratch
+	Command *commands;
+	int32_t commandCapacity;
+	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flag
+	int32_t *tileCount;           // tileTotal + 1
+	int32_t *tileOffset;          // tileTotal + 1 (exclusive scan of tileCount)
+	int32_t *tileFill;            // tileTotal
+	int32_t *tileList;            // pair capacity
+	int32_t pairCapacity;
+	int32_t alphaAny;             // any task uses the Alpha filter: tiles read colour first
+};
+
+DH int32_t findSegment(const int32_t *inclusiveEnd, int32_t count, int32_t index) {
+	int32_t lo = 0, hi = count - 1;
+	while (lo < hi) {
+		int32_t mid = (lo + hi) >> 1;
+		if (inclusiveEnd[mid] > index) hi = mid; else lo = mid + 1;
+	}
+	return lo;
+}
+
+// Kernel entry points (kernels.hip).
+__global__ void k_project(FrameDev f);
+__global__ void k_setup_count(FrameDev f);
+__global__ void k_scan_block_sums(int32_t *blockSums, int32_t blocks, int32_t *total);
+__global__ void k_setup_emit(FrameDev f);
+__global__ void k_bin_count(FrameDev f);
+__global__ void k_scan_tiles_local(FrameDev f);
+__global__ void k_scan_tiles_add(FrameDev f);
+__global__ void k_bin_fill(FrameDev f);
+__global__ void k_raster(FrameDev f);
+__global__ void k_downsample(uint32_t *texels, TextureDesc t, uint32_t targetLevel);
+
+constexpr int32_t SCAN_BLOCK = 256;
+
+}  // namespace dfpsr

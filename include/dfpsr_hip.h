@@ -1,0 +1,144 @@
+/*
+ * dfpsr_hip.h -- C ABI of the MI355X (gfx950) polygon renderer, libdfpsr_hip.so.
+ *
+ * A drop-in for the reference renderer's polygon path: renderer_create / renderer_begin /
+ * renderer_giveTask / renderer_end over modelAPI models, drawing into an OrderedImageRgbaU8
+ * colour canvas and an AlignedImageF32 depth canvas.  Each entry point below names the
+ * reference interface it replaces (paths relative to the reference's Source/ folder).
+ *
+ * Conventions
+ *   - Every function returns DFPSR_HIP_OK (0) or a negative error code; no C++ exception
+ *     crosses the ABI.  dfpsr_hip_last_error() returns a thread-local message for the last
+ *     failure on the calling thread.
+ *   - Plain pointers and sizes only.  Arrays are tightly packed, little-endian.
+ *   - A renderer is used from one host thread at a time, like the reference's Renderer.
+ *   - Colours are packed RGBA8 (red in the lowest byte), the OrderedImageRgbaU8 pack order.
+ *   - Arithmetic follows the reference operation for operation: coverage and depth are
+ *     bit-exact; RGBA8 colours may differ by 1 LSB where the reference's SSE reciprocal
+ *     estimate (CPU-vendor specific) differs from the GPU's.
+ */
+#ifndef DFPSR_HIP_H
+#define DFPSR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	DFPSR_HIP_OK = 0,
+	DFPSR_HIP_ERROR_ARGUMENT = -1, /* null handle, bad size or index                      */
+	DFPSR_HIP_ERROR_STATE = -2,    /* call out of order (begin twice, end without begin)   */
+	DFPSR_HIP_ERROR_DEVICE = -3,   /* a HIP runtime call failed                            */
+	DFPSR_HIP_ERROR_MEMORY = -4    /* device or host allocation failed                     */
+};
+
+/* Filters (implementation/render/constants.h:33). */
+enum { DFPSR_HIP_FILTER_SOLID = 0, DFPSR_HIP_FILTER_ALPHA = 1 };
+
+typedef struct dfpsr_hip_renderer dfpsr_hip_renderer;
+typedef struct dfpsr_hip_model dfpsr_hip_model;
+typedef struct dfpsr_hip_texture dfpsr_hip_texture;
+
+/* A canvas: row 0 at `pixels`, rows `stride_bytes` apart.  pixels == NULL means no canvas
+ * (the reference's empty image handle).  on_device = 1: a device pointer that stays where it
+ * is (the renderer draws in place); 0: host memory, copied to the device at begin and back
+ * at end. */
+typedef struct {
+	void *pixels;
+	int32_t width, height;
+	int32_t stride_bytes;
+	int32_t on_device;
+} dfpsr_hip_image;
+
+/* Camera::createPerspective / Camera::createOrthogonal parameters (implementation/render/
+ * Camera.h:144-156).  location = Transform3D: position xyz, then xAxis, yAxis, zAxis.
+ * For an orthogonal camera width_slope is the half width and near/far are ignored. */
+typedef struct {
+	int32_t perspective;
+	float location[12];
+	float image_width, image_height;
+	float width_slope, near_clip, far_clip;
+} dfpsr_hip_camera;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *dfpsr_hip_last_error(void);
+
+/* Library version string. */
+const char *dfpsr_hip_version(void);
+
+/* ---- Textures (api/textureAPI.h) -------------------------------------------------------- */
+
+/* texture_create_RgbaU8(width, height, resolutions) (api/textureAPI.h:456, textureAPI.cpp:65-78):
+ * a pyramid of power-of-two levels, smallest first, resident on the device. */
+int dfpsr_hip_texture_create(dfpsr_hip_texture **out, int32_t width, int32_t height, int32_t resolutions);
+/* Writes the highest resolution, as texture_getSafePointer(texture, 0, y) row writes do
+ * (api/textureAPI.h:514-517).  pixels: (1 << log2h) rows of (1 << log2w) texels, row stride
+ * in texels. */
+int dfpsr_hip_texture_write_level0(dfpsr_hip_texture *texture, const uint32_t *pixels, int32_t stride_texels);
+/* texture_generatePyramid (api/textureAPI.h:70, textureAPI.cpp:82-87), run on the GPU. */
+int dfpsr_hip_texture_generate_pyramid(dfpsr_hip_texture *texture);
+/* Layout: log2 width/height, smallest mip level index and the pyramid's texel count. */
+int dfpsr_hip_texture_info(const dfpsr_hip_texture *texture, int32_t *log2w, int32_t *log2h, int32_t *max_mip,
+                           int32_t *texel_count);
+/* Copies the whole pyramid buffer (texel_count texels, smallest level first) to host memory. */
+int dfpsr_hip_texture_read(const dfpsr_hip_texture *texture, uint32_t *out);
+/* Device address of the pyramid buffer (for callers that keep data resident). */
+int dfpsr_hip_texture_device_pointer(const dfpsr_hip_texture *texture, uint32_t **out);
+void dfpsr_hip_texture_destroy(dfpsr_hip_texture *texture);
+
+/* ---- Models (api/modelAPI.h) ------------------------------------------------------------ */
+
+/* model_create + model_setFilter (api/modelAPI.h:62, :76). */
+int dfpsr_hip_model_create(dfpsr_hip_model **out, int32_t filter);
+/* model_addPoint for `count` points (api/modelAPI.h:124); replaces any previous points.
+ * The model's bound starts at the origin and grows with each point (Model.cpp:281-311). */
+int dfpsr_hip_model_set_points(dfpsr_hip_model *model, int32_t count, const float *xyz);
+/* model_addEmptyPart + model_addTriangle/model_addQuad + model_setVertexColor +
+ * model_setTexCoord + model_setDiffuseMap/model_setLightMap (api/modelAPI.h:93-258) for one
+ * part: indices = count x 4 point indices (index 3 = -1 for a triangle), colors = count x 4
+ * corners x RGBA, texcoords = count x 4 corners x (u1, v1, u2, v2).  Textures may be NULL
+ * and must outlive the renderer_end of every frame that draws the model (as in the
+ * reference, rendererAPI.h:104-107). */
+int dfpsr_hip_model_add_part(dfpsr_hip_model *model, int32_t count, const int32_t *indices, const float *colors,
+                             const float *texcoords, const dfpsr_hip_texture *diffuse, const dfpsr_hip_texture *light);
+/* model_getBoundingBox (api/modelAPI.h:138). */
+int dfpsr_hip_model_bound(const dfpsr_hip_model *model, float minimum[3], float maximum[3]);
+void dfpsr_hip_model_destroy(dfpsr_hip_model *model);
+
+/* ---- Renderer (api/rendererAPI.h) ------------------------------------------------------- */
+
+/* renderer_create (api/rendererAPI.h:56, rendererAPI.cpp:490-492). */
+int dfpsr_hip_renderer_create(dfpsr_hip_renderer **out);
+void dfpsr_hip_renderer_destroy(dfpsr_hip_renderer *renderer);
+/* renderer_begin (api/rendererAPI.h:65, RendererImpl::beginFrame rendererAPI.cpp:151-168).
+ * Either canvas may be absent; when both exist they must have the same size.  Fails with
+ * DFPSR_HIP_ERROR_STATE when called twice without renderer_end. */
+int dfpsr_hip_renderer_begin(dfpsr_hip_renderer *renderer, const dfpsr_hip_image *color, const dfpsr_hip_image *depth);
+/* renderer_giveTask(renderer, model, modelToWorld, camera) = model_render_threaded
+ * (api/modelAPI.h:299-301, modelAPI.cpp:214-281): box culling, projection of every point and
+ * the quad fan (0,1,2),(0,2,3).  transform = Transform3D as in dfpsr_hip_camera.location. */
+int dfpsr_hip_renderer_give_model(dfpsr_hip_renderer *renderer, const dfpsr_hip_model *model, const float transform[12],
+                                  const dfpsr_hip_camera *camera);
+/* renderer_giveTask_triangle for `count` triangles (api/rendererAPI.h:123-128,
+ * rendererAPI.cpp:503-519).  Corners are given in camera space (count x 3 x xyz); the device
+ * projects them with Camera::cameraToScreen, which is what ProjectedPoint holds.
+ * colors / texcoords: count x 3 corners x 4 floats. */
+int dfpsr_hip_renderer_give_triangles(dfpsr_hip_renderer *renderer, int32_t count, const float *camera_space,
+                                      const float *colors, const float *texcoords, const dfpsr_hip_texture *diffuse,
+                                      const dfpsr_hip_texture *light, int32_t filter, const dfpsr_hip_camera *camera);
+/* renderer_takesTriangles (api/rendererAPI.h:73). */
+int dfpsr_hip_renderer_takes_triangles(const dfpsr_hip_renderer *renderer, int32_t *out);
+/* renderer_end (api/rendererAPI.h:135, RendererImpl::endFrame rendererAPI.cpp:352-402):
+ * draws every given triangle and returns when the canvases hold the result (host canvases
+ * are copied back).  Fails with DFPSR_HIP_ERROR_STATE without renderer_begin. */
+int dfpsr_hip_renderer_end(dfpsr_hip_renderer *renderer);
+/* Statistics of the last renderer_end: draw commands, (tile, command) pairs, tiles. */
+int dfpsr_hip_renderer_stats(const dfpsr_hip_renderer *renderer, int64_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -11,7 +11,7 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 # numpy work; its static heap set-up crashes when dlopen'ed later in the process.
 try:
     from oracle import harness as _h
-    if _h.ref_available():
+    if _h.ref_available() and os.path.isdir("/root/reference/Source"):
         _h.load_ref()
 except Exception:  # the reference build is optional
     pass

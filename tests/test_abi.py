@@ -1,0 +1,67 @@
+"""CPU-side checks of the C ABI: the library loads, exports every function include/dfpsr_hip.h
+declares, and validates arguments that never reach the device."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from dfpsr_amd import _lib
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return _lib.lib()
+
+
+def test_exports_every_declared_symbol(lib):
+    names = _lib.declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version(lib):
+    assert b"gfx950" in lib.dfpsr_hip_version()
+
+
+def test_model_argument_checks(lib):
+    m = C.c_void_p()
+    assert lib.dfpsr_hip_model_create(C.byref(m), 7) == _lib.ERROR_ARGUMENT
+    assert b"filter" in lib.dfpsr_hip_last_error()
+    assert lib.dfpsr_hip_model_create(C.byref(m), 0) == 0
+    pts = np.zeros((3, 3), np.float32)
+    assert lib.dfpsr_hip_model_set_points(m, 3, pts.ctypes.data_as(C.c_void_p)) == 0
+    idx = np.array([[0, 1, 5, -1]], np.int32)
+    col = np.ones((1, 16), np.float32)
+    tex = np.zeros((1, 16), np.float32)
+    rc = lib.dfpsr_hip_model_add_part(m, 1, idx.ctypes.data_as(C.c_void_p), col.ctypes.data_as(C.c_void_p),
+                                      tex.ctypes.data_as(C.c_void_p), None, None)
+    assert rc == _lib.ERROR_ARGUMENT and b"out of range" in lib.dfpsr_hip_last_error()
+    idx[0, 2] = 2
+    rc = lib.dfpsr_hip_model_add_part(m, 1, idx.ctypes.data_as(C.c_void_p), col.ctypes.data_as(C.c_void_p),
+                                      tex.ctypes.data_as(C.c_void_p), None, None)
+    assert rc == 0
+    lib.dfpsr_hip_model_destroy(m)
+
+
+def test_model_bound_includes_origin(lib):
+    """ModelImpl's bound starts at the origin (Model.cpp:281-311)."""
+    m = C.c_void_p()
+    lib.dfpsr_hip_model_create(C.byref(m), 0)
+    pts = np.array([[1, 2, 3], [4, 5, 6]], np.float32)
+    lib.dfpsr_hip_model_set_points(m, 2, pts.ctypes.data_as(C.c_void_p))
+    mn = np.zeros(3, np.float32)
+    mx = np.zeros(3, np.float32)
+    assert lib.dfpsr_hip_model_bound(m, mn.ctypes.data_as(C.c_void_p), mx.ctypes.data_as(C.c_void_p)) == 0
+    assert mn.tolist() == [0, 0, 0] and mx.tolist() == [4, 5, 6]
+    lib.dfpsr_hip_model_destroy(m)
+
+
+def test_null_handles(lib):
+    assert lib.dfpsr_hip_renderer_end(None) == _lib.ERROR_ARGUMENT
+    assert lib.dfpsr_hip_renderer_begin(None, None, None) == _lib.ERROR_ARGUMENT
+    assert lib.dfpsr_hip_texture_generate_pyramid(None) == _lib.ERROR_ARGUMENT
+    assert lib.dfpsr_hip_texture_create(None, 4, 4, 1) == _lib.ERROR_ARGUMENT
+    t = C.c_void_p()
+    assert lib.dfpsr_hip_texture_create(C.byref(t), 0, 4, 1) == _lib.ERROR_ARGUMENT
+    assert lib.dfpsr_hip_texture_create(C.byref(t), 4, 4, 0) == _lib.ERROR_ARGUMENT
