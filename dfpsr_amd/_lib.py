@@ -35,6 +35,16 @@ class Camera(C.Structure):
                 ("far_clip", C.c_float)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("tile_config", C.c_int32), ("persistent_per_cu", C.c_int32), ("use_graph", C.c_int32),
+                ("clear", C.c_int32), ("clear_color", C.c_uint32), ("clear_depth", C.c_float)]
+
+
+class Job(C.Structure):
+    _fields_ = [("color", Image), ("depth", Image), ("model", C.c_void_p), ("transform", C.c_float * 12),
+                ("camera", Camera)]
+
+
 _lib = None
 
 _P = C.c_void_p
@@ -63,6 +73,15 @@ _SIGNATURES = {
     "dfpsr_hip_renderer_takes_triangles": ([_P, _P], C.c_int),
     "dfpsr_hip_renderer_end": ([_P], C.c_int),
     "dfpsr_hip_renderer_stats": ([_P, _P], C.c_int),
+    "dfpsr_hip_renderer_set_timing": ([_P, _I], C.c_int),
+    "dfpsr_hip_renderer_timing": ([_P, _P], C.c_int),
+    "dfpsr_hip_renderer_set_tuning": ([_P, C.POINTER(Tuning)], C.c_int),
+    "dfpsr_hip_batch_create": ([C.POINTER(_P)], C.c_int),
+    "dfpsr_hip_batch_destroy": ([_P], None),
+    "dfpsr_hip_batch_render": ([_P, C.POINTER(Job), _I, C.POINTER(Tuning)], C.c_int),
+    "dfpsr_hip_batch_stats": ([_P, _P], C.c_int),
+    "dfpsr_hip_batch_set_timing": ([_P, _I], C.c_int),
+    "dfpsr_hip_batch_timing": ([_P, _P], C.c_int),
 }
 
 

@@ -10,7 +10,7 @@ from typing import Optional
 
 import numpy as np
 
-from ._lib import Camera, DfpsrError, Image, check, lib
+from ._lib import Camera, DfpsrError, Image, Job, Tuning, check, lib
 
 Filter_Solid = 0
 Filter_Alpha = 1
@@ -126,6 +126,14 @@ class Renderer:
     @property
     def handle(self):
         return self._h
+
+    def set_timing(self, enable: bool) -> None:
+        check(lib().dfpsr_hip_renderer_set_timing(self._h, int(enable)))
+
+    def timing(self):
+        out = np.zeros(5, np.float32)
+        check(lib().dfpsr_hip_renderer_timing(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def stats(self):
         out = np.zeros(4, np.int64)
@@ -247,6 +255,107 @@ def render_scene(scene, renderer: Optional[Renderer] = None, gpu_scene: Optional
     g.give(r)
     renderer_end(r)
     return color, depth
+
+
+# ---------------------------------------------------------------------------------------
+# Tuning and batches
+# ---------------------------------------------------------------------------------------
+
+def tuning(tile_config: int = 0, persistent_per_cu: int = 0, use_graph: bool = False, clear: bool = False,
+           clear_color: int = 0, clear_depth: float = 0.0) -> Tuning:
+    t = Tuning()
+    t.tile_config, t.persistent_per_cu, t.use_graph = tile_config, persistent_per_cu, int(use_graph)
+    t.clear, t.clear_color, t.clear_depth = int(clear), clear_color, clear_depth
+    return t
+
+
+def renderer_setTuning(renderer: Renderer, t: Tuning) -> None:
+    check(lib().dfpsr_hip_renderer_set_tuning(renderer.handle, C.byref(t)))
+
+
+class Batch:
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().dfpsr_hip_batch_create(C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().dfpsr_hip_batch_destroy(self._h)
+            self._h = None
+
+    def render(self, jobs, t: Tuning) -> None:
+        check(lib().dfpsr_hip_batch_render(self._h, jobs, len(jobs), C.byref(t)))
+
+    def set_timing(self, enable: bool) -> None:
+        check(lib().dfpsr_hip_batch_set_timing(self._h, int(enable)))
+
+    def timing(self):
+        out = np.zeros(5, np.float32)
+        check(lib().dfpsr_hip_batch_timing(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def stats(self):
+        out = np.zeros(4, np.int64)
+        check(lib().dfpsr_hip_batch_stats(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+class GpuJobs:
+    """The jobs of a batch resident on the GPU: shared textures, one model per job and device
+    canvases (torch tensors), with the job array for dfpsr_hip_batch_render."""
+
+    def __init__(self, scenes_list, texture_pool=None):
+        import torch
+        self.scenes = scenes_list
+        self.textures = {}
+        self.gpu = []
+        self.colors, self.depths = [], []
+        self.jobs = (Job * len(scenes_list))()
+        for i, sc in enumerate(scenes_list):
+            texs = []
+            ids = getattr(sc, "texture_ids", None)
+            for k, t in enumerate(sc.textures):
+                key = ("shared", ids[k]) if ids else ("own", id(t))
+                if key not in self.textures:
+                    tex = texture_create_RgbaU8(t.width, t.height, t.resolutions)
+                    texture_write_level0(tex, _tile_level0(t))
+                    texture_generatePyramid(tex)
+                    self.textures[key] = tex
+                texs.append(self.textures[key])
+            model = model_create(sc.filter)
+            model_setPoints(model, sc.points)
+            for p in sc.parts:
+                sl = slice(p.polygon_start, p.polygon_end)
+                model_addPart(model, sc.indices[sl], sc.colors[sl], sc.texcoords[sl],
+                              texs[p.diffuse] if p.diffuse >= 0 else None, texs[p.light] if p.light >= 0 else None)
+            color = torch.zeros((sc.height, sc.width), dtype=torch.int32, device="cuda")
+            depth = torch.zeros((sc.height, sc.width), dtype=torch.float32, device="cuda")
+            cam = camera_create(sc.perspective, sc.cam_location, sc.width, sc.height, sc.width_slope, sc.near_clip, sc.far_clip)
+            j = self.jobs[i]
+            j.color = device_image(color)
+            j.depth = device_image(depth)
+            j.model = model.handle
+            j.transform[:] = [float(v) for v in np.asarray(sc.instances[0], dtype=np.float32)]
+            j.camera = cam
+            self.gpu.append(model)
+            self.colors.append(color)
+            self.depths.append(depth)
+
+    def clear(self):
+        for c, d in zip(self.colors, self.depths):
+            c.zero_()
+            d.zero_()
+
+    def image(self, i):
+        return self.colors[i].cpu().numpy().view(np.uint32), self.depths[i].cpu().numpy()
+
+    def render_plain(self, renderer: Renderer) -> None:
+        """The per-job loop of the plain path: renderer_begin / giveTask / end per job."""
+        for i, j in enumerate(self.jobs):
+            renderer_begin(renderer, j.color, j.depth)
+            check(lib().dfpsr_hip_renderer_give_model(renderer.handle, j.model, j.transform, C.byref(j.camera)))
+            renderer_end(renderer)
 
 
 __all__ = [n for n in dir() if not n.startswith("_")] + ["DfpsrError"]

@@ -242,7 +242,26 @@ struct dfpsr_hip_renderer {
 	int32_t width = 0, height = 0;
 	DevBuf colorStage, depthStage;
 	std::vector<std::unique_ptr<dfpsr_hip_model>> triangleModels;  // giveTask_triangle batches of this frame
+	RenderOptions options;
 };
+
+static RenderOptions optionsFrom(const dfpsr_hip_tuning *t) {
+	RenderOptions o;
+	o.tileConfig = t->tile_config;
+	o.persistentPerCU = t->persistent_per_cu;
+	o.graph = t->use_graph != 0;
+	o.clear = t->clear != 0;
+	o.clearColor = t->clear_color;
+	o.clearDepth = t->clear_depth;
+	return o;
+}
+
+static int checkTuning(const dfpsr_hip_tuning *t) {
+	if (!t) return fail(DFPSR_HIP_ERROR_ARGUMENT, "null tuning");
+	if (t->tile_config < 0 || t->tile_config >= TILE_CONFIGS) return fail(DFPSR_HIP_ERROR_ARGUMENT, "tile_config must be 0, 1 or 2");
+	if (t->persistent_per_cu < 0 || t->persistent_per_cu > 64) return fail(DFPSR_HIP_ERROR_ARGUMENT, "persistent_per_cu must be 0..64");
+	return DFPSR_HIP_OK;
+}
 
 static CameraParams cameraFrom(const dfpsr_hip_camera *c) {
 	return deriveCamera(c->perspective, c->location, c->image_width, c->image_height, c->width_slope, c->near_clip, c->far_clip);
@@ -301,7 +320,7 @@ extern "C" int dfpsr_hip_renderer_begin(dfpsr_hip_renderer *r, const dfpsr_hip_i
 	cv.height = r->height;
 	cv.colorStride = hc ? color->stride_bytes / 4 : 0;
 	cv.depthStride = hd ? depth->stride_bytes / 4 : 0;
-	r->frame.reset();
+	r->frame.reset(r->options.tileConfig);
 	r->frame.addCanvas(cv);
 	r->triangleModels.clear();
 	r->receiving = true;
@@ -364,7 +383,7 @@ extern "C" int dfpsr_hip_renderer_end(dfpsr_hip_renderer *r) {
 	if (!r) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_end: null renderer");
 	if (!r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "Called renderer_end without renderer_begin!");
 	r->receiving = false;
-	int rc = r->frame.render();
+	int rc = r->frame.render(r->options);
 	if (rc) return rc;
 	if (hasImage(&r->color) && !r->color.on_device)
 		HIP_TRY(hipMemcpy(r->color.pixels, r->colorStage.ptr, (size_t)r->color.stride_bytes * r->color.height, hipMemcpyDeviceToHost));
@@ -377,5 +396,118 @@ extern "C" int dfpsr_hip_renderer_end(dfpsr_hip_renderer *r) {
 extern "C" int dfpsr_hip_renderer_stats(const dfpsr_hip_renderer *r, int64_t out[4]) {
 	if (!r || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_stats: null argument");
 	for (int i = 0; i < 4; i++) out[i] = r->frame.stats[i];
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_set_timing(dfpsr_hip_renderer *r, int32_t enable) {
+	if (!r) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_set_timing: null renderer");
+	r->frame.timing = enable != 0;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_timing(const dfpsr_hip_renderer *r, float out[5]) {
+	if (!r || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_timing: null argument");
+	for (int i = 0; i < 5; i++) out[i] = r->frame.stageMs[i];
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_set_tuning(dfpsr_hip_renderer *r, const dfpsr_hip_tuning *t) {
+	if (!r) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_set_tuning: null renderer");
+	int rc = checkTuning(t);
+	if (rc) return rc;
+	if (r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "renderer_set_tuning: cannot change tuning between begin and end");
+	r->options = optionsFrom(t);
+	return DFPSR_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------------------
+
+struct dfpsr_hip_batch {
+	Frame frame;
+	std::vector<dfpsr_hip_job> lastJobs;
+	int32_t lastTileConfig = -1;
+};
+
+extern "C" int dfpsr_hip_batch_create(dfpsr_hip_batch **out) {
+	if (!out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_create: null output");
+	std::unique_ptr<dfpsr_hip_batch> b(new (std::nothrow) dfpsr_hip_batch());
+	if (!b) return fail(DFPSR_HIP_ERROR_MEMORY, "batch_create: out of host memory");
+	int rc = b->frame.init();
+	if (rc) return rc;
+	*out = b.release();
+	return DFPSR_HIP_OK;
+}
+
+extern "C" void dfpsr_hip_batch_destroy(dfpsr_hip_batch *b) { delete b; }
+
+extern "C" int dfpsr_hip_batch_render(dfpsr_hip_batch *b, const dfpsr_hip_job *jobs, int32_t count, const dfpsr_hip_tuning *tuning) {
+	if (!b || count < 0 || (count > 0 && !jobs)) return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_render: bad argument");
+	int rc = checkTuning(tuning);
+	if (rc) return rc;
+	RenderOptions o = optionsFrom(tuning);
+	bool same = (int32_t)b->lastJobs.size() == count && b->lastTileConfig == o.tileConfig &&
+	            (count == 0 || std::memcmp(b->lastJobs.data(), jobs, (size_t)count * sizeof(dfpsr_hip_job)) == 0);
+	if (same) {
+		for (int32_t i = 0; i < count; i++)
+			if (jobs[i].model->dirty) { same = false; break; }
+	}
+	if (!same) {
+		b->frame.reset(o.tileConfig);
+		for (int32_t i = 0; i < count; i++) {
+			const dfpsr_hip_job &j = jobs[i];
+			bool hc = hasImage(&j.color), hd = hasImage(&j.depth);
+			if ((hc && !j.color.on_device) || (hd && !j.depth.on_device))
+				return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_render: job canvases must be device memory");
+			if (hc && hd && (j.color.width != j.depth.width || j.color.height != j.depth.height))
+				return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_render: a job's colour and depth canvases differ in size");
+			for (const dfpsr_hip_image *im : {&j.color, &j.depth})
+				if (hasImage(im) && (im->width < 1 || im->height < 1 || im->stride_bytes < im->width * 4 || (im->stride_bytes & 3)))
+					return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_render: bad canvas size or stride");
+			if (!j.model) return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_render: null model");
+			rc = checkCamera(&j.camera);
+			if (rc) return rc;
+			CanvasHost cv;
+			cv.color = hc ? (uint32_t *)j.color.pixels : nullptr;
+			cv.depth = hd ? (float *)j.depth.pixels : nullptr;
+			cv.width = hc ? j.color.width : (hd ? j.depth.width : 0);
+			cv.height = hc ? j.color.height : (hd ? j.depth.height : 0);
+			cv.colorStride = hc ? j.color.stride_bytes / 4 : 0;
+			cv.depthStride = hd ? j.depth.stride_bytes / 4 : 0;
+			if (!hc && !hd) continue;
+			int32_t slot = b->frame.addCanvas(cv);
+			CameraParams cam = cameraFrom(&j.camera);
+			if (boxSeen(cam, j.model->minBound, j.model->maxBound, transformFrom(j.transform)) == 0) continue;
+			rc = const_cast<dfpsr_hip_model *>(j.model)->upload();
+			if (rc) return rc;
+			rc = b->frame.addTask(j.model->view(), j.transform, cam, slot, false);
+			if (rc) return rc;
+		}
+		rc = b->frame.prepare();
+		if (rc) return rc;
+		b->lastJobs.assign(jobs, jobs + count);
+		b->lastTileConfig = o.tileConfig;
+	}
+	rc = b->frame.run(o);
+	if (rc) b->lastJobs.clear();
+	return rc;
+}
+
+extern "C" int dfpsr_hip_batch_stats(const dfpsr_hip_batch *b, int64_t out[4]) {
+	if (!b || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_stats: null argument");
+	for (int i = 0; i < 4; i++) out[i] = b->frame.stats[i];
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_batch_set_timing(dfpsr_hip_batch *b, int32_t enable) {
+	if (!b) return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_set_timing: null batch");
+	b->frame.timing = enable != 0;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_batch_timing(const dfpsr_hip_batch *b, float out[5]) {
+	if (!b || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_timing: null argument");
+	for (int i = 0; i < 5; i++) out[i] = b->frame.stageMs[i];
 	return DFPSR_HIP_OK;
 }

@@ -1,6 +1,6 @@
 // Host-side frame of the polygon path: the tasks given between renderer_begin and
-// renderer_end, their canvases and textures, the device buffers of the pipeline and the
-// launch sequence (frame.hip).
+// renderer_end (or the jobs of a batch), their canvases and textures, the device buffers of
+// the pipeline and the launch sequence (frame.hip).
 #pragma once
 
 #include "pipeline.h"
@@ -52,20 +52,42 @@ struct ModelView {
 	const TextureDesc *const *partLight;
 };
 
+// How a frame is run.
+struct RenderOptions {
+	int32_t tileConfig = 0;          // tile shape, see tileConfigW/H
+	int32_t persistentPerCU = 0;     // 0: one raster workgroup per tile; k: k persistent workgroups per CU
+	bool graph = false;              // capture the launch sequence once and replay it while nothing changes
+	bool clear = false;              // fill the canvases before drawing (image_fill)
+	uint32_t clearColor = 0;
+	float clearDepth = 0.0f;
+};
+
 class Frame {
 public:
 	hipStream_t stream = nullptr;
-	int64_t stats[4] = {0, 0, 0, 0};  // commands, tile pairs, tiles, passes
+	int64_t stats[4] = {0, 0, 0, 0};     // commands, tile pairs, tiles, passes
+	float stageMs[5] = {0, 0, 0, 0, 0};  // projection, setup, binning, raster, whole pass
+	bool timing = false;
 
 	~Frame();
 	int init();
-	void reset();
+	// Starts a new frame whose canvases are cut into tiles of the given configuration.
+	void reset(int32_t tileConfig = 0);
 	int32_t addCanvas(const CanvasHost &canvas);
 	int addTask(const ModelView &model, const float *transform, const CameraParams &camera, int32_t canvas, bool cameraSpacePoints);
-	// Runs the whole pipeline on the frame's stream and waits for it.
-	int render();
+	// Packs and uploads the frame's tables, then runs the pipeline and waits for it.
+	int render(const RenderOptions &options) {
+		int rc = prepare();
+		return rc ? rc : run(options);
+	}
+	// Packs the tables and uploads them when they changed since the last upload.
+	int prepare();
+	// Runs the pipeline over the prepared tables on the frame's stream and waits for it.
+	int run(const RenderOptions &options);
+	int32_t canvasCount() const { return (int32_t)canvases_.size(); }
 
 private:
+	int32_t tileConfig_ = 0;
 	std::vector<TaskDev> tasks_;
 	std::vector<int32_t> taskFanEnd_, taskPointEnd_;
 	std::vector<CanvasDev> canvases_;
@@ -76,12 +98,22 @@ private:
 	std::vector<size_t> partTextureOffset_;
 	bool alphaAny_ = false;
 	int32_t commandCapacity_ = 0, pairCapacity_ = 0;
-	DevBuf dTasks_, dTaskFanEnd_, dTaskPointEnd_, dCanvases_, dCanvasTileEnd_, dTextures_, dPartTexture_;
-	DevBuf dProjected_, dFanCount_, dBlockSums_, dCommands_, dCounters_, dTileCount_, dTileOffset_, dTileFill_, dTileList_;
+	// All per-frame tables in one upload, skipped when identical to the previous frame's.
+	std::vector<uint8_t> tables_, uploadedTables_;
+	DevBuf dTables_;
+	size_t offTasks_ = 0, offFanEnd_ = 0, offPointEnd_ = 0, offCanvases_ = 0, offTileEnd_ = 0, offTextures_ = 0;
+	bool prepared_ = false;
+	DevBuf dProjected_, dFanCount_, dBlockSums_, dCommands_, dCounters_, dTileCount_, dTileOffset_, dTileFill_, dTileList_, dTileQueue_;
 	int32_t *hostCounters_ = nullptr;
+	hipEvent_t events_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+	hipGraph_t graph_ = nullptr;
+	hipGraphExec_t graphExec_ = nullptr;
+	std::vector<uint8_t> graphKey_;
+	int32_t deviceCUs_ = 0;
 
 	int32_t textureSlot(const TextureDesc *desc);
-	int upload(const void *src, size_t bytes, DevBuf &dst);
+	int enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks);
+	void dropGraph();
 };
 
 }  // namespace dfpsr

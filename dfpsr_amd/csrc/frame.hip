@@ -1,7 +1,10 @@
-// The launch sequence of one frame (renderer_end): projection -> triangle setup (count,
-// scan, emit) -> tile binning (count, scan, fill) -> tile raster.  Buffers that depend on
-// data (commands, tile pairs) start from an estimate; a pass that runs out of room raises a
-// flag, leaves the canvases untouched, and the frame is run again with the exact sizes.
+// The launch sequence of one frame (renderer_end, or one batch of jobs):
+//   reset counters -> [clear canvases] -> projection -> triangle setup (count, scan, emit)
+//   -> tile binning (count, scan + tile queue, fill) -> tile raster.
+// Buffers that depend on data (draw commands, tile pairs) start from an estimate; a pass that
+// runs out of room raises a flag, leaves the canvases untouched, and the frame is run again
+// with the exact sizes.  With RenderOptions::graph the sequence is captured once into a
+// hipGraph and replayed while the frame's tables and buffers stay the same.
 #include "frame.h"
 
 #include <cstring>
@@ -9,17 +12,33 @@
 namespace dfpsr {
 
 Frame::~Frame() {
+	dropGraph();
+	for (hipEvent_t e : events_)
+		if (e) (void)hipEventDestroy(e);
 	if (hostCounters_) (void)hipHostFree(hostCounters_);
 	if (stream) (void)hipStreamDestroy(stream);
 }
 
 int Frame::init() {
 	HIP_TRY(hipStreamCreate(&stream));  // blocking: ordered after work on the legacy default stream
-	HIP_TRY(hipHostMalloc((void **)&hostCounters_, 4 * sizeof(int32_t), hipHostMallocDefault));
+	HIP_TRY(hipHostMalloc((void **)&hostCounters_, 8 * sizeof(int32_t), hipHostMallocDefault));
+	for (hipEvent_t &e : events_) HIP_TRY(hipEventCreate(&e));
+	int device = 0;
+	HIP_TRY(hipGetDevice(&device));
+	HIP_TRY(hipDeviceGetAttribute(&deviceCUs_, hipDeviceAttributeMultiprocessorCount, device));
 	return 0;
 }
 
-void Frame::reset() {
+void Frame::dropGraph() {
+	if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+	if (graph_) (void)hipGraphDestroy(graph_);
+	graphExec_ = nullptr;
+	graph_ = nullptr;
+	graphKey_.clear();
+}
+
+void Frame::reset(int32_t tileConfig) {
+	tileConfig_ = (tileConfig >= 0 && tileConfig < TILE_CONFIGS) ? tileConfig : 0;
 	tasks_.clear();
 	taskFanEnd_.clear();
 	taskPointEnd_.clear();
@@ -33,6 +52,7 @@ void Frame::reset() {
 }
 
 int32_t Frame::addCanvas(const CanvasHost &c) {
+	const int32_t tw = tileConfigW(tileConfig_), th = tileConfigH(tileConfig_);
 	CanvasDev d;
 	d.color = c.color;
 	d.depth = c.depth;
@@ -40,8 +60,8 @@ int32_t Frame::addCanvas(const CanvasHost &c) {
 	d.height = c.height;
 	d.colorStride = c.colorStride;
 	d.depthStride = c.depthStride;
-	d.tilesX = (c.width + TILE_W - 1) / TILE_W;
-	d.tilesY = (c.height + TILE_H - 1) / TILE_H;
+	d.tilesX = (c.width + tw - 1) / tw;
+	d.tilesY = (c.height + th - 1) / th;
 	d.tileBase = canvasTileEnd_.empty() ? 0 : canvasTileEnd_.back();
 	canvases_.push_back(d);
 	canvasTileEnd_.push_back(d.tileBase + d.tilesX * d.tilesY);
@@ -67,7 +87,7 @@ int Frame::addTask(const ModelView &m, const float *transform, const CameraParam
 	t.texcoords = m.texcoords;
 	t.fanPolygon = m.fanPolygon;
 	t.polygonPart = m.polygonPart;
-	t.partTexture = nullptr;  // patched at upload
+	t.partTexture = nullptr;  // patched when the tables are packed
 	t.pointCount = m.pointCount;
 	t.fanCount = m.fanCount;
 	t.pointBase = taskPointEnd_.empty() ? 0 : taskPointEnd_.back();
@@ -89,39 +109,88 @@ int Frame::addTask(const ModelView &m, const float *transform, const CameraParam
 	return 0;
 }
 
-int Frame::upload(const void *src, size_t bytes, DevBuf &dst) {
-	if (dst.ensure(bytes) != hipSuccess) return fail(-4, "device allocation failed");
-	if (bytes) HIP_TRY(hipMemcpyAsync(dst.ptr, src, bytes, hipMemcpyHostToDevice, stream));
+static inline int32_t blocksOf(int64_t n, int32_t size) { return (int32_t)((n + size - 1) / size); }
+static inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks) {
+	const bool events = timing && !o.graph;
+	const int32_t tileTotal = f.tileTotal;
+	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, f.counters, 8, 0);
+	hipLaunchKernelGGL(k_fill, dim3(blocksOf(2 * (int64_t)tileTotal + 1, 256)), dim3(256), 0, stream, f.tileCount, 2 * tileTotal + 1, 0);
+	if (o.clear) hipLaunchKernelGGL(k_clear_canvases, dim3(tileTotal), dim3(256), 0, stream, f, o.clearColor, o.clearDepth);
+	if (events) HIP_TRY(hipEventRecord(events_[0], stream));
+	hipLaunchKernelGGL(k_project, dim3(blocksOf(f.pointTotal, 256)), dim3(256), 0, stream, f);
+	if (events) HIP_TRY(hipEventRecord(events_[1], stream));
+	hipLaunchKernelGGL(k_setup_count, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
+	hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, fanBlocks, f.counters + 0);
+	hipLaunchKernelGGL(k_setup_emit, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
+	if (events) HIP_TRY(hipEventRecord(events_[2], stream));
+	hipLaunchKernelGGL(k_bin_count, dim3(blocksOf(f.commandCapacity, 256)), dim3(256), 0, stream, f);
+	hipLaunchKernelGGL(k_scan_tiles_local, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
+	hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, tileBlocks, f.counters + 1);
+	hipLaunchKernelGGL(k_scan_tiles_add, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
+	hipLaunchKernelGGL(k_bin_fill, dim3(blocksOf(f.commandCapacity, 256)), dim3(256), 0, stream, f);
+	if (events) HIP_TRY(hipEventRecord(events_[3], stream));
+	const int32_t persistent = o.persistentPerCU > 0 ? 1 : 0;
+	const int32_t grid = persistent ? deviceCUs_ * o.persistentPerCU : tileTotal;
+	switch (tileConfig_) {
+	case 1: hipLaunchKernelGGL((k_raster<32, 16>), dim3(grid), dim3(128), 0, stream, f, persistent); break;
+	case 2: hipLaunchKernelGGL((k_raster<32, 32>), dim3(grid), dim3(256), 0, stream, f, persistent); break;
+	default: hipLaunchKernelGGL((k_raster<16, 16>), dim3(grid), dim3(64), 0, stream, f, persistent); break;
+	}
+	if (events) HIP_TRY(hipEventRecord(events_[4], stream));
+	HIP_TRY(hipGetLastError());
 	return 0;
 }
 
-static inline int32_t blocksOf(int64_t n, int32_t size) { return (int32_t)((n + size - 1) / size); }
-
-int Frame::render() {
-	stats[0] = stats[1] = stats[2] = stats[3] = 0;
+int Frame::prepare() {
+	prepared_ = false;
 	if (tasks_.empty() || canvases_.empty()) return 0;
+	if (partTexture_.empty()) partTexture_.push_back(-1);
+	if (textures_.empty()) {
+		TextureDesc none;
+		std::memset(&none, 0, sizeof(none));
+		textures_.push_back(none);
+	}
+	const size_t nT = tasks_.size(), nC = canvases_.size();
+	offTasks_ = 0;
+	offFanEnd_ = align256(offTasks_ + nT * sizeof(TaskDev));
+	offPointEnd_ = align256(offFanEnd_ + nT * 4);
+	offCanvases_ = align256(offPointEnd_ + nT * 4);
+	offTileEnd_ = align256(offCanvases_ + nC * sizeof(CanvasDev));
+	offTextures_ = align256(offTileEnd_ + nC * 4);
+	const size_t offParts = align256(offTextures_ + textures_.size() * sizeof(TextureDesc));
+	const size_t total = align256(offParts + partTexture_.size() * 4);
+	if (dTables_.ensure(total) != hipSuccess) return fail(-4, "device allocation failed");
+	uint8_t *base = (uint8_t *)dTables_.ptr;
+	tables_.assign(total, 0);
+	for (size_t i = 0; i < nT; i++) tasks_[i].partTexture = (const int32_t *)(base + offParts) + partTextureOffset_[i];
+	std::memcpy(&tables_[offTasks_], tasks_.data(), nT * sizeof(TaskDev));
+	std::memcpy(&tables_[offFanEnd_], taskFanEnd_.data(), nT * 4);
+	std::memcpy(&tables_[offPointEnd_], taskPointEnd_.data(), nT * 4);
+	std::memcpy(&tables_[offCanvases_], canvases_.data(), nC * sizeof(CanvasDev));
+	std::memcpy(&tables_[offTileEnd_], canvasTileEnd_.data(), nC * 4);
+	std::memcpy(&tables_[offTextures_], textures_.data(), textures_.size() * sizeof(TextureDesc));
+	std::memcpy(&tables_[offParts], partTexture_.data(), partTexture_.size() * 4);
+	if (tables_ != uploadedTables_) {
+		HIP_TRY(hipMemcpyAsync(dTables_.ptr, tables_.data(), total, hipMemcpyHostToDevice, stream));
+		uploadedTables_ = tables_;
+	}
+	prepared_ = true;
+	return 0;
+}
+
+int Frame::run(const RenderOptions &o) {
+	stats[0] = stats[1] = stats[2] = stats[3] = 0;
+	for (float &m : stageMs) m = 0.0f;
+	if (!prepared_ || tasks_.empty() || canvases_.empty()) return 0;
 	const int32_t pointTotal = taskPointEnd_.back();
 	const int32_t fanTotal = taskFanEnd_.back();
 	const int32_t tileTotal = canvasTileEnd_.back();
 	if (fanTotal == 0 || tileTotal == 0) return 0;
+	const size_t nT = tasks_.size(), nC = canvases_.size();
+	uint8_t *base = (uint8_t *)dTables_.ptr;
 
-	// Per-frame tables.
-	if (partTexture_.empty()) partTexture_.push_back(-1);
-	if (dPartTexture_.ensure(partTexture_.size() * 4) != hipSuccess) return fail(-4, "device allocation failed");
-	for (size_t i = 0; i < tasks_.size(); i++) tasks_[i].partTexture = dPartTexture_.as<int32_t>() + partTextureOffset_[i];
-	int rc;
-	if ((rc = upload(partTexture_.data(), partTexture_.size() * 4, dPartTexture_))) return rc;
-	if ((rc = upload(tasks_.data(), tasks_.size() * sizeof(TaskDev), dTasks_))) return rc;
-	if ((rc = upload(taskFanEnd_.data(), taskFanEnd_.size() * 4, dTaskFanEnd_))) return rc;
-	if ((rc = upload(taskPointEnd_.data(), taskPointEnd_.size() * 4, dTaskPointEnd_))) return rc;
-	if ((rc = upload(canvases_.data(), canvases_.size() * sizeof(CanvasDev), dCanvases_))) return rc;
-	if ((rc = upload(canvasTileEnd_.data(), canvasTileEnd_.size() * 4, dCanvasTileEnd_))) return rc;
-	TextureDesc none;
-	std::memset(&none, 0, sizeof(none));
-	if (textures_.empty()) textures_.push_back(none);
-	if ((rc = upload(textures_.data(), textures_.size() * sizeof(TextureDesc), dTextures_))) return rc;
-
-	// Working buffers.
 	const int32_t fanBlocks = blocksOf(fanTotal, SCAN_BLOCK);
 	const int32_t tileBlocks = blocksOf(tileTotal, SCAN_BLOCK);
 	if (commandCapacity_ < fanTotal + 1024) commandCapacity_ = fanTotal + 1024;
@@ -129,8 +198,8 @@ int Frame::render() {
 	if (dProjected_.ensure((size_t)pointTotal * sizeof(Projected)) != hipSuccess ||
 	    dFanCount_.ensure((size_t)fanTotal * 4) != hipSuccess ||
 	    dBlockSums_.ensure((size_t)(fanBlocks > tileBlocks ? fanBlocks : tileBlocks) * 4) != hipSuccess ||
-	    dCounters_.ensure(4 * 4) != hipSuccess || dTileCount_.ensure(((size_t)tileTotal + 1) * 4) != hipSuccess ||
-	    dTileOffset_.ensure(((size_t)tileTotal + 1) * 4) != hipSuccess || dTileFill_.ensure((size_t)tileTotal * 4) != hipSuccess)
+	    dCounters_.ensure(8 * 4) != hipSuccess || dTileCount_.ensure((2 * (size_t)tileTotal + 1) * 4) != hipSuccess ||
+	    dTileOffset_.ensure(((size_t)tileTotal + 1) * 4) != hipSuccess || dTileQueue_.ensure((size_t)tileTotal * 4) != hipSuccess)
 		return fail(-4, "device allocation failed");
 
 	for (int pass = 1; pass <= 3; pass++) {
@@ -138,55 +207,77 @@ int Frame::render() {
 		    dTileList_.ensure((size_t)pairCapacity_ * 4) != hipSuccess)
 			return fail(-4, "device allocation failed");
 		FrameDev f;
-		f.tasks = dTasks_.as<TaskDev>();
-		f.taskCount = (int32_t)tasks_.size();
-		f.taskFanEnd = dTaskFanEnd_.as<int32_t>();
-		f.taskPointEnd = dTaskPointEnd_.as<int32_t>();
+		std::memset(&f, 0, sizeof(f));
+		f.tasks = (const TaskDev *)(base + offTasks_);
+		f.taskCount = (int32_t)nT;
+		f.taskFanEnd = (const int32_t *)(base + offFanEnd_);
+		f.taskPointEnd = (const int32_t *)(base + offPointEnd_);
 		f.fanTotal = fanTotal;
 		f.pointTotal = pointTotal;
-		f.canvases = dCanvases_.as<CanvasDev>();
-		f.canvasCount = (int32_t)canvases_.size();
-		f.canvasTileEnd = dCanvasTileEnd_.as<int32_t>();
+		f.canvases = (const CanvasDev *)(base + offCanvases_);
+		f.canvasCount = (int32_t)nC;
+		f.canvasTileEnd = (const int32_t *)(base + offTileEnd_);
 		f.tileTotal = tileTotal;
-		f.textures = dTextures_.as<TextureDesc>();
+		f.textures = (const TextureDesc *)(base + offTextures_);
 		f.projected = dProjected_.as<Projected>();
 		f.fanCount = dFanCount_.as<int32_t>();
 		f.blockSums = dBlockSums_.as<int32_t>();
 		f.commands = dCommands_.as<Command>();
 		f.commandCapacity = commandCapacity_;
 		f.counters = dCounters_.as<int32_t>();
+		f.queueHead = dCounters_.as<int32_t>() + 4;
 		f.tileCount = dTileCount_.as<int32_t>();
+		f.tileFill = dTileCount_.as<int32_t>() + tileTotal + 1;
 		f.tileOffset = dTileOffset_.as<int32_t>();
-		f.tileFill = dTileFill_.as<int32_t>();
 		f.tileList = dTileList_.as<int32_t>();
+		f.tileQueue = dTileQueue_.as<int32_t>();
 		f.pairCapacity = pairCapacity_;
+		f.tileW = tileConfigW(tileConfig_);
+		f.tileH = tileConfigH(tileConfig_);
 		f.alphaAny = alphaAny_ ? 1 : 0;
 
-		HIP_TRY(hipMemsetAsync(f.counters, 0, 16, stream));
-		HIP_TRY(hipMemsetAsync(f.tileCount, 0, ((size_t)tileTotal + 1) * 4, stream));
-		HIP_TRY(hipMemsetAsync(f.tileFill, 0, (size_t)tileTotal * 4, stream));
-		hipLaunchKernelGGL(k_project, dim3(blocksOf(pointTotal, 256)), dim3(256), 0, stream, f);
-		hipLaunchKernelGGL(k_setup_count, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
-		hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, fanBlocks, f.counters + 0);
-		hipLaunchKernelGGL(k_setup_emit, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
-		hipLaunchKernelGGL(k_bin_count, dim3(blocksOf(commandCapacity_, 256)), dim3(256), 0, stream, f);
-		hipLaunchKernelGGL(k_scan_tiles_local, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
-		hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, tileBlocks, f.counters + 1);
-		hipLaunchKernelGGL(k_scan_tiles_add, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
-		hipLaunchKernelGGL(k_bin_fill, dim3(blocksOf(commandCapacity_, 256)), dim3(256), 0, stream, f);
-		hipLaunchKernelGGL(k_raster, dim3(tileTotal), dim3(TILE_LANES), 0, stream, f);
-		HIP_TRY(hipGetLastError());
+		if (o.graph) {
+			// The graph stays valid while the launch arguments and options are the same.
+			std::vector<uint8_t> key(sizeof(FrameDev) + sizeof(RenderOptions) + sizeof(int32_t));
+			std::memcpy(key.data(), &f, sizeof(FrameDev));
+			std::memcpy(key.data() + sizeof(FrameDev), &o, sizeof(RenderOptions));
+			std::memcpy(key.data() + sizeof(FrameDev) + sizeof(RenderOptions), &tileConfig_, sizeof(int32_t));
+			if (!graphExec_ || key != graphKey_) {
+				dropGraph();
+				HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+				int rc = enqueue(f, o, fanBlocks, tileBlocks);
+				hipGraph_t g = nullptr;
+				hipError_t e = hipStreamEndCapture(stream, &g);
+				if (rc) return rc;
+				if (e != hipSuccess) return fail(-3, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+				graph_ = g;
+				HIP_TRY(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
+				graphKey_ = key;
+			}
+			if (timing) HIP_TRY(hipEventRecord(events_[0], stream));
+			HIP_TRY(hipGraphLaunch(graphExec_, stream));
+			if (timing) HIP_TRY(hipEventRecord(events_[4], stream));
+		} else {
+			int rc = enqueue(f, o, fanBlocks, tileBlocks);
+			if (rc) return rc;
+		}
 		HIP_TRY(hipMemcpyAsync(hostCounters_, f.counters, 16, hipMemcpyDeviceToHost, stream));
 		HIP_TRY(hipStreamSynchronize(stream));
 		stats[0] = hostCounters_[0];
 		stats[1] = hostCounters_[1];
-		stats[2] = tileTotal;
+		stats[2] = hostCounters_[3];
 		stats[3] = pass;
+		if (timing) {
+			if (!o.graph)
+				for (int k = 0; k < 4; k++) HIP_TRY(hipEventElapsedTime(&stageMs[k], events_[k], events_[k + 1]));
+			HIP_TRY(hipEventElapsedTime(&stageMs[4], events_[0], events_[4]));
+		}
 		int32_t overflow = hostCounters_[2];
 		if (overflow == 0) return 0;
 		if (overflow & 1) commandCapacity_ = hostCounters_[0] + 1024;
 		if (overflow & 2) pairCapacity_ = hostCounters_[1] + 1024;
 		if ((overflow & 1) && pairCapacity_ < 4 * commandCapacity_) pairCapacity_ = 4 * commandCapacity_;
+		dropGraph();
 	}
 	return fail(-4, "renderer_end: buffers did not converge");
 }

@@ -160,7 +160,7 @@ DH Rect canvasRect(const CanvasDev &cv) { return rectSides(0, 0, cv.width, cv.he
 
 // Pixel region a command can touch: columns of cut(wholeBound, canvas) rounded out to quads,
 // rows of the aligned raster bound (ITriangle2D.cpp:70-84).
-DH TileRange commandTiles(const Command &c, const CanvasDev &cv) {
+DH TileRange commandTiles(const Command &c, const CanvasDev &cv, int32_t tw, int32_t th) {
 	TileRange r;
 	Rect wb = triangleBound(l2{c.fx[0], c.fy[0]}, l2{c.fx[1], c.fy[1]}, l2{c.fx[2], c.fy[2]});
 	Rect full = canvasRect(cv);
@@ -168,10 +168,10 @@ DH TileRange commandTiles(const Command &c, const CanvasDev &cv) {
 	if (!r.any) { r.tx0 = r.tx1 = r.ty0 = r.ty1 = 0; return r; }
 	Rect clip = cutRect(wb, full);
 	Rect al = alignedRasterBound(wb, full);
-	r.tx0 = clip.l / TILE_W;
-	r.tx1 = (clip.right() - 1) / TILE_W;
-	r.ty0 = al.t / TILE_H;
-	r.ty1 = (al.bottom() - 2) / TILE_H;
+	r.tx0 = clip.l / tw;
+	r.tx1 = (clip.right() - 1) / tw;
+	r.ty0 = al.t / th;
+	r.ty1 = (al.bottom() - 2) / th;
 	if (r.ty1 > cv.tilesY - 1) r.ty1 = cv.tilesY - 1;
 	return r;
 }
@@ -181,7 +181,7 @@ __global__ void k_bin_count(FrameDev f) {
 	if (c >= f.counters[0] || f.counters[2]) return;
 	const Command &cmd = f.commands[c];
 	const CanvasDev &cv = f.canvases[cmd.canvas];
-	TileRange r = commandTiles(cmd, cv);
+	TileRange r = commandTiles(cmd, cv, f.tileW, f.tileH);
 	if (!r.any) return;
 	for (int32_t ty = r.ty0; ty <= r.ty1; ty++)
 		for (int32_t tx = r.tx0; tx <= r.tx1; tx++) atomicAdd(&f.tileCount[cv.tileBase + ty * cv.tilesX + tx], 1);
@@ -198,7 +198,10 @@ __global__ void k_scan_tiles_local(FrameDev f) {
 
 __global__ void k_scan_tiles_add(FrameDev f) {
 	int32_t i = blockIdx.x * SCAN_BLOCK + threadIdx.x;
-	if (i < f.tileTotal) f.tileOffset[i] += f.blockSums[blockIdx.x];
+	if (i < f.tileTotal) {
+		f.tileOffset[i] += f.blockSums[blockIdx.x];
+		if (f.tileCount[i] > 0) f.tileQueue[atomicAdd(&f.counters[3], 1)] = i;
+	}
 	if (i == f.tileTotal - 1) {
 		int32_t end = f.tileOffset[i] + f.tileCount[i];
 		f.tileOffset[f.tileTotal] = end;
@@ -211,7 +214,7 @@ __global__ void k_bin_fill(FrameDev f) {
 	if (c >= f.counters[0] || f.counters[2]) return;
 	const Command &cmd = f.commands[c];
 	const CanvasDev &cv = f.canvases[cmd.canvas];
-	TileRange r = commandTiles(cmd, cv);
+	TileRange r = commandTiles(cmd, cv, f.tileW, f.tileH);
 	if (!r.any) return;
 	for (int32_t ty = r.ty0; ty <= r.ty1; ty++) {
 		for (int32_t tx = r.tx0; tx <= r.tx1; tx++) {
@@ -226,15 +229,16 @@ __global__ void k_bin_fill(FrameDev f) {
 // Tile raster / shade.
 // ---------------------------------------------------------------------------------------
 
-// Bitonic sort of n ints in LDS by one wavefront (n <= TILE_SORT_CAP).
+// Bitonic sort of n ints in LDS by the workgroup's NL lanes (n <= TILE_SORT_CAP).
+template <int NL>
 __device__ void sortLds(int32_t *a, int32_t n) {
 	int32_t size = 1;
 	while (size < n) size <<= 1;
-	for (int32_t i = n + (int32_t)threadIdx.x; i < size; i += TILE_LANES) a[i] = 0x7FFFFFFF;
+	for (int32_t i = n + (int32_t)threadIdx.x; i < size; i += NL) a[i] = 0x7FFFFFFF;
 	__syncthreads();
 	for (int32_t k = 2; k <= size; k <<= 1) {
 		for (int32_t j = k >> 1; j > 0; j >>= 1) {
-			for (int32_t i = threadIdx.x; i < size; i += TILE_LANES) {
+			for (int32_t i = threadIdx.x; i < size; i += NL) {
 				int32_t ixj = i ^ j;
 				if (ixj > i) {
 					int32_t x = a[i], y = a[ixj];
@@ -268,8 +272,6 @@ __device__ __forceinline__ void setDepth(QuadPixels &q, int s, float v) {
 	if (s == 0) q.depth[0] = v; else if (s == 1) q.depth[1] = v; else if (s == 2) q.depth[2] = v; else q.depth[3] = v;
 	q.depthDirty |= 1u << s;
 }
-
-struct RowPairLds { RowInterval rows[TILE_H]; };
 
 // Draws one command into the lane's quad: the row-pair logic of fillShapeSuper
 // (fillerTemplates.h:246-385), the incremental weight stepping of fillRowSuper (:195-244)
@@ -419,25 +421,22 @@ __device__ void drawCommand(const FrameDev &f, const Command &cmd, const TaskDev
 	}
 }
 
-__global__ void __launch_bounds__(TILE_LANES) k_raster(FrameDev f) {
-	__shared__ int32_t list[TILE_SORT_CAP];
-	__shared__ RowInterval rows[TILE_H];
-	__shared__ int32_t roundInfo[2];
-	const int32_t tile = blockIdx.x;
-	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvas untouched
+// One tile: load the lanes' pixels, draw every binned command in increasing command index
+// (submission order), write back the pixels that changed.
+template <int TW, int TH>
+__device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowInterval *rows, int32_t *roundInfo) {
+	constexpr int32_t NL = (TW / 2) * (TH / 2);
 	const int32_t n = f.tileCount[tile];
-	if (n == 0) return;
 	const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
 	const CanvasDev cv = f.canvases[ci];
 	const int32_t local = tile - cv.tileBase;
-	const int32_t tileX0 = (local % cv.tilesX) * TILE_W;
-	const int32_t tileY0 = (local / cv.tilesX) * TILE_H;
+	const int32_t tileX0 = (local % cv.tilesX) * TW;
+	const int32_t tileY0 = (local / cv.tilesX) * TH;
 	const int32_t lane = threadIdx.x;
-	const int32_t px = tileX0 + 2 * (lane % (TILE_W / 2));
-	const int32_t py = tileY0 + 2 * (lane / (TILE_W / 2));
+	const int32_t px = tileX0 + 2 * (lane % (TW / 2));
+	const int32_t py = tileY0 + 2 * (lane / (TW / 2));
 	const int32_t *src = f.tileList + f.tileOffset[tile];
 
-	// Load the quad's pixels that exist on the canvas.
 	QuadPixels q;
 	q.colorDirty = q.depthDirty = 0;
 	for (int s = 0; s < 4; s++) {
@@ -447,44 +446,46 @@ __global__ void __launch_bounds__(TILE_LANES) k_raster(FrameDev f) {
 		q.color[s] = (in && cv.color && f.alphaAny) ? cv.color[(size_t)y * cv.colorStride + x] : 0u;
 	}
 
-	// Process the tile's commands in increasing command index, in rounds of at most
-	// TILE_SORT_CAP; each round takes the next commands above the last one drawn.
+	// Commands in increasing index, in rounds of at most TILE_SORT_CAP: each round takes the
+	// commands in (last, threshold] with the largest threshold that keeps the round in LDS.
 	int32_t last = -1;
 	int32_t done = 0;
 	while (done < n) {
 		int32_t take;
-		if (n - done <= TILE_SORT_CAP && done == 0) {
-			for (int32_t i = lane; i < n; i += TILE_LANES) list[i] = src[i];
+		if (done == 0 && n <= TILE_SORT_CAP) {
+			for (int32_t i = lane; i < n; i += NL) list[i] = src[i];
 			take = n;
 		} else {
-			// Largest threshold with at most TILE_SORT_CAP commands in (last, threshold].
-			int32_t lo = last + 1, hi = 0x7FFFFFFE;
-			if (lane == 0) { roundInfo[0] = lo; roundInfo[1] = hi; }
+			if (lane == 0) { roundInfo[0] = last + 1; roundInfo[1] = 0x7FFFFFFE; }
 			while (true) {
 				__syncthreads();
-				lo = roundInfo[0]; hi = roundInfo[1];
+				int32_t lo = roundInfo[0], hi = roundInfo[1];
 				if (lo >= hi) break;
 				int32_t mid = lo + ((hi - lo + 1) >> 1);
-				int32_t cnt = 0;
-				for (int32_t i = lane; i < n; i += TILE_LANES) { int32_t v = src[i]; cnt += (v > last && v <= mid) ? 1 : 0; }
-				for (int32_t o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
 				__syncthreads();
-				if (lane == 0) { if (cnt <= TILE_SORT_CAP) roundInfo[0] = mid; else roundInfo[1] = mid - 1; }
+				if (lane == 0) roundInfo[2] = 0;
+				__syncthreads();
+				int32_t cnt = 0;
+				for (int32_t i = lane; i < n; i += NL) { int32_t v = src[i]; cnt += (v > last && v <= mid) ? 1 : 0; }
+				if (cnt) atomicAdd(&roundInfo[2], cnt);
+				__syncthreads();
+				if (lane == 0) { if (roundInfo[2] <= TILE_SORT_CAP) roundInfo[0] = mid; else roundInfo[1] = mid - 1; }
 			}
 			__syncthreads();
 			int32_t threshold = roundInfo[0];
-			if (lane == 0) roundInfo[1] = 0;
 			__syncthreads();
-			for (int32_t i = lane; i < n; i += TILE_LANES) {
+			if (lane == 0) roundInfo[2] = 0;
+			__syncthreads();
+			for (int32_t i = lane; i < n; i += NL) {
 				int32_t v = src[i];
-				if (v > last && v <= threshold) list[atomicAdd(&roundInfo[1], 1)] = v;
+				if (v > last && v <= threshold) list[atomicAdd(&roundInfo[2], 1)] = v;
 			}
 			__syncthreads();
-			take = roundInfo[1];
+			take = roundInfo[2];
 			last = threshold;
 		}
 		__syncthreads();
-		sortLds(list, take);
+		sortLds<NL>(list, take);
 		for (int32_t i = 0; i < take; i++) {
 			const int32_t c = __builtin_amdgcn_readfirstlane(list[i]);
 			const Command cmd = f.commands[c];
@@ -494,13 +495,13 @@ __global__ void __launch_bounds__(TILE_LANES) k_raster(FrameDev f) {
 			const bool depthRead = cv.depth != nullptr;
 			const bool depthWrite = cv.depth != nullptr && (solid || cv.color == nullptr);
 			const bool alphaFilter = colorWrite && !solid;
-			// Rows of this command inside the tile (rasterizeTriangle over the aligned bound).
+			// Rows of this command inside the tile: rasterizeTriangle over the aligned bound.
 			Rect wb = triangleBound(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]});
 			Rect clipR = cutRect(wb, canvasRect(cv));
 			Rect al = alignedRasterBound(wb, canvasRect(cv));
 			Rect rowClip = rectSides(clipR.l, al.t, clipR.right(), al.bottom());
 			__syncthreads();
-			if (lane < TILE_H) {
+			if (lane < TH) {
 				int32_t y = tileY0 + lane;
 				if (y >= al.t && y < al.bottom()) {
 					l2 A{cmd.fx[0], cmd.fy[0]}, B{cmd.fx[1], cmd.fy[1]}, C{cmd.fx[2], cmd.fy[2]};
@@ -521,6 +522,58 @@ __global__ void __launch_bounds__(TILE_LANES) k_raster(FrameDev f) {
 		if (x >= cv.width || y >= cv.height) continue;
 		if ((q.colorDirty >> s) & 1u) cv.color[(size_t)y * cv.colorStride + x] = q.color[s];
 		if ((q.depthDirty >> s) & 1u) cv.depth[(size_t)y * cv.depthStride + x] = q.depth[s];
+	}
+}
+
+// The raster kernel over the queue of non-empty tiles.  persistent = 0: workgroup b takes
+// queue slot b (the grid covers every tile; surplus workgroups exit).  persistent = 1: a fixed
+// grid whose workgroups pull slots from an atomic head until the queue is empty, so every
+// workgroup reaches the exit once all tiles are handed out.
+template <int TW, int TH>
+__global__ void __launch_bounds__((TW / 2) * (TH / 2)) k_raster(FrameDev f, int32_t persistent) {
+	__shared__ int32_t list[TILE_SORT_CAP];
+	__shared__ RowInterval rows[TH];
+	__shared__ int32_t roundInfo[3];
+	__shared__ int32_t slotLds;
+	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
+	const int32_t queued = f.counters[3];
+	int32_t slot = blockIdx.x;
+	while (true) {
+		if (persistent) {
+			if (threadIdx.x == 0) slotLds = atomicAdd(f.queueHead, 1);
+			__syncthreads();
+			slot = slotLds;
+			__syncthreads();
+		}
+		if (slot >= queued) break;
+		rasterTile<TW, TH>(f, f.tileQueue[slot], list, rows, roundInfo);
+		if (!persistent) break;
+		__syncthreads();
+	}
+}
+
+template __global__ void k_raster<16, 16>(FrameDev, int32_t);
+template __global__ void k_raster<32, 16>(FrameDev, int32_t);
+template __global__ void k_raster<32, 32>(FrameDev, int32_t);
+
+__global__ void k_fill(int32_t *p, int32_t n, int32_t value) {
+	int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = value;
+}
+
+// image_fill of every canvas (colour and depth), one thread per pixel of the frame's tiles.
+__global__ void k_clear_canvases(FrameDev f, uint32_t color, float depth) {
+	const int32_t tile = blockIdx.x;
+	if (tile >= f.tileTotal) return;
+	const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
+	const CanvasDev cv = f.canvases[ci];
+	const int32_t local = tile - cv.tileBase;
+	const int32_t x0 = (local % cv.tilesX) * f.tileW, y0 = (local / cv.tilesX) * f.tileH;
+	for (int32_t i = threadIdx.x; i < f.tileW * f.tileH; i += blockDim.x) {
+		int32_t x = x0 + i % f.tileW, y = y0 + i / f.tileW;
+		if (x >= cv.width || y >= cv.height) continue;
+		if (cv.color) cv.color[(size_t)y * cv.colorStride + x] = color;
+		if (cv.depth) cv.depth[(size_t)y * cv.depthStride + x] = depth;
 	}
 }
 

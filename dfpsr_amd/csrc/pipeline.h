@@ -5,12 +5,13 @@
 
 namespace dfpsr {
 
-// Screen tiles: 16x16 pixels, one 64-lane wavefront per tile and one lane per 2x2 quad.
-// Both sides are even, so a row pair and a quad never straddle two tiles.
-constexpr int32_t TILE_W = 16;
-constexpr int32_t TILE_H = 16;
-constexpr int32_t TILE_LANES = (TILE_W / 2) * (TILE_H / 2);
-static_assert(TILE_LANES == 64, "one wavefront per tile");
+// Screen tiles are TW x TH pixels with even sides, so a row pair and a 2x2 quad never
+// straddle two tiles; the raster kernel gives each tile one workgroup of (TW/2)*(TH/2)
+// lanes, one lane per quad.  Instantiated shapes (the autotuner picks among them):
+//   0: 16x16 (64 lanes, one wavefront)   1: 32x16 (128 lanes)   2: 32x32 (256 lanes)
+constexpr int32_t TILE_CONFIGS = 3;
+DH int32_t tileConfigW(int32_t c) { return c == 0 ? 16 : 32; }
+DH int32_t tileConfigH(int32_t c) { return c == 2 ? 32 : 16; }
 // Commands a tile sorts in LDS at once; longer lists are processed in ordered rounds.
 constexpr int32_t TILE_SORT_CAP = 1024;
 
@@ -66,12 +67,15 @@ struct FrameDev {
 	int32_t *blockSums;           // scan scratch
 	Command *commands;
 	int32_t commandCapacity;
-	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flag
+	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flags, [3] queued tiles
 	int32_t *tileCount;           // tileTotal + 1
 	int32_t *tileOffset;          // tileTotal + 1 (exclusive scan of tileCount)
 	int32_t *tileFill;            // tileTotal
 	int32_t *tileList;            // pair capacity
 	int32_t pairCapacity;
+	int32_t *tileQueue;           // tiles with at least one command (counters[3] of them)
+	int32_t *queueHead;           // next queue slot for persistent raster workgroups
+	int32_t tileW, tileH;
 	int32_t alphaAny;             // any task uses the Alpha filter: tiles read colour first
 };
 
@@ -93,7 +97,9 @@ __global__ void k_bin_count(FrameDev f);
 __global__ void k_scan_tiles_local(FrameDev f);
 __global__ void k_scan_tiles_add(FrameDev f);
 __global__ void k_bin_fill(FrameDev f);
-__global__ void k_raster(FrameDev f);
+template <int TW, int TH> __global__ void k_raster(FrameDev f, int32_t persistent);
+__global__ void k_fill(int32_t *p, int32_t n, int32_t value);
+__global__ void k_clear_canvases(FrameDev f, uint32_t color, float depth);
 __global__ void k_downsample(uint32_t *texels, TextureDesc t, uint32_t targetLevel);
 
 constexpr int32_t SCAN_BLOCK = 256;

@@ -175,7 +175,7 @@ def value_noise(rng_seed: int, lattice: int, xs: np.ndarray, zs: np.ndarray) -> 
 
 
 def terrain_heights(n: int, seed: int, octaves=6, amplitude=40.0, lacunarity=2.0, gain=0.5,
-                    base_period=64.0) -> np.ndarray:
+                    base_period=64.0, lattice=256) -> np.ndarray:
     """fBm of value noise in [0,1), centred and scaled so heights span about +-amplitude/2."""
     zs, xs = np.meshgrid(np.arange(n, dtype=np.float64), np.arange(n, dtype=np.float64), indexing="ij")
     total = np.zeros((n, n))
@@ -183,7 +183,7 @@ def terrain_heights(n: int, seed: int, octaves=6, amplitude=40.0, lacunarity=2.0
     amp = 1.0
     freq = 1.0 / base_period
     for o in range(octaves):
-        total += amp * value_noise(seed * 1000 + o + 1, 256, xs * freq, zs * freq)
+        total += amp * value_noise(seed * 1000 + o + 1, lattice, xs * freq, zs * freq)
         norm += amp
         amp *= gain
         freq *= lacunarity
@@ -300,3 +300,57 @@ def single_triangles(width, height, tris_screen, colors=None, depth=2.0, texture
     return Scene(name=name, width=width, height=height, points=pts, indices=idx, colors=cols,
                  texcoords=tex, parts=parts, textures=textures, cam_location=identity_transform(),
                  width_slope=1.0, far_clip=100.0, seed=seed)
+
+
+# ---------------------------------------------------------------------------
+# Config 2: thousands of independent jobs with uneven canvases and triangle counts.
+# ---------------------------------------------------------------------------
+
+def orbit_transform(distance: float, yaw: float, pitch: float) -> np.ndarray:
+    """Camera on a sphere of `distance` around the origin, looking at it."""
+    cy, sy = math.cos(yaw), math.sin(yaw)
+    cp, sp = math.cos(pitch), math.sin(pitch)
+    forward = np.array([-sy * cp, -sp, -cy * cp])
+    position = -forward * distance
+    return look_transform(position, math.atan2(forward[0], forward[2]), math.asin(-forward[1]))
+
+
+def batch_jobs(count: int = 4096, seed: int = 2, min_tris: int = 100, max_tris: int = 20000,
+               texture_count: int = 16) -> List[Scene]:
+    """Uneven jobs: canvas width U[64,640] and height U[48,480] rounded to multiples of 4;
+    triangle count log-uniform in [min_tris, max_tris]; half are random-cube soups (as config 0)
+    and half terrain patches of n x n points with 2(n-1)^2 close to the drawn count (a 32x32
+    patch is 1,922 triangles); half of the jobs sample a 256x256 RGBA8 texture with a full mip
+    chain (one of `texture_count` shared textures); each camera orbits its object."""
+    rng = Rng(seed)
+    textures = [noise_checker_texture(256, seed * 100 + t, 9, check=16 + 8 * (t % 4)) for t in range(texture_count)]
+    jobs = []
+    for j in range(count):
+        w = int(rng.integers(1, 64, 641)[0]) // 4 * 4
+        h = int(rng.integers(1, 48, 481)[0]) // 4 * 4
+        tris = int(round(math.exp(float(rng.uniform(1, math.log(min_tris), math.log(max_tris))[0]))))
+        terrain_job = bool(rng.integers(1, 0, 2)[0])
+        textured = bool(rng.integers(1, 0, 2)[0])
+        tex_id = int(rng.integers(1, 0, texture_count)[0])
+        yaw = float(rng.uniform(1, 0.0, 2.0 * math.pi)[0])
+        pitch = float(rng.uniform(1, 0.15, 1.2)[0])
+        job_seed = seed * 1000003 + j
+        if terrain_job:
+            n = max(4, int(round(math.sqrt(tris / 2.0))) + 1)
+            heights = terrain_heights(n, job_seed, octaves=4, amplitude=0.3 * n, base_period=max(4.0, n / 2.0), lattice=32)
+            pts, quad, colors, tex = terrain_mesh(n, heights, uv_repeat=2.0)
+            sc = Scene(name=f"job{j}", width=w, height=h, points=pts, indices=quad, colors=colors, texcoords=tex,
+                       parts=[Part(0, quad.shape[0], diffuse=0 if textured else -1)],
+                       textures=[textures[tex_id]] if textured else [],
+                       cam_location=orbit_transform(0.9 * n, yaw, pitch), width_slope=1.0, seed=job_seed)
+        else:
+            sc = cube_soup(width=w, height=h, triangles=tris, seed=job_seed, name=f"job{j}")
+            sc.cam_location = orbit_transform(2.5, yaw, pitch * 0.5)
+            if textured:
+                r2 = Rng(job_seed + 77)
+                sc.texcoords[:, :3, :2] = r2.uniform(tris * 6, -1.0, 2.0).reshape(tris, 3, 2)
+                sc.parts = [Part(0, tris, diffuse=0)]
+                sc.textures = [textures[tex_id]]
+        sc.texture_ids = [tex_id] if textured else []
+        jobs.append(sc)
+    return jobs

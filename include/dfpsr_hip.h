@@ -134,8 +134,58 @@ int dfpsr_hip_renderer_takes_triangles(const dfpsr_hip_renderer *renderer, int32
  * draws every given triangle and returns when the canvases hold the result (host canvases
  * are copied back).  Fails with DFPSR_HIP_ERROR_STATE without renderer_begin. */
 int dfpsr_hip_renderer_end(dfpsr_hip_renderer *renderer);
-/* Statistics of the last renderer_end: draw commands, (tile, command) pairs, tiles. */
+/* Statistics of the last renderer_end: draw commands, (tile, command) pairs, tiles, passes. */
 int dfpsr_hip_renderer_stats(const dfpsr_hip_renderer *renderer, int64_t out[4]);
+/* Per-stage device timing with HIP events on the renderer's stream (no counterpart in the
+ * reference, which has no profiler hooks).  When enabled, renderer_end records the device
+ * milliseconds of its last pass: [0] projection, [1] triangle setup, [2] tile binning,
+ * [3] tile raster, [4] the whole pass. */
+int dfpsr_hip_renderer_set_timing(dfpsr_hip_renderer *renderer, int32_t enable);
+int dfpsr_hip_renderer_timing(const dfpsr_hip_renderer *renderer, float out[5]);
+
+/* ---- Tuning and batched rendering (no counterpart in the reference) --------------------- */
+
+/* How a frame is executed on the device.  tile_config picks the screen tile of the raster
+ * kernel (0: 16x16, 1: 32x16, 2: 32x32 pixels); persistent_per_cu = 0 gives every non-empty
+ * tile its own workgroup, k > 0 launches k workgroups per CU that pull tiles from a queue;
+ * use_graph = 1 captures the launch sequence into a hipGraph and replays it while the work
+ * is unchanged; clear = 1 fills the colour canvas with clear_color and the depth canvas with
+ * clear_depth first (the caller's image_fill).  Results never depend on these settings. */
+typedef struct {
+	int32_t tile_config;
+	int32_t persistent_per_cu;
+	int32_t use_graph;
+	int32_t clear;
+	uint32_t clear_color;
+	float clear_depth;
+} dfpsr_hip_tuning;
+
+/* Tuning used by renderer_end (default: 16x16 tiles, one workgroup per tile, no graph). */
+int dfpsr_hip_renderer_set_tuning(dfpsr_hip_renderer *renderer, const dfpsr_hip_tuning *tuning);
+
+/* One independent render job: what renderer_begin(color, depth) + renderer_giveTask(model,
+ * transform, camera) + renderer_end does, on device canvases (on_device = 1). */
+typedef struct {
+	dfpsr_hip_image color, depth;
+	const dfpsr_hip_model *model;
+	float transform[12];
+	dfpsr_hip_camera camera;
+} dfpsr_hip_job;
+
+typedef struct dfpsr_hip_batch dfpsr_hip_batch;
+
+int dfpsr_hip_batch_create(dfpsr_hip_batch **out);
+void dfpsr_hip_batch_destroy(dfpsr_hip_batch *batch);
+/* Draws every job in one launch sequence over all jobs' triangles and tiles, returning when
+ * every canvas holds its image.  Each image is identical to the one the plain renderer draws
+ * for that job alone.  A call with the same job array as the previous call reuses the
+ * uploaded tables (and, with use_graph, replays the captured graph). */
+int dfpsr_hip_batch_render(dfpsr_hip_batch *batch, const dfpsr_hip_job *jobs, int32_t job_count,
+                           const dfpsr_hip_tuning *tuning);
+/* Statistics and timing of the last dfpsr_hip_batch_render, as for the renderer. */
+int dfpsr_hip_batch_stats(const dfpsr_hip_batch *batch, int64_t out[4]);
+int dfpsr_hip_batch_set_timing(dfpsr_hip_batch *batch, int32_t enable);
+int dfpsr_hip_batch_timing(const dfpsr_hip_batch *batch, float out[5]);
 
 #ifdef __cplusplus
 }

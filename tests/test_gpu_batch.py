@@ -1,0 +1,73 @@
+"""The batched form: many independent jobs in one launch sequence.  Every job's image must be
+identical to what the plain per-job renderer draws, for every tuning (tile shape, persistent
+tile queue, graph replay), and match the CPU oracle to the parity bar."""
+import numpy as np
+import pytest
+
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
+from dfpsr_amd import scenes as S
+from oracle import harness as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import torch
+    assert torch.cuda.is_available()
+    from dfpsr_amd import api
+    jobs = S.batch_jobs(48, seed=5, max_tris=6000)
+    g = api.GpuJobs(jobs)
+    r = api.renderer_create()
+    g.clear()
+    g.render_plain(r)
+    torch.cuda.synchronize()
+    plain = [g.image(i) for i in range(len(jobs))]
+    return api, jobs, g, plain
+
+
+TUNINGS = [(0, 0, False), (1, 0, False), (2, 0, False), (0, 4, False), (2, 2, True), (0, 8, True)]
+
+
+@pytest.mark.parametrize("tile_config,persistent,graph", TUNINGS)
+def test_batch_equals_plain_loop(setup, tile_config, persistent, graph):
+    api, jobs, g, plain = setup
+    print(f"[gpu] batch tile_config={tile_config} persistent={persistent} graph={graph}", flush=True)
+    b = api.Batch()
+    t = api.tuning(tile_config, persistent, graph, clear=True)
+    for rep in range(2 if graph else 1):  # a replayed graph must give the same images
+        g.clear()
+        b.render(g.jobs, t)
+        for i in range(len(jobs)):
+            c, d = g.image(i)
+            assert np.array_equal(c, plain[i][0]), f"job {i} colour (rep {rep})"
+            assert np.array_equal(d.view(np.uint32), plain[i][1].view(np.uint32)), f"job {i} depth (rep {rep})"
+    assert b.stats()[0] > 0
+
+
+def test_batch_jobs_match_oracle(setup):
+    api, jobs, g, plain = setup
+    for i in range(0, len(jobs), 6):
+        rc, rd = H.oracle_render(jobs[i])
+        c, d = plain[i]
+        assert np.array_equal(d.view(np.uint32), rd.view(np.uint32)), f"job {i} depth"
+        a = c.view(np.uint8).reshape(-1, 4).astype(int)
+        b = rc.view(np.uint8).reshape(-1, 4).astype(int)
+        bad = np.any(np.abs(a - b) > 1, axis=1)
+        assert bad.sum() <= int(0.001 * bad.size), f"job {i}: {bad.sum()} pixels beyond 1 LSB"
+
+
+def test_renderer_tuning_does_not_change_results(setup):
+    api, jobs, g, plain = setup
+    sc = S.terrain(width=400, height=300, grid=64, texture_size=256, resolutions=9)
+    ref = api.render_scene(sc)
+    for cfg in [(1, 0, False), (2, 4, True), (0, 2, True)]:
+        r = api.renderer_create()
+        api.renderer_setTuning(r, api.tuning(*cfg))
+        for _ in range(2):
+            got = api.render_scene(sc, r)
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1].view(np.uint32), ref[1].view(np.uint32)), cfg
