@@ -45,6 +45,12 @@ class Job(C.Structure):
                 ("camera", Camera)]
 
 
+class View(C.Structure):
+    _fields_ = [("color", C.c_void_p), ("visibility", C.c_void_p), ("depth", C.c_void_p), ("grad_color", C.c_void_p),
+                ("width", C.c_int32), ("height", C.c_int32), ("stride_pixels", C.c_int32),
+                ("transform", C.c_float * 12), ("camera", Camera)]
+
+
 _lib = None
 
 _P = C.c_void_p
@@ -82,6 +88,13 @@ _SIGNATURES = {
     "dfpsr_hip_batch_stats": ([_P, _P], C.c_int),
     "dfpsr_hip_batch_set_timing": ([_P, _I], C.c_int),
     "dfpsr_hip_batch_timing": ([_P, _P], C.c_int),
+    "dfpsr_hip_diff_create": ([C.POINTER(_P)], C.c_int),
+    "dfpsr_hip_diff_destroy": ([_P], None),
+    "dfpsr_hip_diff_forward": ([_P, _P, C.POINTER(View), _I, _P, _I, _I, _P, C.POINTER(Tuning)], C.c_int),
+    "dfpsr_hip_diff_backward": ([_P, _P, _P, C.POINTER(Tuning)], C.c_int),
+    "dfpsr_hip_diff_stats": ([_P, _P], C.c_int),
+    "dfpsr_hip_diff_set_timing": ([_P, _I], C.c_int),
+    "dfpsr_hip_diff_timing": ([_P, _P], C.c_int),
 }
 
 

@@ -10,7 +10,7 @@ from typing import Optional
 
 import numpy as np
 
-from ._lib import Camera, DfpsrError, Image, Job, Tuning, check, lib
+from ._lib import Camera, DfpsrError, Image, Job, Tuning, View, check, lib
 
 Filter_Solid = 0
 Filter_Alpha = 1
@@ -356,6 +356,84 @@ class GpuJobs:
             renderer_begin(renderer, j.color, j.depth)
             check(lib().dfpsr_hip_renderer_give_model(renderer.handle, j.model, j.transform, C.byref(j.camera)))
             renderer_end(renderer)
+
+
+# ---------------------------------------------------------------------------------------
+# Differentiable rendering
+# ---------------------------------------------------------------------------------------
+
+class DiffRenderer:
+    """dfpsr_hip_diff: forward with the differentiable float shader, adjoint with visibility
+    fixed from the forward.  Buffers are torch tensors on the GPU."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().dfpsr_hip_diff_create(C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().dfpsr_hip_diff_destroy(self._h)
+            self._h = None
+
+    def forward(self, model: Model, views, texture, multipliers=None, t: Optional[Tuning] = None) -> None:
+        import math
+        t = t or tuning(clear=True)
+        h, w = texture.shape[0], texture.shape[1]
+        assert texture.is_contiguous() and texture.shape[2] == 4
+        check(lib().dfpsr_hip_diff_forward(self._h, model.handle, views, len(views), texture.data_ptr(),
+                                           int(math.log2(w)), int(math.log2(h)),
+                                           multipliers.data_ptr() if multipliers is not None else None, C.byref(t)))
+
+    def backward(self, grad_texture=None, grad_multipliers=None, t: Optional[Tuning] = None) -> None:
+        t = t or tuning()
+        check(lib().dfpsr_hip_diff_backward(self._h, grad_texture.data_ptr() if grad_texture is not None else None,
+                                            grad_multipliers.data_ptr() if grad_multipliers is not None else None,
+                                            C.byref(t)))
+
+    def set_timing(self, enable: bool) -> None:
+        check(lib().dfpsr_hip_diff_set_timing(self._h, int(enable)))
+
+    def timing(self):
+        out = np.zeros(5, np.float32)
+        check(lib().dfpsr_hip_diff_timing(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def stats(self):
+        out = np.zeros(4, np.int64)
+        check(lib().dfpsr_hip_diff_stats(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+class GpuViews:
+    """Views of one model for the differentiable renderer: the model (points, triangles, uv)
+    and per view colour / visibility / depth / gradient buffers as torch tensors."""
+
+    def __init__(self, view_scenes):
+        import torch
+        first = view_scenes[0]
+        self.model = model_create(0)
+        model_setPoints(self.model, first.points)
+        model_addPart(self.model, first.indices, first.colors, first.texcoords)
+        self.point_count = first.points.shape[0]
+        n = len(view_scenes)
+        self.views = (View * n)()
+        self.color, self.visibility, self.depth, self.grad = [], [], [], []
+        for i, sc in enumerate(view_scenes):
+            col = torch.zeros((sc.height, sc.width, 4), dtype=torch.float32, device="cuda")
+            vis = torch.full((sc.height, sc.width), -1, dtype=torch.int32, device="cuda")
+            dep = torch.zeros((sc.height, sc.width), dtype=torch.float32, device="cuda")
+            grad = torch.zeros((sc.height, sc.width, 4), dtype=torch.float32, device="cuda")
+            v = self.views[i]
+            v.color, v.visibility, v.depth, v.grad_color = col.data_ptr(), vis.data_ptr(), dep.data_ptr(), grad.data_ptr()
+            v.width, v.height, v.stride_pixels = sc.width, sc.height, sc.width
+            v.transform[:] = [float(x) for x in np.asarray(sc.instances[0], dtype=np.float32)]
+            v.camera = camera_create(sc.perspective, sc.cam_location, sc.width, sc.height, sc.width_slope,
+                                     sc.near_clip, sc.far_clip)
+            self.color.append(col)
+            self.visibility.append(vis)
+            self.depth.append(dep)
+            self.grad.append(grad)
 
 
 __all__ = [n for n in dir() if not n.startswith("_")] + ["DfpsrError"]

@@ -511,3 +511,103 @@ extern "C" int dfpsr_hip_batch_timing(const dfpsr_hip_batch *b, float out[5]) {
 	for (int i = 0; i < 5; i++) out[i] = b->frame.stageMs[i];
 	return DFPSR_HIP_OK;
 }
+
+// ---------------------------------------------------------------------------------------
+// Differentiable rendering
+// ---------------------------------------------------------------------------------------
+
+struct dfpsr_hip_diff {
+	Frame frame;
+	std::vector<dfpsr_hip_view> lastViews;
+	const dfpsr_hip_model *lastModel = nullptr;
+};
+
+extern "C" int dfpsr_hip_diff_create(dfpsr_hip_diff **out) {
+	if (!out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "diff_create: null output");
+	std::unique_ptr<dfpsr_hip_diff> d(new (std::nothrow) dfpsr_hip_diff());
+	if (!d) return fail(DFPSR_HIP_ERROR_MEMORY, "diff_create: out of host memory");
+	int rc = d->frame.init();
+	if (rc) return rc;
+	*out = d.release();
+	return DFPSR_HIP_OK;
+}
+
+extern "C" void dfpsr_hip_diff_destroy(dfpsr_hip_diff *d) { delete d; }
+
+extern "C" int dfpsr_hip_diff_forward(dfpsr_hip_diff *d, const dfpsr_hip_model *model, const dfpsr_hip_view *views, int32_t count,
+                                      const float *texture, int32_t log2w, int32_t log2h, const float *multipliers,
+                                      const dfpsr_hip_tuning *tuning) {
+	if (!d || !model || count < 0 || (count > 0 && !views) || !texture)
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, "diff_forward: bad argument");
+	if (log2w < 0 || log2w > 15 || log2h < 0 || log2h > 15) return fail(DFPSR_HIP_ERROR_ARGUMENT, "diff_forward: texture size");
+	int rc = checkTuning(tuning);
+	if (rc) return rc;
+	RenderOptions o = optionsFrom(tuning);
+	o.tileConfig = 0;
+	o.mode = 1;
+	bool same = d->lastModel == model && !model->dirty && (int32_t)d->lastViews.size() == count &&
+	            (count == 0 || std::memcmp(d->lastViews.data(), views, (size_t)count * sizeof(dfpsr_hip_view)) == 0);
+	d->frame.setDifferentiable(texture, log2w, log2h, multipliers);
+	if (!same) {
+		d->frame.reset(0);
+		rc = const_cast<dfpsr_hip_model *>(model)->upload();
+		if (rc) return rc;
+		for (int32_t i = 0; i < count; i++) {
+			const dfpsr_hip_view &v = views[i];
+			if (!v.color || !v.visibility || !v.depth || v.width < 1 || v.height < 1 || v.stride_pixels < v.width)
+				return fail(DFPSR_HIP_ERROR_ARGUMENT, "diff_forward: a view needs colour, visibility and depth buffers");
+			rc = checkCamera(&v.camera);
+			if (rc) return rc;
+			CanvasHost cv;
+			cv.color = nullptr;
+			cv.depth = v.depth;
+			cv.width = v.width;
+			cv.height = v.height;
+			cv.colorStride = 0;
+			cv.depthStride = v.stride_pixels;
+			cv.diffColor = v.color;
+			cv.visibility = v.visibility;
+			cv.gradColor = v.grad_color;
+			cv.diffStride = v.stride_pixels;
+			int32_t slot = d->frame.addCanvas(cv);
+			CameraParams cam = cameraFrom(&v.camera);
+			if (boxSeen(cam, model->minBound, model->maxBound, transformFrom(v.transform)) == 0) continue;
+			rc = d->frame.addTask(model->view(), v.transform, cam, slot, false);
+			if (rc) return rc;
+		}
+		rc = d->frame.prepare();
+		if (rc) return rc;
+		d->lastViews.assign(views, views + count);
+		d->lastModel = model;
+	}
+	rc = d->frame.run(o);
+	if (rc) d->lastViews.clear();
+	return rc;
+}
+
+extern "C" int dfpsr_hip_diff_backward(dfpsr_hip_diff *d, float *gradTexture, float *gradMultipliers, const dfpsr_hip_tuning *tuning) {
+	if (!d) return fail(DFPSR_HIP_ERROR_ARGUMENT, "diff_backward: null argument");
+	int rc = checkTuning(tuning);
+	if (rc) return rc;
+	for (const dfpsr_hip_view &v : d->lastViews)
+		if (!v.grad_color) return fail(DFPSR_HIP_ERROR_ARGUMENT, "diff_backward: every view needs grad_color");
+	return d->frame.backward(gradTexture, gradMultipliers, optionsFrom(tuning));
+}
+
+extern "C" int dfpsr_hip_diff_stats(const dfpsr_hip_diff *d, int64_t out[4]) {
+	if (!d || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "diff_stats: null argument");
+	for (int i = 0; i < 4; i++) out[i] = d->frame.stats[i];
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_diff_set_timing(dfpsr_hip_diff *d, int32_t enable) {
+	if (!d) return fail(DFPSR_HIP_ERROR_ARGUMENT, "diff_set_timing: null argument");
+	d->frame.timing = enable != 0;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_diff_timing(const dfpsr_hip_diff *d, float out[5]) {
+	if (!d || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "diff_timing: null argument");
+	for (int i = 0; i < 5; i++) out[i] = d->frame.stageMs[i];
+	return DFPSR_HIP_OK;
+}

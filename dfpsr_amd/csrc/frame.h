@@ -37,6 +37,11 @@ struct CanvasHost {
 	float *depth;
 	int32_t width, height;
 	int32_t colorStride, depthStride;  // in elements
+	// Differentiable canvases (null otherwise): float RGBA, visibility ids, incoming gradient.
+	float *diffColor = nullptr;
+	int32_t *visibility = nullptr;
+	const float *gradColor = nullptr;
+	int32_t diffStride = 0;
 };
 
 // What a task needs of a model: device arrays and the textures of each part.
@@ -60,6 +65,7 @@ struct RenderOptions {
 	bool clear = false;              // fill the canvases before drawing (image_fill)
 	uint32_t clearColor = 0;
 	float clearDepth = 0.0f;
+	int32_t mode = 0;                // 0: the reference's RGBA8 shaders; 1: the differentiable float shader
 };
 
 class Frame {
@@ -85,6 +91,16 @@ public:
 	// Runs the pipeline over the prepared tables on the frame's stream and waits for it.
 	int run(const RenderOptions &options);
 	int32_t canvasCount() const { return (int32_t)canvases_.size(); }
+	// Inputs of the differentiable shader: float RGBA texture (level 0, device) and per-point RGB
+	// multipliers (device, null = all 1).
+	void setDifferentiable(const float *texels, int32_t log2w, int32_t log2h, const float *multipliers) {
+		ftex_ = texels;
+		ftexLog2w_ = log2w;
+		ftexLog2h_ = log2h;
+		multipliers_ = multipliers;
+	}
+	// Adjoint of the last differentiable run: adds into the gradient buffers (either may be null).
+	int backward(float *gradTexture, float *gradMultipliers, const RenderOptions &options);
 
 private:
 	int32_t tileConfig_ = 0;
@@ -110,6 +126,14 @@ private:
 	hipGraphExec_t graphExec_ = nullptr;
 	std::vector<uint8_t> graphKey_;
 	int32_t deviceCUs_ = 0;
+	const float *ftex_ = nullptr;
+	int32_t ftexLog2w_ = 0, ftexLog2h_ = 0;
+	const float *multipliers_ = nullptr;
+	FrameDev lastRun_;
+	bool lastRunValid_ = false;
+	hipGraph_t bwdGraph_ = nullptr;
+	hipGraphExec_t bwdGraphExec_ = nullptr;
+	std::vector<uint8_t> bwdGraphKey_;
 
 	int32_t textureSlot(const TextureDesc *desc);
 	int enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks);

@@ -35,6 +35,11 @@ void Frame::dropGraph() {
 	graphExec_ = nullptr;
 	graph_ = nullptr;
 	graphKey_.clear();
+	if (bwdGraphExec_) (void)hipGraphExecDestroy(bwdGraphExec_);
+	if (bwdGraph_) (void)hipGraphDestroy(bwdGraph_);
+	bwdGraphExec_ = nullptr;
+	bwdGraph_ = nullptr;
+	bwdGraphKey_.clear();
 }
 
 void Frame::reset(int32_t tileConfig) {
@@ -63,6 +68,10 @@ int32_t Frame::addCanvas(const CanvasHost &c) {
 	d.tilesX = (c.width + tw - 1) / tw;
 	d.tilesY = (c.height + th - 1) / th;
 	d.tileBase = canvasTileEnd_.empty() ? 0 : canvasTileEnd_.back();
+	d.diffColor = c.diffColor;
+	d.visibility = c.visibility;
+	d.gradColor = c.gradColor;
+	d.diffStride = c.diffStride;
 	canvases_.push_back(d);
 	canvasTileEnd_.push_back(d.tileBase + d.tilesX * d.tilesY);
 	return (int32_t)canvases_.size() - 1;
@@ -133,7 +142,9 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	if (events) HIP_TRY(hipEventRecord(events_[3], stream));
 	const int32_t persistent = o.persistentPerCU > 0 ? 1 : 0;
 	const int32_t grid = persistent ? deviceCUs_ * o.persistentPerCU : tileTotal;
-	switch (tileConfig_) {
+	if (o.mode == 1) {
+		hipLaunchKernelGGL(k_raster_diff, dim3(grid), dim3(64), 0, stream, f, persistent);
+	} else switch (tileConfig_) {
 	case 1: hipLaunchKernelGGL((k_raster<32, 16>), dim3(grid), dim3(128), 0, stream, f, persistent); break;
 	case 2: hipLaunchKernelGGL((k_raster<32, 32>), dim3(grid), dim3(256), 0, stream, f, persistent); break;
 	default: hipLaunchKernelGGL((k_raster<16, 16>), dim3(grid), dim3(64), 0, stream, f, persistent); break;
@@ -235,6 +246,13 @@ int Frame::run(const RenderOptions &o) {
 		f.tileW = tileConfigW(tileConfig_);
 		f.tileH = tileConfigH(tileConfig_);
 		f.alphaAny = alphaAny_ ? 1 : 0;
+		f.mode = o.mode;
+		f.ftexTexels = ftex_;
+		f.ftexLog2w = ftexLog2w_;
+		f.ftexLog2h = ftexLog2h_;
+		f.multipliers = multipliers_;
+		lastRun_ = f;
+		lastRunValid_ = false;
 
 		if (o.graph) {
 			// The graph stays valid while the launch arguments and options are the same.
@@ -273,13 +291,60 @@ int Frame::run(const RenderOptions &o) {
 			HIP_TRY(hipEventElapsedTime(&stageMs[4], events_[0], events_[4]));
 		}
 		int32_t overflow = hostCounters_[2];
-		if (overflow == 0) return 0;
+		if (overflow == 0) {
+			lastRunValid_ = true;
+			return 0;
+		}
 		if (overflow & 1) commandCapacity_ = hostCounters_[0] + 1024;
 		if (overflow & 2) pairCapacity_ = hostCounters_[1] + 1024;
 		if ((overflow & 1) && pairCapacity_ < 4 * commandCapacity_) pairCapacity_ = 4 * commandCapacity_;
 		dropGraph();
 	}
 	return fail(-4, "renderer_end: buffers did not converge");
+}
+
+int Frame::backward(float *gradTexture, float *gradMultipliers, const RenderOptions &o) {
+	if (!lastRunValid_ || lastRun_.mode != 1) return fail(-2, "backward needs a differentiable forward pass first");
+	FrameDev f = lastRun_;
+	f.gradTexture = gradTexture;
+	f.gradMultipliers = gradMultipliers;
+	int32_t *head = f.counters + 5;
+	const int32_t grid = o.persistentPerCU > 0 ? deviceCUs_ * o.persistentPerCU : f.tileTotal;
+	auto enqueueBackward = [&]() -> int {
+		hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, head, 1, 0);
+		hipLaunchKernelGGL(k_backward, dim3(grid), dim3(256), 0, stream, f, head);
+		HIP_TRY(hipGetLastError());
+		return 0;
+	};
+	if (timing) HIP_TRY(hipEventRecord(events_[0], stream));
+	if (o.graph) {
+		std::vector<uint8_t> key(sizeof(FrameDev) + sizeof(int32_t));
+		std::memcpy(key.data(), &f, sizeof(FrameDev));
+		std::memcpy(key.data() + sizeof(FrameDev), &grid, sizeof(int32_t));
+		if (!bwdGraphExec_ || key != bwdGraphKey_) {
+			if (bwdGraphExec_) (void)hipGraphExecDestroy(bwdGraphExec_);
+			if (bwdGraph_) (void)hipGraphDestroy(bwdGraph_);
+			bwdGraphExec_ = nullptr;
+			bwdGraph_ = nullptr;
+			HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+			int rc = enqueueBackward();
+			hipGraph_t g = nullptr;
+			hipError_t e = hipStreamEndCapture(stream, &g);
+			if (rc) return rc;
+			if (e != hipSuccess) return fail(-3, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+			bwdGraph_ = g;
+			HIP_TRY(hipGraphInstantiate(&bwdGraphExec_, bwdGraph_, nullptr, nullptr, 0));
+			bwdGraphKey_ = key;
+		}
+		HIP_TRY(hipGraphLaunch(bwdGraphExec_, stream));
+	} else {
+		int rc = enqueueBackward();
+		if (rc) return rc;
+	}
+	if (timing) HIP_TRY(hipEventRecord(events_[4], stream));
+	HIP_TRY(hipStreamSynchronize(stream));
+	if (timing) HIP_TRY(hipEventElapsedTime(&stageMs[4], events_[0], events_[4]));
+	return 0;
 }
 
 }  // namespace dfpsr

@@ -251,38 +251,44 @@ __device__ void sortLds(int32_t *a, int32_t n) {
 	}
 }
 
-// The four pixels of a lane's quad, held in registers for the whole tile.
-struct QuadPixels {
+// The four pixels of a lane's quad, held in registers for the whole tile.  MODE 0 holds the
+// RGBA8 colours of the reference's shaders; MODE 1 (differentiable) holds float RGBA and the
+// visibility id (draw command * 4 + quad lane) of the last write.
+template <int MODE> struct QuadPixels;
+template <> struct QuadPixels<0> {
 	uint32_t color[4];
 	float depth[4];
 	uint32_t colorDirty, depthDirty;
 };
+template <> struct QuadPixels<1> {
+	float r[4], g[4], b[4], a[4];
+	int32_t vis[4];
+	float depth[4];
+	uint32_t colorDirty, depthDirty;
+};
 
-__device__ __forceinline__ uint32_t getColor(const QuadPixels &q, int s) {
-	return s == 0 ? q.color[0] : (s == 1 ? q.color[1] : (s == 2 ? q.color[2] : q.color[3]));
+template <typename T> __device__ __forceinline__ T pick4(const T *v, int s) {
+	return s == 0 ? v[0] : (s == 1 ? v[1] : (s == 2 ? v[2] : v[3]));
 }
-__device__ __forceinline__ float getDepth(const QuadPixels &q, int s) {
-	return s == 0 ? q.depth[0] : (s == 1 ? q.depth[1] : (s == 2 ? q.depth[2] : q.depth[3]));
-}
-__device__ __forceinline__ void setColor(QuadPixels &q, int s, uint32_t v) {
-	if (s == 0) q.color[0] = v; else if (s == 1) q.color[1] = v; else if (s == 2) q.color[2] = v; else q.color[3] = v;
-	q.colorDirty |= 1u << s;
-}
-__device__ __forceinline__ void setDepth(QuadPixels &q, int s, float v) {
-	if (s == 0) q.depth[0] = v; else if (s == 1) q.depth[1] = v; else if (s == 2) q.depth[2] = v; else q.depth[3] = v;
-	q.depthDirty |= 1u << s;
+template <typename T> __device__ __forceinline__ void put4(T *v, int s, T x) {
+	if (s == 0) v[0] = x; else if (s == 1) v[1] = x; else if (s == 2) v[2] = x; else v[3] = x;
 }
 
-// Draws one command into the lane's quad: the row-pair logic of fillShapeSuper
-// (fillerTemplates.h:246-385), the incremental weight stepping of fillRowSuper (:195-244)
-// and the visibility / shading / writes of fillQuadSuper (:154-187).
-__device__ void drawCommand(const FrameDev &f, const Command &cmd, const TaskDev &task, const CanvasDev &cv,
-                            const RowInterval *rows, int32_t rowTop, int32_t rowBottom, int32_t tileY0,
-                            int32_t px, int32_t py, QuadPixels &q, bool colorWrite, bool depthRead, bool depthWrite,
-                            bool alphaFilter) {
-	if (py < rowTop || py >= rowBottom) return;
-	RowInterval upper = rows[py - tileY0];
-	RowInterval lower = rows[py + 1 - tileY0];
+// One quad of one command: which lanes the command covers and their weight-plane values,
+// stepped exactly as the reference's loops step them -- the row-pair logic of fillShapeSuper
+// (fillerTemplates.h:246-385), the incremental stepping of fillRowSuper (:195-244) and the
+// side clipping of clipPixels (:92-108).
+struct QuadEval {
+	float l0[4], l1[4], l2[4];  // depth (1/W or W) and the two depth-divided vertex weights per lane
+	int slot[4];                // pixel written by each lane (a missing row aliases the other row)
+	bool clip[4];
+	bool clipSides;
+};
+
+// upper/lower: row intervals of rows py and py + 1 (py even).  Returns false when the command
+// does not touch the quad at px (even).
+__device__ __forceinline__ bool evalQuad(const Command &cmd, RowInterval upper, RowInterval lower, int32_t px, int32_t py,
+                                         int32_t canvasHeight, QuadEval &e) {
 	int32_t outerStart = tmin(upper.left, lower.left);
 	int32_t outerEnd = tmax(upper.right, lower.right);
 	int32_t innerStart = tmax(upper.left, lower.left);
@@ -291,89 +297,102 @@ __device__ void drawCommand(const FrameDev &f, const Command &cmd, const TaskDev
 	int32_t outerBlockEnd = (int32_t)(((uint32_t)outerEnd + 1u) & ~1u);
 	int32_t innerBlockStart = (int32_t)(((uint32_t)innerStart + 1u) & ~1u);
 	int32_t innerBlockEnd = (int32_t)((uint32_t)innerEnd & ~1u);
-	if (py + 1 >= cv.height) lower.right = lower.left;  // the missing last row of an odd canvas
+	if (py + 1 >= canvasHeight) lower.right = lower.left;  // the missing last row of an odd canvas
 	bool hasTop = upper.right > upper.left;
 	bool hasBottom = lower.right > lower.left;
-	if (!(hasTop || hasBottom)) return;
-	if (px < outerBlockStart || px >= outerBlockEnd) return;
+	if (!(hasTop || hasBottom)) return false;
+	if (px < outerBlockStart || px >= outerBlockEnd) return false;
 
-	// Weights of this quad, stepped exactly as the reference's loops step them.
 	const f3 dx = cmd.dx;
 	const f3 dx2 = scale3(dx, 2.0f);
 	f3 wU = weightAtPixel(Projection{cmd.start, cmd.dx, cmd.dy}, outerBlockStart, py);
 	f3 wL = add3(wU, cmd.dy);
-	bool clipSides = true;
-	float l0[4], l1[4], l2v[4];
+	e.clipSides = true;
 	bool splitSpan = innerBlockEnd > innerBlockStart;
-	if (!splitSpan || px < innerBlockStart) {
-		for (int32_t x = outerBlockStart; x < px; x += 2) { wU = add3(wU, dx2); wL = add3(wL, dx2); }
-		l0[0] = wU.x; l0[1] = wU.x + dx.x; l0[2] = wL.x; l0[3] = wL.x + dx.x;
-		l1[0] = wU.y; l1[1] = wU.y + dx.y; l1[2] = wL.y; l1[3] = wL.y + dx.y;
-		l2v[0] = wU.z; l2v[1] = wU.z + dx.z; l2v[2] = wL.z; l2v[3] = wL.z + dx.z;
-	} else {
+	if (splitSpan && px >= innerBlockStart && px < innerBlockEnd) {
+		// Unclipped inner span: lanes start at the span and step per lane.
 		for (int32_t x = outerBlockStart; x < innerBlockStart; x += 2) { wU = add3(wU, dx2); wL = add3(wL, dx2); }
-		if (px < innerBlockEnd) {
-			clipSides = false;
-			l0[0] = wU.x; l0[1] = wU.x + dx.x; l0[2] = wL.x; l0[3] = wL.x + dx.x;
-			l1[0] = wU.y; l1[1] = wU.y + dx.y; l1[2] = wL.y; l1[3] = wL.y + dx.y;
-			l2v[0] = wU.z; l2v[1] = wU.z + dx.z; l2v[2] = wL.z; l2v[3] = wL.z + dx.z;
-			for (int32_t x = innerBlockStart; x < px; x += 2) {
-				for (int l = 0; l < 4; l++) { l0[l] = l0[l] + dx2.x; l1[l] = l1[l] + dx2.y; l2v[l] = l2v[l] + dx2.z; }
-			}
+		e.clipSides = false;
+		e.l0[0] = wU.x; e.l0[1] = wU.x + dx.x; e.l0[2] = wL.x; e.l0[3] = wL.x + dx.x;
+		e.l1[0] = wU.y; e.l1[1] = wU.y + dx.y; e.l1[2] = wL.y; e.l1[3] = wL.y + dx.y;
+		e.l2[0] = wU.z; e.l2[1] = wU.z + dx.z; e.l2[2] = wL.z; e.l2[3] = wL.z + dx.z;
+		for (int32_t x = innerBlockStart; x < px; x += 2) {
+			for (int l = 0; l < 4; l++) { e.l0[l] = e.l0[l] + dx2.x; e.l1[l] = e.l1[l] + dx2.y; e.l2[l] = e.l2[l] + dx2.z; }
+		}
+	} else {
+		if (!splitSpan || px < innerBlockStart) {
+			for (int32_t x = outerBlockStart; x < px; x += 2) { wU = add3(wU, dx2); wL = add3(wL, dx2); }
 		} else {
+			for (int32_t x = outerBlockStart; x < innerBlockStart; x += 2) { wU = add3(wU, dx2); wL = add3(wL, dx2); }
 			float quadCount = (float)((innerBlockEnd - innerBlockStart) / 2);
 			wU = add3(wU, scale3(dx2, quadCount));
 			wL = add3(wL, scale3(dx2, quadCount));
 			for (int32_t x = innerBlockEnd; x < px; x += 2) { wU = add3(wU, dx2); wL = add3(wL, dx2); }
-			l0[0] = wU.x; l0[1] = wU.x + dx.x; l0[2] = wL.x; l0[3] = wL.x + dx.x;
-			l1[0] = wU.y; l1[1] = wU.y + dx.y; l1[2] = wL.y; l1[3] = wL.y + dx.y;
-			l2v[0] = wU.z; l2v[1] = wU.z + dx.z; l2v[2] = wL.z; l2v[3] = wL.z + dx.z;
 		}
+		e.l0[0] = wU.x; e.l0[1] = wU.x + dx.x; e.l0[2] = wL.x; e.l0[3] = wL.x + dx.x;
+		e.l1[0] = wU.y; e.l1[1] = wU.y + dx.y; e.l1[2] = wL.y; e.l1[3] = wL.y + dx.y;
+		e.l2[0] = wU.z; e.l2[1] = wU.z + dx.z; e.l2[2] = wL.z; e.l2[3] = wL.z + dx.z;
 	}
-
-	// Lanes 0-1 use the upper row and 2-3 the lower row; a missing row aliases the other one.
-	const int s0 = hasTop ? 0 : 2, s1 = hasTop ? 1 : 3, s2 = hasBottom ? 2 : 0, s3 = hasBottom ? 3 : 1;
-	const int slot[4] = {s0, s1, s2, s3};
-	bool clip[4];
-	if (clipSides) {
+	e.slot[0] = hasTop ? 0 : 2;
+	e.slot[1] = hasTop ? 1 : 3;
+	e.slot[2] = hasBottom ? 2 : 0;
+	e.slot[3] = hasBottom ? 3 : 1;
+	if (e.clipSides) {
 		int32_t x2 = px + 1;
-		clip[0] = px >= upper.left && px < upper.right;
-		clip[1] = x2 >= upper.left && x2 < upper.right;
-		clip[2] = px >= lower.left && px < lower.right;
-		clip[3] = x2 >= lower.left && x2 < lower.right;
+		e.clip[0] = px >= upper.left && px < upper.right;
+		e.clip[1] = x2 >= upper.left && x2 < upper.right;
+		e.clip[2] = px >= lower.left && px < lower.right;
+		e.clip[3] = x2 >= lower.left && x2 < lower.right;
 	} else {
-		clip[0] = clip[1] = clip[2] = clip[3] = true;
+		e.clip[0] = e.clip[1] = e.clip[2] = e.clip[3] = true;
 	}
-	const bool affine = (cmd.flags & 1) != 0;
-	bool vis[4];
+	return true;
+}
+
+// Vertex weights of a lane (fillRowSuper, fillerTemplates.h:222-238): perspective divides the
+// depth-divided weights by 1/W.
+__device__ __forceinline__ void laneWeights(const QuadEval &e, int l, bool affine, float &wa, float &wb, float &wc) {
+	if (affine) {
+		wb = e.l1[l];
+		wc = e.l2[l];
+	} else {
+		float w = refinedReciprocal(e.l0[l]);
+		wb = e.l1[l] * w;
+		wc = e.l2[l] * w;
+	}
+	wa = 1.0f - (wb + wc);
+}
+
+// Depth test of getVisibility (fillerTemplates.h:110-152).
+template <int MODE>
+__device__ __forceinline__ bool quadVisibility(const QuadEval &e, const QuadPixels<MODE> &q, bool depthRead, bool affine, bool *vis) {
 	bool anyVis = false;
 	for (int l = 0; l < 4; l++) {
 		bool front = true;
 		if (depthRead) {
-			if (clipSides && !clip[l]) {
+			if (e.clipSides && !e.clip[l]) {
 				front = false;
 			} else {
-				float old = getDepth(q, slot[l]);
-				front = affine ? (l0[l] < old) : (l0[l] > old);
+				float old = pick4(q.depth, e.slot[l]);
+				front = affine ? (e.l0[l] < old) : (e.l0[l] > old);
 			}
 		}
-		vis[l] = clip[l] && front;
+		vis[l] = e.clip[l] && front;
 		anyVis |= vis[l];
 	}
-	if (!anyVis) return;
+	return anyVis;
+}
+
+// The reference's shading of one quad (fillQuadSuper, fillerTemplates.h:154-187, with the
+// RgbaMultiply shaders of RgbaMultiply.h:74-175).
+__device__ void drawCommand(const FrameDev &f, const Command &cmd, const TaskDev &task, const QuadEval &e,
+                            QuadPixels<0> &q, bool colorWrite, bool depthRead, bool depthWrite, bool alphaFilter) {
+	const bool affine = (cmd.flags & 1) != 0;
+	bool vis[4];
+	if (!quadVisibility<0>(e, q, depthRead, affine, vis)) return;
 	if (colorWrite) {
 		float wa[4], wb[4], wc[4];
-		for (int l = 0; l < 4; l++) {
-			if (affine) {
-				wb[l] = l1[l];
-				wc[l] = l2v[l];
-			} else {
-				float w = refinedReciprocal(l0[l]);
-				wb[l] = l1[l] * w;
-				wc[l] = l2v[l] * w;
-			}
-			wa[l] = 1.0f - (wb[l] + wc[l]);
-		}
+		for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
 		// Shader inputs of the source triangle (Shader.h:37-57, RgbaMultiply.h:37-58).
 		int32_t fp = task.fanPolygon[cmd.fan];
 		int32_t poly = fp >> 1, k = fp & 1;
@@ -398,7 +417,7 @@ __device__ void drawCommand(const FrameDev &f, const Command &cmd, const TaskDev
 		if (alphaFilter) {
 			float tr[4], tg[4], tb[4], ta[4];
 			for (int l = 0; l < 4; l++) {
-				uint32_t old = (!clipSides || vis[l]) ? getColor(q, slot[l]) : 0u;
+				uint32_t old = (!e.clipSides || vis[l]) ? pick4(q.color, e.slot[l]) : 0u;
 				unpackColor(old, tr[l], tg[l], tb[l], ta[l]);
 			}
 			for (int l = 0; l < 4; l++) {
@@ -411,19 +430,118 @@ __device__ void drawCommand(const FrameDev &f, const Command &cmd, const TaskDev
 			}
 		}
 		for (int l = 0; l < 4; l++) {
-			if (vis[l]) setColor(q, slot[l], packSaturated(r[l], g[l], b[l], a[l]));
+			if (vis[l]) {
+				put4(q.color, e.slot[l], packSaturated(r[l], g[l], b[l], a[l]));
+				q.colorDirty |= 1u << e.slot[l];
+			}
 		}
 	}
 	if (depthWrite) {
 		for (int l = 0; l < 4; l++) {
-			if (vis[l]) setDepth(q, slot[l], l0[l]);
+			if (vis[l]) {
+				put4(q.depth, e.slot[l], e.l0[l]);
+				q.depthDirty |= 1u << e.slot[l];
+			}
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------------------
+// Differentiable shading: colour = interpolated per-point RGB multipliers x bilinear sample
+// of a float RGBA texture (level 0), alpha = the texture's alpha.  The texel choice and the
+// 8-bit sub-texel weights are those of texture_sample_bilinear at the highest resolution
+// (textureAPI.h:342-438); the adjoint below scatters through the same weights.
+// ---------------------------------------------------------------------------------------
+
+struct BilinearTaps {
+	uint32_t index[4];  // texel indices: upper left, upper right, lower left, lower right
+	float weight[4];
+};
+
+DH BilinearTaps bilinearTaps(int32_t log2w, int32_t log2h, float u, float v) {
+	uint32_t scaleU = 256u << log2w, scaleV = 256u << log2h;
+	uint32_t maskX = (1u << log2w) - 1u, maskY = (1u << log2h) - 1u;
+	uint32_t subX = truncU32((u + 256.0f) * fromU32(scaleU)) - 128u;
+	uint32_t subY = truncU32((v + 256.0f) * fromU32(scaleV)) - 128u;
+	float fx = (float)(subX & 255u) * (1.0f / 256.0f), fy = (float)(subY & 255u) * (1.0f / 256.0f);
+	uint32_t left = (subX >> 8) & maskX, top = (subY >> 8) & maskY;
+	uint32_t right = (left + 1u) & maskX, bottom = (top + 1u) & maskY;
+	BilinearTaps t;
+	t.index[0] = (top << log2w) | left;
+	t.index[1] = (top << log2w) | right;
+	t.index[2] = (bottom << log2w) | left;
+	t.index[3] = (bottom << log2w) | right;
+	t.weight[0] = (1.0f - fx) * (1.0f - fy);
+	t.weight[1] = fx * (1.0f - fy);
+	t.weight[2] = (1.0f - fx) * fy;
+	t.weight[3] = fx * fy;
+	return t;
+}
+
+// Inputs of the differentiable shader for one command's source triangle.
+struct DiffInputs {
+	f3 u, v;               // u1 and v1 at the three corners
+	int32_t point[3];      // model point of each corner (per-point multipliers)
+};
+
+__device__ __forceinline__ DiffInputs diffInputs(const TaskDev &task, const Command &cmd) {
+	int32_t fp = task.fanPolygon[cmd.fan];
+	int32_t poly = fp >> 1, k = fp & 1;
+	const float *tc = task.texcoords + poly * 16;
+	const int32_t ca = 0, cb = (1 + k) * 4, cc = (2 + k) * 4;
+	DiffInputs d;
+	d.u = make3(tc[ca], tc[cb], tc[cc]);
+	d.v = make3(tc[ca + 1], tc[cb + 1], tc[cc + 1]);
+	d.point[0] = task.indices[poly * 4];
+	d.point[1] = task.indices[poly * 4 + 1 + k];
+	d.point[2] = task.indices[poly * 4 + 2 + k];
+	return d;
+}
+
+__device__ __forceinline__ float multiplier(const FrameDev &f, int32_t point, int ch) {
+	return f.multipliers ? f.multipliers[point * 3 + ch] : 1.0f;
+}
+
+__device__ void drawCommandDiff(const FrameDev &f, const Command &cmd, int32_t commandIndex, const TaskDev &task,
+                                const QuadEval &e, QuadPixels<1> &q, bool depthRead, bool depthWrite) {
+	const bool affine = (cmd.flags & 1) != 0;
+	bool vis[4];
+	if (!quadVisibility<1>(e, q, depthRead, affine, vis)) return;
+	const DiffInputs d = diffInputs(task, cmd);
+	float m[3][3];
+	for (int c = 0; c < 3; c++)
+		for (int ch = 0; ch < 3; ch++) m[c][ch] = multiplier(f, d.point[c], ch);
+	for (int l = 0; l < 4; l++) {
+		if (!vis[l]) continue;
+		float wa, wb, wc;
+		laneWeights(e, l, affine, wa, wb, wc);
+		float u = interp(d.u, wa, wb, wc), v = interp(d.v, wa, wb, wc);
+		BilinearTaps taps = bilinearTaps(f.ftexLog2w, f.ftexLog2h, u, v);
+		float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+		for (int k = 0; k < 4; k++) {
+			const float4 texel = *(const float4 *)(f.ftexTexels + (size_t)taps.index[k] * 4);
+			t[0] += taps.weight[k] * texel.x;
+			t[1] += taps.weight[k] * texel.y;
+			t[2] += taps.weight[k] * texel.z;
+			t[3] += taps.weight[k] * texel.w;
+		}
+		const int s = e.slot[l];
+		put4(q.r, s, (wa * m[0][0] + wb * m[1][0] + wc * m[2][0]) * t[0]);
+		put4(q.g, s, (wa * m[0][1] + wb * m[1][1] + wc * m[2][1]) * t[1]);
+		put4(q.b, s, (wa * m[0][2] + wb * m[1][2] + wc * m[2][2]) * t[2]);
+		put4(q.a, s, t[3]);
+		put4(q.vis, s, commandIndex * 4 + l);
+		q.colorDirty |= 1u << s;
+		if (depthWrite) {
+			put4(q.depth, s, e.l0[l]);
+			q.depthDirty |= 1u << s;
 		}
 	}
 }
 
 // One tile: load the lanes' pixels, draw every binned command in increasing command index
 // (submission order), write back the pixels that changed.
-template <int TW, int TH>
+template <int TW, int TH, int MODE>
 __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowInterval *rows, int32_t *roundInfo) {
 	constexpr int32_t NL = (TW / 2) * (TH / 2);
 	const int32_t n = f.tileCount[tile];
@@ -437,13 +555,13 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 	const int32_t py = tileY0 + 2 * (lane / (TW / 2));
 	const int32_t *src = f.tileList + f.tileOffset[tile];
 
-	QuadPixels q;
+	QuadPixels<MODE> q;
 	q.colorDirty = q.depthDirty = 0;
 	for (int s = 0; s < 4; s++) {
 		int32_t x = px + (s & 1), y = py + (s >> 1);
 		bool in = x < cv.width && y < cv.height;
 		q.depth[s] = (in && cv.depth) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
-		q.color[s] = (in && cv.color && f.alphaAny) ? cv.color[(size_t)y * cv.colorStride + x] : 0u;
+		if constexpr (MODE == 0) q.color[s] = (in && cv.color && f.alphaAny) ? cv.color[(size_t)y * cv.colorStride + x] : 0u;
 	}
 
 	// Commands in increasing index, in rounds of at most TILE_SORT_CAP: each round takes the
@@ -491,9 +609,9 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 			const Command cmd = f.commands[c];
 			const TaskDev &task = f.tasks[cmd.task];
 			const bool solid = task.filter == 0;
-			const bool colorWrite = cv.color != nullptr;
+			const bool colorWrite = MODE == 1 || cv.color != nullptr;
 			const bool depthRead = cv.depth != nullptr;
-			const bool depthWrite = cv.depth != nullptr && (solid || cv.color == nullptr);
+			const bool depthWrite = cv.depth != nullptr && (solid || !colorWrite);
 			const bool alphaFilter = colorWrite && !solid;
 			// Rows of this command inside the tile: rasterizeTriangle over the aligned bound.
 			Rect wb = triangleBound(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]});
@@ -511,7 +629,11 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 				}
 			}
 			__syncthreads();
-			drawCommand(f, cmd, task, cv, rows, al.t, al.bottom(), tileY0, px, py, q, colorWrite, depthRead, depthWrite, alphaFilter);
+			QuadEval e;
+			if (py >= al.t && py < al.bottom() && evalQuad(cmd, rows[py - tileY0], rows[py + 1 - tileY0], px, py, cv.height, e)) {
+				if constexpr (MODE == 0) drawCommand(f, cmd, task, e, q, colorWrite, depthRead, depthWrite, alphaFilter);
+				else drawCommandDiff(f, cmd, c, task, e, q, depthRead, depthWrite);
+			}
 		}
 		done += take;
 		if (take == 0) break;
@@ -520,7 +642,15 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 	for (int s = 0; s < 4; s++) {
 		int32_t x = px + (s & 1), y = py + (s >> 1);
 		if (x >= cv.width || y >= cv.height) continue;
-		if ((q.colorDirty >> s) & 1u) cv.color[(size_t)y * cv.colorStride + x] = q.color[s];
+		if ((q.colorDirty >> s) & 1u) {
+			if constexpr (MODE == 0) {
+				cv.color[(size_t)y * cv.colorStride + x] = q.color[s];
+			} else {
+				size_t p = (size_t)y * cv.diffStride + x;
+				*(float4 *)(cv.diffColor + p * 4) = make_float4(q.r[s], q.g[s], q.b[s], q.a[s]);
+				cv.visibility[p] = q.vis[s];
+			}
+		}
 		if ((q.depthDirty >> s) & 1u) cv.depth[(size_t)y * cv.depthStride + x] = q.depth[s];
 	}
 }
@@ -531,6 +661,7 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 // workgroup reaches the exit once all tiles are handed out.
 template <int TW, int TH>
 __global__ void __launch_bounds__((TW / 2) * (TH / 2)) k_raster(FrameDev f, int32_t persistent) {
+	if (f.mode == 1) return;  // the differentiable frames use k_raster_diff
 	__shared__ int32_t list[TILE_SORT_CAP];
 	__shared__ RowInterval rows[TH];
 	__shared__ int32_t roundInfo[3];
@@ -546,7 +677,7 @@ __global__ void __launch_bounds__((TW / 2) * (TH / 2)) k_raster(FrameDev f, int3
 			__syncthreads();
 		}
 		if (slot >= queued) break;
-		rasterTile<TW, TH>(f, f.tileQueue[slot], list, rows, roundInfo);
+		rasterTile<TW, TH, 0>(f, f.tileQueue[slot], list, rows, roundInfo);
 		if (!persistent) break;
 		__syncthreads();
 	}
@@ -555,6 +686,102 @@ __global__ void __launch_bounds__((TW / 2) * (TH / 2)) k_raster(FrameDev f, int3
 template __global__ void k_raster<16, 16>(FrameDev, int32_t);
 template __global__ void k_raster<32, 16>(FrameDev, int32_t);
 template __global__ void k_raster<32, 32>(FrameDev, int32_t);
+
+// The differentiable forward: the same tile walk with the float shader, persistent or not.
+__global__ void __launch_bounds__(64) k_raster_diff(FrameDev f, int32_t persistent) {
+	__shared__ int32_t list[TILE_SORT_CAP];
+	__shared__ RowInterval rows[16];
+	__shared__ int32_t roundInfo[3];
+	__shared__ int32_t slotLds;
+	if (f.counters[2]) return;
+	const int32_t queued = f.counters[3];
+	int32_t slot = blockIdx.x;
+	while (true) {
+		if (persistent) {
+			if (threadIdx.x == 0) slotLds = atomicAdd(f.queueHead, 1);
+			__syncthreads();
+			slot = slotLds;
+			__syncthreads();
+		}
+		if (slot >= queued) break;
+		rasterTile<16, 16, 1>(f, f.tileQueue[slot], list, rows, roundInfo);
+		if (!persistent) break;
+		__syncthreads();
+	}
+}
+
+// The adjoint of the differentiable forward with visibility fixed: each pixel re-derives the
+// interpolants of the (command, lane) that drew it -- the same rows and the same weight
+// stepping as the forward -- and scatters dL/dcolour into the four bilinear texels and the
+// three corners' RGB multipliers with float atomics.  Workgroups pull tiles from a queue
+// (all tiles of all views) until it is empty.
+__global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
+	__shared__ int32_t tileLds;
+	while (true) {
+		if (threadIdx.x == 0) tileLds = atomicAdd(head, 1);
+		__syncthreads();
+		const int32_t tile = tileLds;
+		__syncthreads();
+		if (tile >= f.tileTotal) break;
+		const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
+		const CanvasDev cv = f.canvases[ci];
+		const int32_t local = tile - cv.tileBase;
+		const int32_t x = (local % cv.tilesX) * f.tileW + (int32_t)(threadIdx.x % 16);
+		const int32_t y = (local / cv.tilesX) * f.tileH + (int32_t)(threadIdx.x / 16);
+		if (x >= cv.width || y >= cv.height) continue;
+		const size_t p = (size_t)y * cv.diffStride + x;
+		const int32_t id = cv.visibility[p];
+		if (id < 0) continue;
+		const int32_t c = id >> 2, lane = id & 3;
+		const Command cmd = f.commands[c];
+		const TaskDev &task = f.tasks[cmd.task];
+		Rect wb = triangleBound(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]});
+		Rect full = rectSides(0, 0, cv.width, cv.height);
+		Rect clipR = cutRect(wb, full);
+		Rect al = alignedRasterBound(wb, full);
+		Rect rowClip = rectSides(clipR.l, al.t, clipR.right(), al.bottom());
+		l2 A{cmd.fx[0], cmd.fy[0]}, B{cmd.fx[1], cmd.fy[1]}, C{cmd.fx[2], cmd.fy[2]};
+		bool degenerate = sameCorner(A, B) || sameCorner(B, C) || sameCorner(C, A);
+		EdgeSetup es[3] = {edgeSetup(A, B, rowClip), edgeSetup(B, C, rowClip), edgeSetup(C, A, rowClip)};
+		const int32_t py = y & ~1, px = x & ~1;
+		RowInterval upper = rowInterval(es, degenerate, rowClip, py - al.t);
+		RowInterval lower = rowInterval(es, degenerate, rowClip, py + 1 - al.t);
+		QuadEval e;
+		if (!evalQuad(cmd, upper, lower, px, py, cv.height, e)) continue;
+		const bool affine = (cmd.flags & 1) != 0;
+		float wa, wbw, wc;
+		laneWeights(e, lane, affine, wa, wbw, wc);
+		const DiffInputs d = diffInputs(task, cmd);
+		float u = interp(d.u, wa, wbw, wc), v = interp(d.v, wa, wbw, wc);
+		BilinearTaps taps = bilinearTaps(f.ftexLog2w, f.ftexLog2h, u, v);
+		float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+		for (int k = 0; k < 4; k++) {
+			const float4 texel = *(const float4 *)(f.ftexTexels + (size_t)taps.index[k] * 4);
+			t[0] += taps.weight[k] * texel.x;
+			t[1] += taps.weight[k] * texel.y;
+			t[2] += taps.weight[k] * texel.z;
+			t[3] += taps.weight[k] * texel.w;
+		}
+		const float4 g = *(const float4 *)(cv.gradColor + p * 4);
+		const float w3[3] = {wa, wbw, wc};
+		float mixed[3];
+		for (int ch = 0; ch < 3; ch++)
+			mixed[ch] = wa * multiplier(f, d.point[0], ch) + wbw * multiplier(f, d.point[1], ch) + wc * multiplier(f, d.point[2], ch);
+		if (f.gradTexture) {
+			const float gt[4] = {g.x * mixed[0], g.y * mixed[1], g.z * mixed[2], g.w};
+			for (int k = 0; k < 4; k++) {
+				if (taps.weight[k] == 0.0f) continue;
+				float *dst = f.gradTexture + (size_t)taps.index[k] * 4;
+				for (int ch = 0; ch < 4; ch++) unsafeAtomicAdd(dst + ch, gt[ch] * taps.weight[k]);
+			}
+		}
+		if (f.gradMultipliers) {
+			const float gm[3] = {g.x * t[0], g.y * t[1], g.z * t[2]};
+			for (int c3 = 0; c3 < 3; c3++)
+				for (int ch = 0; ch < 3; ch++) unsafeAtomicAdd(f.gradMultipliers + d.point[c3] * 3 + ch, gm[ch] * w3[c3]);
+		}
+	}
+}
 
 __global__ void k_fill(int32_t *p, int32_t n, int32_t value) {
 	int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -574,6 +801,11 @@ __global__ void k_clear_canvases(FrameDev f, uint32_t color, float depth) {
 		if (x >= cv.width || y >= cv.height) continue;
 		if (cv.color) cv.color[(size_t)y * cv.colorStride + x] = color;
 		if (cv.depth) cv.depth[(size_t)y * cv.depthStride + x] = depth;
+		if (cv.diffColor) {
+			size_t p = (size_t)y * cv.diffStride + x;
+			*(float4 *)(cv.diffColor + p * 4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			cv.visibility[p] = -1;
+		}
 	}
 }
 

@@ -41,6 +41,12 @@ struct CanvasDev {
 	int32_t colorStride, depthStride;  // in elements
 	int32_t tileBase;                  // first tile of this canvas in the frame's tile numbering
 	int32_t tilesX, tilesY;
+	// Differentiable frames: float RGBA colour, visibility id (command * 4 + lane or -1) and the
+	// incoming gradient of the colour, all with diffStride pixels per row.
+	float *diffColor;
+	int32_t *visibility;
+	const float *gradColor;
+	int32_t diffStride;
 };
 
 // One draw command: a (sub-)triangle that passed culling, clipping and the front-face test.
@@ -77,6 +83,14 @@ struct FrameDev {
 	int32_t *queueHead;           // next queue slot for persistent raster workgroups
 	int32_t tileW, tileH;
 	int32_t alphaAny;             // any task uses the Alpha filter: tiles read colour first
+	// Differentiable frames (mode 1): float RGBA texture (level 0), per-point RGB multipliers
+	// and the gradient buffers of the adjoint.
+	int32_t mode;
+	const float *ftexTexels;
+	int32_t ftexLog2w, ftexLog2h;
+	const float *multipliers;
+	float *gradTexture;
+	float *gradMultipliers;
 };
 
 DH int32_t findSegment(const int32_t *inclusiveEnd, int32_t count, int32_t index) {
@@ -98,6 +112,8 @@ __global__ void k_scan_tiles_local(FrameDev f);
 __global__ void k_scan_tiles_add(FrameDev f);
 __global__ void k_bin_fill(FrameDev f);
 template <int TW, int TH> __global__ void k_raster(FrameDev f, int32_t persistent);
+__global__ void k_raster_diff(FrameDev f, int32_t persistent);
+__global__ void k_backward(FrameDev f, int32_t *head);
 __global__ void k_fill(int32_t *p, int32_t n, int32_t value);
 __global__ void k_clear_canvases(FrameDev f, uint32_t color, float depth);
 __global__ void k_downsample(uint32_t *texels, TextureDesc t, uint32_t targetLevel);

@@ -354,3 +354,118 @@ def batch_jobs(count: int = 4096, seed: int = 2, min_tris: int = 100, max_tris: 
         sc.texture_ids = [tex_id] if textured else []
         jobs.append(sc)
     return jobs
+
+
+# ---------------------------------------------------------------------------
+# Configs 3 and 4: a displaced geodesic sphere seen from many views.
+# ---------------------------------------------------------------------------
+
+def look_at(position, target=(0.0, 0.0, 0.0), world_up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    """Camera location at `position` looking at `target`: x right, y up, z forward."""
+    p = np.asarray(position, dtype=np.float64)
+    forward = np.asarray(target, dtype=np.float64) - p
+    forward /= np.linalg.norm(forward)
+    up = np.asarray(world_up, dtype=np.float64)
+    if abs(np.dot(up, forward)) > 0.999:
+        up = np.array([0.0, 0.0, 1.0])
+    right = np.cross(up, forward)
+    right /= np.linalg.norm(right)
+    up = np.cross(forward, right)
+    return np.concatenate([p, right, up, forward]).astype(np.float32)
+
+
+def fibonacci_directions(count: int, seed: int = 0) -> np.ndarray:
+    """`count` near-uniform unit directions (Fibonacci sphere), rotated by a seeded angle."""
+    k = np.arange(count, dtype=np.float64) + 0.5
+    y = 1.0 - 2.0 * k / count
+    r = np.sqrt(np.maximum(0.0, 1.0 - y * y))
+    phi = k * math.pi * (3.0 - math.sqrt(5.0)) + float(Rng(seed).uniform(1, 0.0, 2.0 * math.pi)[0])
+    return np.stack([r * np.cos(phi), y, r * np.sin(phi)], axis=1)
+
+
+def geodesic_sphere(frequency: int, seed: int, amplitude: float = 0.1, waves: int = 12):
+    """Icosahedron whose faces are split into frequency^2 triangles (20 * frequency^2 in all),
+    pushed to the unit sphere and displaced radially by amplitude * a seeded sum of `waves`
+    random plane waves.  Face-edge vertices are duplicated per face (same position).
+    Returns points (P, 3), triangles (T, 3) and per-corner uv (T, 3, 2)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                      [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    faces = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6],
+             [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+             [8, 6, 7], [9, 8, 1]]
+    n = frequency
+    ii, jj = [], []
+    for i in range(n + 1):
+        for j in range(n + 1 - i):
+            ii.append(i)
+            jj.append(j)
+    ii = np.array(ii)
+    jj = np.array(jj)
+    index = -np.ones((n + 1, n + 1), dtype=np.int64)
+    index[ii, jj] = np.arange(len(ii))
+    tri_local = []
+    for i in range(n):
+        for j in range(n - i):
+            tri_local.append((index[i, j], index[i + 1, j], index[i, j + 1]))
+            if i + j < n - 1:
+                tri_local.append((index[i + 1, j], index[i + 1, j + 1], index[i, j + 1]))
+    tri_local = np.array(tri_local, dtype=np.int64)
+    pts, tris = [], []
+    for f, (a, b, c) in enumerate(faces):
+        A, B, C = verts[a], verts[b], verts[c]
+        p = A[None, :] + (B - A)[None, :] * (ii / n)[:, None] + (C - A)[None, :] * (jj / n)[:, None]
+        tris.append(tri_local + f * len(ii))
+        pts.append(p)
+    pts = np.concatenate(pts)
+    tris = np.concatenate(tris)
+    d = pts / np.linalg.norm(pts, axis=1, keepdims=True)
+    rng = Rng(seed)
+    k = rng.uniform(waves * 3, -1.0, 1.0).reshape(waves, 3).astype(np.float64) * 6.0
+    phase = rng.uniform(waves, 0.0, 2.0 * math.pi).astype(np.float64)
+    noise = np.sin(d @ k.T + phase[None, :]).sum(axis=1) / math.sqrt(waves)
+    noise /= max(1e-9, np.abs(noise).max())
+    pts = d * (1.0 + amplitude * noise)[:, None]
+    u = np.arctan2(d[:, 0], d[:, 2]) / (2.0 * math.pi) + 0.5
+    v = np.arccos(np.clip(d[:, 1], -1.0, 1.0)) / math.pi
+    uv = np.stack([u[tris], v[tris]], axis=-1)
+    # Triangles across the u seam: lift the small u values by one lap (the texture repeats).
+    span = uv[:, :, 0].max(axis=1) - uv[:, :, 0].min(axis=1)
+    seam = span > 0.5
+    uv[:, :, 0] = np.where(seam[:, None] & (uv[:, :, 0] < 0.5), uv[:, :, 0] + 1.0, uv[:, :, 0])
+    # The icosahedron's faces wind counter-clockwise seen from outside, which the screen's
+    # downward y turns into the clockwise order ITriangle2D::isFrontfacing keeps.
+    return pts.astype(np.float32), tris.astype(np.int32), uv.astype(np.float32)
+
+
+def sphere_views(frequency: int, views: int, width: int, height: int, seed: int = 3, radius: float = 3.0,
+                 fov_deg: float = 60.0, name: str = "sphere") -> List[Scene]:
+    """One Scene per view of the displaced geodesic sphere, cameras on a Fibonacci sphere."""
+    pts, tris, uv = geodesic_sphere(frequency, seed)
+    m = tris.shape[0]
+    idx = np.full((m, 4), -1, dtype=np.int32)
+    idx[:, :3] = tris
+    tex = np.zeros((m, 4, 4), dtype=np.float32)
+    tex[:, :3, :2] = uv
+    colors = np.ones((m, 4, 4), dtype=np.float32)
+    dirs = fibonacci_directions(views, seed)
+    out = []
+    for k in range(views):
+        cam = look_at(dirs[k] * radius)
+        out.append(Scene(name=f"{name}_view{k}", width=width, height=height, points=pts, indices=idx, colors=colors,
+                         texcoords=tex, parts=[Part(0, m)], textures=[], cam_location=cam,
+                         width_slope=float(np.float32(math.tan(math.radians(fov_deg) / 2))), near_clip=0.01,
+                         far_clip=100.0, seed=seed))
+    return out
+
+
+def float_texture(size: int, seed: int, smooth: int = 8) -> np.ndarray:
+    """A seeded RGBA float32 texture (size x size x 4) in [0, 1): blurred noise plus stripes."""
+    rng = Rng(seed)
+    base = rng.uniform((size // smooth) * (size // smooth) * 4).reshape(size // smooth, size // smooth, 4)
+    up = np.repeat(np.repeat(base, smooth, axis=0), smooth, axis=1)
+    yy, xx = np.meshgrid(np.arange(size), np.arange(size), indexing="ij")
+    stripes = 0.5 + 0.5 * np.sin(2 * math.pi * (xx + 2 * yy) / (size / 8.0))
+    out = 0.6 * up + 0.4 * stripes[..., None]
+    out[..., 3] = 1.0
+    return out.astype(np.float32)

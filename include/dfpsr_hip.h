@@ -187,6 +187,47 @@ int dfpsr_hip_batch_stats(const dfpsr_hip_batch *batch, int64_t out[4]);
 int dfpsr_hip_batch_set_timing(dfpsr_hip_batch *batch, int32_t enable);
 int dfpsr_hip_batch_timing(const dfpsr_hip_batch *batch, float out[5]);
 
+/* ---- Differentiable rendering (no counterpart in the reference) ------------------------- */
+
+/* One view of a differentiable frame.  All buffers are device memory with stride_pixels
+ * pixels per row: color = RGBA float32, visibility = the draw command * 4 + quad lane that
+ * wrote each pixel (-1 where nothing was drawn), depth = the depth canvas, grad_color =
+ * dL/dcolor (read by dfpsr_hip_diff_backward, may be NULL for the forward). */
+typedef struct {
+	float *color;
+	int32_t *visibility;
+	float *depth;
+	const float *grad_color;
+	int32_t width, height;
+	int32_t stride_pixels;
+	float transform[12];
+	dfpsr_hip_camera camera;
+} dfpsr_hip_view;
+
+typedef struct dfpsr_hip_diff dfpsr_hip_diff;
+
+int dfpsr_hip_diff_create(dfpsr_hip_diff **out);
+void dfpsr_hip_diff_destroy(dfpsr_hip_diff *diff);
+/* Forward: draws the model's triangles into every view with the reference's coverage, depth
+ * test and perspective-correct weights, shading colour.rgb = (per-point RGB multipliers
+ * interpolated by the vertex weights) x bilinear(texture, u1, v1) and colour.a = the
+ * texture's alpha.  The bilinear taps and 8-bit sub-texel weights are those of
+ * texture_sample_bilinear at the highest resolution (api/textureAPI.h:342-438).
+ * texture: device RGBA float32, (1 << log2h) rows of (1 << log2w) texels; multipliers:
+ * device, point_count x 3 floats (NULL = all 1).  With tuning->clear the views are cleared
+ * first (colour 0, depth clear_depth, visibility -1). */
+int dfpsr_hip_diff_forward(dfpsr_hip_diff *diff, const dfpsr_hip_model *model, const dfpsr_hip_view *views,
+                           int32_t view_count, const float *texture, int32_t log2w, int32_t log2h,
+                           const float *multipliers, const dfpsr_hip_tuning *tuning);
+/* Adjoint of the last forward with its visibility fixed: adds dL/dtexture (texture layout) and
+ * dL/dmultipliers (point_count x 3) from every view's grad_color, recomputing each pixel's
+ * interpolants.  Either output may be NULL.  Float atomics: sums are order-dependent. */
+int dfpsr_hip_diff_backward(dfpsr_hip_diff *diff, float *grad_texture, float *grad_multipliers,
+                            const dfpsr_hip_tuning *tuning);
+int dfpsr_hip_diff_stats(const dfpsr_hip_diff *diff, int64_t out[4]);
+int dfpsr_hip_diff_set_timing(dfpsr_hip_diff *diff, int32_t enable);
+int dfpsr_hip_diff_timing(const dfpsr_hip_diff *diff, float out[5]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -134,6 +134,45 @@ def load_oracle():
     return lib
 
 
+def oracle_diff_forward(scene, texture: np.ndarray, multipliers=None):
+    """Differentiable float shader on the oracle's raster: (colour HxWx4, visibility HxW,
+    depth HxW, weights HxWx5)."""
+    lib = load_oracle()
+    lib.oracle_diff_forward.argtypes = [C.POINTER(SceneDesc), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.oracle_diff_forward.restype = C.c_int32
+    d, keep = make_desc(scene, False)
+    tex = np.ascontiguousarray(texture, dtype=np.float32)
+    mul = None if multipliers is None else np.ascontiguousarray(multipliers, dtype=np.float32)
+    h, w = scene.height, scene.width
+    color = np.zeros((h, w, 4), np.float32)
+    vis = np.zeros((h, w), np.int32)
+    depth = np.zeros((h, w), np.float32)
+    weights = np.zeros((h, w, 5), np.float32)
+    rc = lib.oracle_diff_forward(C.byref(d), ptr(tex), int(tex.shape[1]).bit_length() - 1, int(tex.shape[0]).bit_length() - 1,
+                                 None if mul is None else ptr(mul), ptr(color), ptr(vis), ptr(depth), ptr(weights))
+    if rc < 0:
+        raise RuntimeError("oracle_diff_forward failed")
+    return color, vis, depth, weights
+
+
+def oracle_diff_backward(scene, texture, multipliers, visibility, weights, grad_color, point_count):
+    """Adjoint of oracle_diff_forward in float64: (dL/dtexture, dL/dmultipliers)."""
+    lib = load_oracle()
+    lib.oracle_diff_backward.argtypes = [C.POINTER(SceneDesc), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    d, keep = make_desc(scene, False)
+    tex = np.ascontiguousarray(texture, dtype=np.float32)
+    mul = None if multipliers is None else np.ascontiguousarray(multipliers, dtype=np.float32)
+    gt = np.zeros(tex.shape, np.float64)
+    gm = np.zeros((point_count, 3), np.float64)
+    lib.oracle_diff_backward(C.byref(d), ptr(tex), int(tex.shape[1]).bit_length() - 1, int(tex.shape[0]).bit_length() - 1,
+                             None if mul is None else ptr(mul), ptr(np.ascontiguousarray(visibility, np.int32)),
+                             ptr(np.ascontiguousarray(weights, np.float32)), ptr(np.ascontiguousarray(grad_color, np.float32)),
+                             ptr(gt), ptr(gm))
+    return gt, gm
+
+
 def ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 

@@ -375,6 +375,7 @@ struct TriangleInput {
 	V3 u1, v1, u2, v2;
 	V3 red, green, blue, alpha;
 	int32_t diffuse, light;  // texture index or -1
+	int32_t point[3];        // model points of the corners (differentiable shader's multipliers)
 };
 
 struct Command {
@@ -770,6 +771,19 @@ inline uint32_t packSaturated(float r, float g, float b, float a) {
 	return truncU32(minps(r, 255.1f)) | (truncU32(minps(g, 255.1f)) << 8) | (truncU32(minps(b, 255.1f)) << 16) | (truncU32(minps(a, 255.1f)) << 24);
 }
 
+// Inputs and outputs of the differentiable float shader (no reference counterpart): colour =
+// interpolated per-point RGB multipliers x bilinear(float RGBA texture, level 0), alpha = the
+// texture's alpha; taps and 8-bit sub-texel weights as texture_sample_bilinear at the highest
+// resolution (textureAPI.h:342-438).
+struct DiffTarget {
+	const float *texture;
+	int32_t log2w, log2h;
+	const float *multipliers;  // point x 3, or null for all 1
+	float *color;              // W*H*4
+	int32_t *visibility;       // W*H: command * 4 + lane, or -1
+	float *weights;            // W*H*5: wa, wb, wc, u, v of the lane that drew the pixel
+};
+
 struct FillContext {
 	const Canvas *canvas;
 	const std::vector<Texture> *textures;
@@ -777,7 +791,33 @@ struct FillContext {
 	ShaderChoice shader;
 	bool colorWrite, depthRead, depthWrite, alphaFilter, affine;
 	int32_t maxHeight;
+	const DiffTarget *diff;
+	int32_t commandIndex;
 };
+
+struct Taps { uint32_t index[4]; float weight[4]; };
+
+Taps bilinearTaps(int32_t log2w, int32_t log2h, float u, float v) {
+	uint32_t scaleU = 256u << log2w, scaleV = 256u << log2h;
+	uint32_t maskX = (1u << log2w) - 1u, maskY = (1u << log2h) - 1u;
+	uint32_t subX = truncU32((u + 256.0f) * fromU32(scaleU)) - 128u;
+	uint32_t subY = truncU32((v + 256.0f) * fromU32(scaleV)) - 128u;
+	float fx = (float)(subX & 255u) * (1.0f / 256.0f), fy = (float)(subY & 255u) * (1.0f / 256.0f);
+	uint32_t left = (subX >> 8) & maskX, top = (subY >> 8) & maskY;
+	uint32_t right = (left + 1u) & maskX, bottom = (top + 1u) & maskY;
+	Taps t;
+	t.index[0] = (top << log2w) | left;
+	t.index[1] = (top << log2w) | right;
+	t.index[2] = (bottom << log2w) | left;
+	t.index[3] = (bottom << log2w) | right;
+	t.weight[0] = (1.0f - fx) * (1.0f - fy);
+	t.weight[1] = fx * (1.0f - fy);
+	t.weight[2] = (1.0f - fx) * fy;
+	t.weight[3] = fx * fy;
+	return t;
+}
+
+inline float multiplierOf(const DiffTarget &d, int32_t point, int ch) { return d.multipliers ? d.multipliers[point * 3 + ch] : 1.0f; }
 
 // fillQuadSuper (fillerTemplates.h:154-187) for the quad whose upper-left pixel is (x, yUpper).
 // rowUpper/rowLower are where the lanes 0-1 and 2-3 read and write (the lower lanes alias
@@ -811,6 +851,29 @@ void fillQuad(const FillContext &f, bool clipSides, int32_t x, int32_t rowUpper,
 		vis[l] = clip[l] && front;
 	}
 	if (!(vis[0] || vis[1] || vis[2] || vis[3])) return;
+	if (f.diff) {
+		const DiffTarget &d = *f.diff;
+		const TriangleInput &in = *f.input;
+		for (int l = 0; l < 4; l++) {
+			if (!vis[l]) continue;
+			float wa = 1.0f - (wb[l] + wc[l]);
+			float u = interp(in.u1, wa, wb[l], wc[l]), v = interp(in.v1, wa, wb[l], wc[l]);
+			Taps taps = bilinearTaps(d.log2w, d.log2h, u, v);
+			float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+			for (int k = 0; k < 4; k++)
+				for (int ch = 0; ch < 4; ch++) t[ch] += taps.weight[k] * d.texture[(size_t)taps.index[k] * 4 + ch];
+			size_t p = (size_t)py[l] * cv.width + px[l];
+			for (int ch = 0; ch < 3; ch++)
+				d.color[p * 4 + ch] = (wa * multiplierOf(d, in.point[0], ch) + wb[l] * multiplierOf(d, in.point[1], ch) +
+				                       wc[l] * multiplierOf(d, in.point[2], ch)) * t[ch];
+			d.color[p * 4 + 3] = t[3];
+			d.visibility[p] = f.commandIndex * 4 + l;
+			d.weights[p * 5 + 0] = wa; d.weights[p * 5 + 1] = wb[l]; d.weights[p * 5 + 2] = wc[l];
+			d.weights[p * 5 + 3] = u; d.weights[p * 5 + 4] = v;
+			if (f.depthWrite) cv.depth[p] = depth[l];
+		}
+		return;
+	}
 	if (f.colorWrite) {
 		float wa[4];
 		for (int l = 0; l < 4; l++) wa[l] = 1.0f - (wb[l] + wc[l]);
@@ -918,6 +981,7 @@ void fillShape(const FillContext &f, const Projection &proj, int32_t startRow, i
 struct Frame {
 	Camera camera;
 	Canvas canvas;
+	const DiffTarget *diff = nullptr;
 	int32_t filter;
 	std::vector<Texture> textures;
 	std::vector<TriangleInput> inputs;
@@ -925,7 +989,7 @@ struct Frame {
 };
 
 // executeTriangleDrawing (renderCore.cpp:203-217) with the shader dispatch of fillShape (fillerTemplates.h:387-441).
-void executeCommand(const Frame &fr, const Command &cmd, const Rect &slice) {
+void executeCommand(const Frame &fr, const Command &cmd, int32_t commandIndex, const Rect &slice) {
 	const Canvas &cv = fr.canvas;
 	Rect full = rectFromSides(0, 0, cv.width, cv.height);
 	Rect clip = cut(full, slice);
@@ -949,14 +1013,22 @@ void executeCommand(const Frame &fr, const Command &cmd, const Rect &slice) {
 	f.depthWrite = hasDepth && (solid || !hasColor);
 	f.alphaFilter = hasColor && !solid;
 	f.maxHeight = cv.height;
-	if (!hasColor && !hasDepth) return;
+	f.diff = fr.diff;
+	f.commandIndex = commandIndex;
+	if (fr.diff) {
+		f.colorWrite = false;
+		f.alphaFilter = false;
+		f.depthWrite = hasDepth;
+	} else if (!hasColor && !hasDepth) {
+		return;
+	}
 	fillShape(f, proj, bound.t, rowCount, rows.data());
 }
 
 // renderTriangleFromData + renderTriangleWithShader + drawClippedTriangle + drawSubTriangle
 // (renderCore.cpp:220-341): culling, clipping and the commands a triangle adds to the queue.
 void giveTriangle(Frame &fr, const Projected &a, const Projected &b, const Projected &c, const TriangleInput &in, int32_t source) {
-	if (fr.canvas.color == nullptr && fr.canvas.depth == nullptr) return;
+	if (fr.canvas.color == nullptr && fr.canvas.depth == nullptr && fr.diff == nullptr) return;
 	Tri tri = makeTri(a, b, c);
 	if (triangleVisibility(tri, fr.camera.cull) == Hidden) return;
 	if (fr.filter != 0 && almostZero3(in.alpha)) return;
@@ -1035,6 +1107,7 @@ void giveModel(Frame &fr, const scene_desc *s, const Transform &modelToWorld, in
 				in.u1 = V3{TA[0], TB[0], TC[0]}; in.v1 = V3{TA[1], TB[1], TC[1]};
 				in.u2 = V3{TA[2], TB[2], TC[2]}; in.v2 = V3{TA[3], TB[3], TC[3]};
 				in.diffuse = diffuse; in.light = light;
+				in.point[0] = idx[ca]; in.point[1] = idx[cb]; in.point[2] = idx[cc];
 				giveTriangle(fr, proj[(size_t)idx[ca]], proj[(size_t)idx[cb]], proj[(size_t)idx[cc]], in, sourceCounter);
 				sourceCounter++;
 			}
@@ -1078,7 +1151,7 @@ void giveAll(const scene_desc *s, Frame &fr) {
 void executeAll(const Frame &fr, int32_t threads) {
 	Rect full = rectFromSides(0, 0, fr.canvas.width, fr.canvas.height);
 	if (threads <= 1) {
-		for (const Command &cmd : fr.commands) executeCommand(fr, cmd, full);
+		for (size_t i = 0; i < fr.commands.size(); i++) executeCommand(fr, fr.commands[i], (int32_t)i, full);
 		return;
 	}
 	const int32_t jobCount = 12;
@@ -1095,7 +1168,7 @@ void executeAll(const Frame &fr, int32_t threads) {
 	for (int32_t w = 0; w < workers; w++) {
 		pool.emplace_back([&, w]() {
 			for (int32_t j = w; j < jobCount; j += workers)
-				for (const Command &cmd : fr.commands) executeCommand(fr, cmd, slices[j]);
+				for (size_t i = 0; i < fr.commands.size(); i++) executeCommand(fr, fr.commands[i], (int32_t)i, slices[j]);
 		});
 	}
 	for (auto &t : pool) t.join();
@@ -1160,6 +1233,73 @@ uint32_t oracle_texture_query(int32_t log2w, int32_t log2h, int32_t maxMip, int3
 }
 
 uint32_t oracle_mix_colors(uint32_t a, uint32_t b, uint32_t weight) { return mixColors(a, b, weight); }
+
+int32_t oracle_diff_forward(const scene_desc *s, const float *texture, int32_t log2w, int32_t log2h, const float *multipliers,
+                            float *outColor, int32_t *outVisibility, float *outDepth, float *outWeights) {
+	if (s->width < 1 || s->height < 1 || !s->has_depth) return -1;
+	size_t n = (size_t)s->width * (size_t)s->height;
+	DiffTarget d;
+	d.texture = texture; d.log2w = log2w; d.log2h = log2h; d.multipliers = multipliers;
+	d.color = outColor; d.visibility = outVisibility; d.weights = outWeights;
+	for (size_t i = 0; i < n; i++) {
+		outVisibility[i] = -1;
+		for (int ch = 0; ch < 4; ch++) outColor[i * 4 + ch] = 0.0f;
+		for (int k = 0; k < 5; k++) outWeights[i * 5 + k] = 0.0f;
+	}
+	Frame fr;
+	std::vector<uint32_t> color;
+	std::vector<float> depth;
+	buildFrame(s, fr, color, depth);
+	fr.canvas.color = nullptr;
+	fr.diff = &d;
+	loadCanvas(s, fr);
+	giveAll(s, fr);
+	executeAll(fr, 1);
+	memcpy(outDepth, depth.data(), n * 4);
+	return (int32_t)fr.commands.size();
+}
+
+// Adjoint of oracle_diff_forward in double precision, from the recorded weights of each pixel.
+void oracle_diff_backward(const scene_desc *s, const float *texture, int32_t log2w, int32_t log2h, const float *multipliers,
+                          const int32_t *visibility, const float *weights, const float *gradColor, double *gradTexture,
+                          double *gradMultipliers) {
+	Frame fr;
+	std::vector<uint32_t> color;
+	std::vector<float> depth;
+	buildFrame(s, fr, color, depth);
+	DiffTarget d;
+	d.texture = texture; d.log2w = log2w; d.log2h = log2h; d.multipliers = multipliers;
+	fr.diff = &d;
+	giveAll(s, fr);
+	size_t n = (size_t)s->width * (size_t)s->height;
+	for (size_t p = 0; p < n; p++) {
+		int32_t id = visibility[p];
+		if (id < 0) continue;
+		const TriangleInput &in = fr.inputs[fr.commands[(size_t)(id >> 2)].input];
+		double w[3] = {weights[p * 5], weights[p * 5 + 1], weights[p * 5 + 2]};
+		Taps taps = bilinearTaps(log2w, log2h, weights[p * 5 + 3], weights[p * 5 + 4]);
+		double t[4] = {0, 0, 0, 0};
+		for (int k = 0; k < 4; k++)
+			for (int ch = 0; ch < 4; ch++) t[ch] += (double)taps.weight[k] * texture[(size_t)taps.index[k] * 4 + ch];
+		double mixed[3];
+		for (int ch = 0; ch < 3; ch++) {
+			mixed[ch] = 0.0;
+			for (int c = 0; c < 3; c++) mixed[ch] += w[c] * multiplierOf(d, in.point[c], ch);
+		}
+		const float *g = gradColor + p * 4;
+		if (gradTexture) {
+			for (int k = 0; k < 4; k++) {
+				double *dst = gradTexture + (size_t)taps.index[k] * 4;
+				for (int ch = 0; ch < 3; ch++) dst[ch] += (double)g[ch] * mixed[ch] * taps.weight[k];
+				dst[3] += (double)g[3] * taps.weight[k];
+			}
+		}
+		if (gradMultipliers) {
+			for (int c = 0; c < 3; c++)
+				for (int ch = 0; ch < 3; ch++) gradMultipliers[(size_t)in.point[c] * 3 + ch] += (double)g[ch] * t[ch] * w[c];
+		}
+	}
+}
 
 uint32_t oracle_pack_saturated(float r, float g, float b, float a) { return packSaturated(r, g, b, a); }
 

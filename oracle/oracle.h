@@ -35,6 +35,17 @@ uint32_t oracle_texture_query(int32_t log2w, int32_t log2h, int32_t max_mip, int
                               int32_t op, uint32_t x, uint32_t y, uint32_t mip, float u, float v);
 /* texture_interpolate_color_linear of one pixel pair (textureAPI.h:302-315). */
 uint32_t oracle_mix_colors(uint32_t a, uint32_t b, uint32_t weight);
+/* Differentiable float shader on the same raster (no reference counterpart): colour.rgb =
+ * interpolated per-point RGB multipliers x bilinear(texture), colour.a = texture alpha.
+ * Writes colour (W*H*4), visibility (command * 4 + lane, -1 elsewhere), depth and the weights
+ * wa, wb, wc, u, v of each drawn pixel (W*H*5).  Returns the command count. */
+int32_t oracle_diff_forward(const scene_desc *scene, const float *texture, int32_t log2w, int32_t log2h,
+                            const float *multipliers, float *out_color, int32_t *out_visibility, float *out_depth,
+                            float *out_weights);
+/* Its adjoint in double precision: adds dL/dtexture and dL/dmultipliers from grad_color. */
+void oracle_diff_backward(const scene_desc *scene, const float *texture, int32_t log2w, int32_t log2h,
+                          const float *multipliers, const int32_t *visibility, const float *weights,
+                          const float *grad_color, double *grad_texture, double *grad_multipliers);
 /* packOrder_floatToSaturatedByte for RGBA (PackOrder.h:191-212). */
 uint32_t oracle_pack_saturated(float r, float g, float b, float a);
 void oracle_rasterize(const int64_t *corners, int32_t left, int32_t top, int32_t width, int32_t height, int32_t *out_rows);

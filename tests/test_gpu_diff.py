@@ -1,0 +1,215 @@
+"""The differentiable form: forward on the reference raster with a float shader, adjoint with
+visibility fixed from the forward.
+
+Checks: the forward against the CPU oracle (visibility and depth bit-exact, colour to float
+rounding); the adjoint against the oracle's float64 adjoint built on the same coverage and
+against the dot-product identity <g, F(T)> = <F^T g, T> (the colour is linear in the texture
+and in the multipliers); central finite differences on the full 8-view 1280x720 fit; the Adam
+fit itself; and the 256-view batched form against the per-view form.
+"""
+import numpy as np
+import pytest
+
+try:
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+from dfpsr_amd import scenes as S
+from oracle import harness as H
+
+pytestmark = pytest.mark.gpu
+
+# Relative tolerances written into the checks below.
+FORWARD_RTOL = 1e-5          # colour vs oracle: weights differ only by the reciprocal estimate
+ADJOINT_RTOL = 1e-3          # GPU float32 atomics vs the oracle's float64 adjoint
+FD_RTOL = 1e-3               # analytic gradient vs central finite differences (h = 1e-3)
+BATCHED_GRAD_RTOL = 1e-4     # 256-view batched adjoint vs the sum of per-view adjoints
+
+
+@pytest.fixture(scope="module")
+def api():
+    assert torch is not None and torch.cuda.is_available()
+    from dfpsr_amd import api as A
+    A.lib()
+    return A
+
+
+class RenderFunction(torch.autograd.Function if torch is not None else object):
+    """torch.autograd wrapper of the C ABI, for these tests only."""
+
+    @staticmethod
+    def forward(ctx, texture, multipliers, renderer, gv, tuning):
+        renderer.forward(gv.model, gv.views, texture, multipliers, tuning)
+        ctx.renderer, ctx.gv, ctx.tuning = renderer, gv, tuning
+        ctx.shapes = (texture.shape, multipliers.shape)
+        return torch.stack([c.clone() for c in gv.color])
+
+    @staticmethod
+    def backward(ctx, grad):
+        gv = ctx.gv
+        for i, g in enumerate(gv.grad):
+            g.copy_(grad[i])
+        gt = torch.zeros(ctx.shapes[0], dtype=torch.float32, device="cuda")
+        gm = torch.zeros(ctx.shapes[1], dtype=torch.float32, device="cuda")
+        ctx.renderer.backward(gt, gm, ctx.tuning)
+        return gt, gm, None, None, None
+
+
+def small_views():
+    return S.sphere_views(12, 3, 320, 180, seed=4)
+
+
+def test_forward_matches_oracle(api):
+    views = small_views()
+    gv = api.GpuViews(views)
+    tex = S.float_texture(64, 1)
+    mul = (1.0 + 0.3 * S.Rng(8).uniform(views[0].points.shape[0] * 3, -1, 1)).reshape(-1, 3).astype(np.float32)
+    r = api.DiffRenderer()
+    r.forward(gv.model, gv.views, torch.from_numpy(tex).cuda(), torch.from_numpy(mul).cuda(), api.tuning(clear=True))
+    torch.cuda.synchronize()
+    for i, sc in enumerate(views):
+        col, vis, dep, w = H.oracle_diff_forward(sc, tex, mul)
+        assert np.array_equal(gv.visibility[i].cpu().numpy(), vis), f"view {i} visibility"
+        assert np.array_equal(gv.depth[i].cpu().numpy().view(np.uint32), dep.view(np.uint32)), f"view {i} depth"
+        np.testing.assert_allclose(gv.color[i].cpu().numpy(), col, rtol=FORWARD_RTOL, atol=1e-6)
+        assert (vis >= 0).mean() > 0.3
+
+
+def test_adjoint_matches_oracle_and_dot_product(api):
+    views = small_views()
+    gv = api.GpuViews(views)
+    tex = S.float_texture(64, 2)
+    mul = (1.0 + 0.3 * S.Rng(9).uniform(views[0].points.shape[0] * 3, -1, 1)).reshape(-1, 3).astype(np.float32)
+    r = api.DiffRenderer()
+    tex_d, mul_d = torch.from_numpy(tex).cuda(), torch.from_numpy(mul).cuda()
+    r.forward(gv.model, gv.views, tex_d, mul_d, api.tuning(clear=True))
+    rng = np.random.default_rng(3)
+    grads = [rng.standard_normal((v.height, v.width, 4)).astype(np.float32) for v in views]
+    for g, buf in zip(grads, gv.grad):
+        buf.copy_(torch.from_numpy(g))
+    gt = torch.zeros_like(tex_d)
+    gm = torch.zeros_like(mul_d)
+    r.backward(gt, gm, api.tuning())
+    torch.cuda.synchronize()
+    ref_t = np.zeros(tex.shape)
+    ref_m = np.zeros(mul.shape)
+    dot = 0.0
+    for i, sc in enumerate(views):
+        col, vis, dep, w = H.oracle_diff_forward(sc, tex, mul)
+        a, b = H.oracle_diff_backward(sc, tex, mul, vis, w, grads[i], mul.shape[0])
+        ref_t += a
+        ref_m += b
+        dot += float((grads[i].astype(np.float64) * gv.color[i].cpu().numpy()).sum())
+    got_t, got_m = gt.cpu().numpy().astype(np.float64), gm.cpu().numpy().astype(np.float64)
+    assert np.max(np.abs(got_t - ref_t)) <= ADJOINT_RTOL * np.max(np.abs(ref_t))
+    assert np.max(np.abs(got_m - ref_m)) <= ADJOINT_RTOL * np.max(np.abs(ref_m))
+    # Dot-product identities: colour is linear in the texture, and its RGB in the multipliers.
+    assert abs(dot - float((got_t * tex).sum())) <= 1e-4 * abs(dot)
+    dot_rgb = sum(float((grads[i][..., :3].astype(np.float64) * gv.color[i].cpu().numpy()[..., :3]).sum())
+                  for i in range(len(views)))
+    assert abs(dot_rgb - float((got_m * mul).sum())) <= 1e-4 * abs(dot_rgb)
+
+
+@pytest.fixture(scope="module")
+def fit_setup(api):
+    """Config 3: 8 views at 1280x720 of a 50,000-triangle displaced geodesic sphere (frequency 50),
+    a 512x512 RGBA float texture and per-point RGB multipliers."""
+    views = S.sphere_views(50, 8, 1280, 720, seed=3)
+    gv = api.GpuViews(views)
+    hidden = torch.from_numpy(S.float_texture(512, 77, smooth=16)).cuda()
+    ones = torch.ones((gv.point_count, 3), dtype=torch.float32, device="cuda")
+    r = api.DiffRenderer()
+    t = api.tuning(clear=True, persistent_per_cu=4, use_graph=True)
+    r.forward(gv.model, gv.views, hidden, ones, t)
+    target = torch.stack([c.clone() for c in gv.color])
+    return gv, r, t, target
+
+
+def test_config3_finite_differences(api, fit_setup):
+    gv, r, t, target = fit_setup
+    texture = torch.full((512, 512, 4), 0.5, dtype=torch.float32, device="cuda")
+    mul = torch.ones((gv.point_count, 3), dtype=torch.float32, device="cuda")
+
+    def render():
+        r.forward(gv.model, gv.views, texture, mul, t)
+        return torch.stack(gv.color).clone()
+
+    img = render()
+    for i, g in enumerate(gv.grad):
+        g.copy_(2.0 * (img[i] - target[i]))
+    gt = torch.zeros_like(texture)
+    r.backward(gt, None, api.tuning())
+    analytic = gt.cpu().numpy().reshape(-1)
+    rng = np.random.default_rng(1000)
+    texels = rng.integers(0, 512 * 512, 1000)
+    channels = rng.integers(0, 4, 1000)
+    h = 1e-3
+    scale = np.abs(analytic).max()
+    target64 = target.double()
+    errors = []
+    flat = texture.view(-1)
+    for tx, ch in zip(texels, channels):
+        k = int(tx) * 4 + int(ch)
+        flat[k] += h
+        ip = render()
+        flat[k] -= 2 * h
+        im = render()
+        flat[k] += h
+        # L(T+h) - L(T-h) = sum (ip - im)(ip + im - 2 target): exactly zero where nothing changed.
+        num = float(((ip.double() - im.double()) * (ip.double() + im.double() - 2.0 * target64)).sum())
+        fd = num / (2 * h)
+        an = float(analytic[k])
+        errors.append(abs(fd - an) / max(abs(an), 1e-3 * scale))
+    errors = np.array(errors)
+    visible = int((np.abs(analytic[texels * 4 + channels]) > 0).sum())
+    print(f"[gpu] fd: max rel err {errors.max():.2e}, median {np.median(errors):.2e}, {visible} of 1000 texels visible",
+          flush=True)
+    assert visible > 100
+    assert errors.max() <= FD_RTOL
+
+
+def test_config3_adam_fit(api, fit_setup):
+    gv, r, t, target = fit_setup
+    texture = torch.full((512, 512, 4), 0.5, dtype=torch.float32, device="cuda", requires_grad=True)
+    mul = torch.ones((gv.point_count, 3), dtype=torch.float32, device="cuda", requires_grad=True)
+    opt = torch.optim.Adam([texture, mul], lr=1e-2)
+    losses = []
+    for step in range(200):
+        opt.zero_grad()
+        img = RenderFunction.apply(texture, mul, r, gv, t)
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss))
+    print(f"[gpu] fit: loss {losses[0]:.2f} -> {losses[-1]:.2f}", flush=True)
+    assert losses[-1] < 0.1 * losses[0]
+
+
+def test_config4_batched_equals_per_view(api):
+    """Config 4: 256 views of 256x256, a 20,480-triangle sphere, a shared 1024x1024 texture.
+    One persistent, graph-captured launch per pass against the per-view form."""
+    views = S.sphere_views(32, 256, 256, 256, seed=11)
+    gv = api.GpuViews(views)
+    tex = torch.from_numpy(S.float_texture(1024, 5, smooth=32)).cuda()
+    rng = np.random.default_rng(4)
+    for g in gv.grad:
+        g.copy_(torch.from_numpy(rng.standard_normal((256, 256, 4)).astype(np.float32)))
+    batched = api.DiffRenderer()
+    t = api.tuning(clear=True, persistent_per_cu=4, use_graph=True)
+    batched.forward(gv.model, gv.views, tex, None, t)
+    images = [c.clone() for c in gv.color]
+    vis = [v.clone() for v in gv.visibility]
+    g_batched = torch.zeros_like(tex)
+    batched.backward(g_batched, None, t)
+    single = api.DiffRenderer()
+    g_sum = torch.zeros_like(tex)
+    for i in range(len(views)):
+        one = (type(gv.views[0]) * 1)(gv.views[i])
+        single.forward(gv.model, one, tex, None, api.tuning(clear=True))
+        assert torch.equal(gv.color[i], images[i]), f"view {i} image"
+        assert torch.equal(gv.visibility[i], vis[i]), f"view {i} visibility"
+        single.backward(g_sum, None, api.tuning())
+    torch.cuda.synchronize()
+    diff = (g_batched - g_sum).abs().max().item()
+    assert diff <= BATCHED_GRAD_RTOL * g_sum.abs().max().item()
