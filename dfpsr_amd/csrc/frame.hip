@@ -126,6 +126,7 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	const int32_t tileTotal = f.tileTotal;
 	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, f.counters, 8, 0);
 	hipLaunchKernelGGL(k_fill, dim3(blocksOf(2 * (int64_t)tileTotal + 1, 256)), dim3(256), 0, stream, f.tileCount, 2 * tileTotal + 1, 0);
+	hipLaunchKernelGGL(k_fill, dim3(blocksOf(f.canvasCount, 256)), dim3(256), 0, stream, f.canvasCommandBase, f.canvasCount, 0x7FFFFFFF);
 	if (o.clear) hipLaunchKernelGGL(k_clear_canvases, dim3(tileTotal), dim3(256), 0, stream, f, o.clearColor, o.clearDepth);
 	if (events) HIP_TRY(hipEventRecord(events_[0], stream));
 	hipLaunchKernelGGL(k_project, dim3(blocksOf(f.pointTotal, 256)), dim3(256), 0, stream, f);
@@ -210,7 +211,8 @@ int Frame::run(const RenderOptions &o) {
 	    dFanCount_.ensure((size_t)fanTotal * 4) != hipSuccess ||
 	    dBlockSums_.ensure((size_t)(fanBlocks > tileBlocks ? fanBlocks : tileBlocks) * 4) != hipSuccess ||
 	    dCounters_.ensure(8 * 4) != hipSuccess || dTileCount_.ensure((2 * (size_t)tileTotal + 1) * 4) != hipSuccess ||
-	    dTileOffset_.ensure(((size_t)tileTotal + 1) * 4) != hipSuccess || dTileQueue_.ensure((size_t)tileTotal * 4) != hipSuccess)
+	    dTileOffset_.ensure(((size_t)tileTotal + 1) * 4) != hipSuccess || dTileQueue_.ensure((size_t)tileTotal * 4) != hipSuccess ||
+	    dCanvasBase_.ensure(nC * 4) != hipSuccess)
 		return fail(-4, "device allocation failed");
 
 	for (int pass = 1; pass <= 3; pass++) {
@@ -242,6 +244,7 @@ int Frame::run(const RenderOptions &o) {
 		f.tileOffset = dTileOffset_.as<int32_t>();
 		f.tileList = dTileList_.as<int32_t>();
 		f.tileQueue = dTileQueue_.as<int32_t>();
+		f.canvasCommandBase = dCanvasBase_.as<int32_t>();
 		f.pairCapacity = pairCapacity_;
 		f.tileW = tileConfigW(tileConfig_);
 		f.tileH = tileConfigH(tileConfig_);

@@ -146,6 +146,8 @@ __global__ void k_setup_emit(FrameDev f) {
 		atomicOr(&f.counters[2], 1);
 		return;
 	}
+	const int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
+	atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
 	setupTriangle<true>(f, g, f.commands + offset);
 }
 
@@ -632,7 +634,7 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 			QuadEval e;
 			if (py >= al.t && py < al.bottom() && evalQuad(cmd, rows[py - tileY0], rows[py + 1 - tileY0], px, py, cv.height, e)) {
 				if constexpr (MODE == 0) drawCommand(f, cmd, task, e, q, colorWrite, depthRead, depthWrite, alphaFilter);
-				else drawCommandDiff(f, cmd, c, task, e, q, depthRead, depthWrite);
+				else drawCommandDiff(f, cmd, c - f.canvasCommandBase[ci], task, e, q, depthRead, depthWrite);
 			}
 		}
 		done += take;
@@ -732,7 +734,7 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 		const size_t p = (size_t)y * cv.diffStride + x;
 		const int32_t id = cv.visibility[p];
 		if (id < 0) continue;
-		const int32_t c = id >> 2, lane = id & 3;
+		const int32_t c = (id >> 2) + f.canvasCommandBase[ci], lane = id & 3;
 		const Command cmd = f.commands[c];
 		const TaskDev &task = f.tasks[cmd.task];
 		Rect wb = triangleBound(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]});

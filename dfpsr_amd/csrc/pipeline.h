@@ -80,6 +80,7 @@ struct FrameDev {
 	int32_t *tileList;            // pair capacity
 	int32_t pairCapacity;
 	int32_t *tileQueue;           // tiles with at least one command (counters[3] of them)
+	int32_t *canvasCommandBase;   // per canvas: index of its first draw command (visibility ids count from it)
 	int32_t *queueHead;           // next queue slot for persistent raster workgroups
 	int32_t tileW, tileH;
 	int32_t alphaAny;             // any task uses the Alpha filter: tiles read colour first
