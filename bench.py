@@ -14,7 +14,16 @@ second), frames/s, the whole frame's algorithmic GB/s, the roofline of the domin
 (the tile raster kernel, timed live with HIP events on the renderer's stream) and the
 reference CPU renderer timed on this machine's host cores.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+Two more objects ride on the same line (skip them with --no-batch / --no-diff):
+  "batched": 4,096 uneven jobs (canvases 64..640 x 48..480, 100..20,000 triangles, cube soups
+      and terrain patches, shared 256^2 textures) drawn by one batched, graph-replayed launch
+      sequence with the autotuned tile shape, against the per-job loop of the plain renderer
+      on the same GPU; images compared for identity.
+  "differentiable": forward + adjoint steps of the 8-view 1280x720 fit (50,000-triangle
+      sphere, 512^2 float texture, per-point RGB multipliers) and of the 256-view 256x256
+      batch (20,480 triangles, 1024^2 texture).
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-batch] [--no-diff]
 """
 from __future__ import annotations
 
@@ -79,6 +88,100 @@ def traffic_from_profiles():
         return None
 
 
+def progress(message: str) -> None:
+    print(f"[bench] {message}", file=sys.stderr, flush=True)
+
+
+def batched_form(repeats: int = 5) -> dict:
+    """4,096 uneven jobs: the batched form (autotuned, graph replay) against the per-job loop of
+    the plain renderer, images compared for identity."""
+    import torch
+    from dfpsr_amd import api, autotune, scenes
+    t0 = time.perf_counter()
+    jobs = scenes.batch_jobs()
+    gpu = api.GpuJobs(jobs)
+    setup_s = time.perf_counter() - t0
+    progress(f"batched: {len(jobs)} jobs resident after {setup_s:.1f} s")
+    batch = api.Batch()
+    stored = autotune.load()
+    if stored is None:
+        best, table = batch.autotune(gpu.jobs, 3, api.tuning(clear=True))
+        tile_config, per_cu, source = int(best.tile_config), int(best.persistent_per_cu), "swept in this run"
+    else:
+        (tile_config, per_cu), source = stored, "dfpsr_amd/tuning_gfx950.json"
+    t = api.tuning(tile_config=tile_config, persistent_per_cu=per_cu, use_graph=True, clear=True)
+    for _ in range(2):
+        batch.render(gpu.jobs, t)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(repeats):
+        batch.render(gpu.jobs, t)
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - start) / repeats
+    progress(f"batched: {batch_s * 1e3:.2f} ms per batch")
+    batched_colors = [c.clone() for c in gpu.colors]
+    batched_depths = [d.clone() for d in gpu.depths]
+    renderer = api.renderer_create()
+    api.renderer_setTuning(renderer, api.tuning(clear=True))
+    gpu.render_plain(renderer)  # warm-up pass
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    gpu.render_plain(renderer)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - start
+    progress(f"batched: plain loop {plain_s * 1e3:.2f} ms")
+    identical = all(torch.equal(a, b) for a, b in zip(batched_colors, gpu.colors)) and \
+        all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(batched_depths, gpu.depths))
+    stats = batch.stats()
+    n = len(jobs)
+    return {"workload": f"{n} uneven jobs (seed 2): {int(sum(j.triangle_count for j in jobs))} triangles, "
+                        f"{int(sum(j.width * j.height for j in jobs))} canvas pixels",
+            "frames_per_s": round(n / batch_s, 1), "ms_per_batch": round(batch_s * 1e3, 3),
+            "plain_loop_frames_per_s": round(n / plain_s, 1), "plain_loop_ms": round(plain_s * 1e3, 3),
+            "speedup_vs_plain_loop": round(plain_s / batch_s, 2), "identical_images": bool(identical),
+            "tuning": {"tile": autotune.TILE_SHAPES[tile_config], "persistent_per_cu": per_cu, "use_graph": 1,
+                       "source": source},
+            "draw_commands": int(stats[0]), "tile_pairs": int(stats[1]), "setup_s": round(setup_s, 1)}
+
+
+def differentiable_form(steps: int = 10) -> dict:
+    """Forward + adjoint steps of the two differentiable workloads."""
+    import torch
+    from dfpsr_amd import api, scenes
+    out = {}
+    for key, views, tex_size, with_mul in (("fit_8x1280x720", scenes.sphere_views(50, 8, 1280, 720, seed=3), 512, True),
+                                           ("batch_256x256x256", scenes.sphere_views(32, 256, 256, 256, seed=11), 1024, False)):
+        gv = api.GpuViews(views)
+        tex = torch.from_numpy(scenes.float_texture(tex_size, 5, smooth=16)).cuda()
+        mul = torch.ones((gv.point_count, 3), dtype=torch.float32, device="cuda") if with_mul else None
+        gt = torch.zeros_like(tex)
+        gm = torch.zeros_like(mul) if with_mul else None
+        for g in gv.grad:
+            g.fill_(1e-3)
+        r = api.DiffRenderer()
+        t = api.tuning(clear=True, persistent_per_cu=4, use_graph=True)
+        for _ in range(2):
+            r.forward(gv.model, gv.views, tex, mul, t)
+            r.backward(gt, gm, t)
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        for _ in range(steps):
+            r.forward(gv.model, gv.views, tex, mul, t)
+        torch.cuda.synchronize()
+        fwd = (time.perf_counter() - start) / steps
+        start = time.perf_counter()
+        for _ in range(steps):
+            r.backward(gt, gm, t)
+        torch.cuda.synchronize()
+        bwd = (time.perf_counter() - start) / steps
+        pixels = sum(v.width * v.height for v in views)
+        progress(f"differentiable {key}: forward {fwd * 1e3:.3f} ms, backward {bwd * 1e3:.3f} ms")
+        out[key] = {"views": len(views), "triangles": int(views[0].triangle_count), "texture": tex_size,
+                    "forward_ms": round(fwd * 1e3, 3), "backward_ms": round(bwd * 1e3, 3),
+                    "step_mpix_per_s": round(pixels / (fwd + bwd) / 1e6, 1)}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,9 +189,22 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES between two device buffers three times (calibrates HBM counters)")
     args = ap.parse_args()
     if args.cpu_baseline_only:
         print(json.dumps(cpu_baseline()))
+        return 0
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "launches": 3}))
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -129,6 +245,7 @@ def main() -> int:
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     stats = renderer.stats()
+    progress(f"plain: {elapsed / args.steps * 1e3:.3f} ms per frame")
 
     ms_per_step = elapsed / args.steps * 1e3
     raster_avg_ms = float(np.mean(raster_ms))
@@ -161,6 +278,13 @@ def main() -> int:
                      "algorithmic_bytes": RASTER_BYTES, "avg_launch_ms": round(raster_avg_ms, 5)},
         "cpu_baseline": None,
     }
+    del gpu_scene, renderer, color, depth
+    if not args.no_batch:
+        line["batched"] = batched_form()
+        torch.cuda.empty_cache()
+    if not args.no_diff:
+        line["differentiable"] = differentiable_form()
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         try:
             res = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only"], capture_output=True,

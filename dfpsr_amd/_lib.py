@@ -88,6 +88,7 @@ _SIGNATURES = {
     "dfpsr_hip_batch_stats": ([_P, _P], C.c_int),
     "dfpsr_hip_batch_set_timing": ([_P, _I], C.c_int),
     "dfpsr_hip_batch_timing": ([_P, _P], C.c_int),
+    "dfpsr_hip_batch_autotune": ([_P, C.POINTER(Job), _I, _I, C.POINTER(Tuning), C.POINTER(Tuning), _P], C.c_int),
     "dfpsr_hip_diff_create": ([C.POINTER(_P)], C.c_int),
     "dfpsr_hip_diff_destroy": ([_P], None),
     "dfpsr_hip_diff_forward": ([_P, _P, C.POINTER(View), _I, _P, _I, _I, _P, C.POINTER(Tuning)], C.c_int),

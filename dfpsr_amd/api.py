@@ -300,6 +300,15 @@ class Batch:
         check(lib().dfpsr_hip_batch_stats(self._h, out.ctypes.data_as(C.c_void_p)))
         return out
 
+    def autotune(self, jobs, repeats: int = 3, base: Optional[Tuning] = None):
+        """Sweeps tile shape x persistent workgroups per CU (graph replay on) and returns
+        (best Tuning, 3 x 5 array of median milliseconds)."""
+        best = Tuning()
+        ms = np.zeros((3, 5), np.float32)
+        check(lib().dfpsr_hip_batch_autotune(self._h, jobs, len(jobs), repeats, C.byref(base) if base is not None else None,
+                                             C.byref(best), ms.ctypes.data_as(C.c_void_p)))
+        return best, ms
+
 
 class GpuJobs:
     """The jobs of a batch resident on the GPU: shared textures, one model per job and device

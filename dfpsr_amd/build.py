@@ -43,7 +43,28 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed to build libdfpsr_hip.so")
     os.replace(LIB + ".tmp", LIB)
+    build_facade_program(verbose)
     return LIB
+
+
+FACADE_SOURCE = os.path.join(ROOT, "tests", "host", "facade_render.cpp")
+FACADE_PROGRAM = os.path.join(ROOT, "tests", "host", "_build", "facade_render")
+
+
+def build_facade_program(verbose: bool = False) -> str:
+    """The C++14 facade's test program: plain g++ at the reference's language level, linked
+    against the in-tree library."""
+    os.makedirs(os.path.dirname(FACADE_PROGRAM), exist_ok=True)
+    cmd = ["g++", "-std=c++14", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), FACADE_SOURCE,
+           "-L" + HERE, "-ldfpsr_hip", "-Wl,-rpath," + os.path.relpath(HERE, os.path.dirname(FACADE_PROGRAM)).join(["$ORIGIN/", ""]),
+           "-o", FACADE_PROGRAM]
+    if verbose:
+        print(" ".join(cmd))
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed to build the facade test program")
+    return FACADE_PROGRAM
 
 
 if __name__ == "__main__":

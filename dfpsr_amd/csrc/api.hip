@@ -7,6 +7,7 @@
 #include "frame.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -509,6 +510,45 @@ extern "C" int dfpsr_hip_batch_set_timing(dfpsr_hip_batch *b, int32_t enable) {
 extern "C" int dfpsr_hip_batch_timing(const dfpsr_hip_batch *b, float out[5]) {
 	if (!b || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_timing: null argument");
 	for (int i = 0; i < 5; i++) out[i] = b->frame.stageMs[i];
+	return DFPSR_HIP_OK;
+}
+
+static const int32_t AUTOTUNE_PERSISTENT[DFPSR_HIP_AUTOTUNE_PERSISTENT_STEPS] = {0, 1, 2, 4, 8};
+
+extern "C" int dfpsr_hip_batch_autotune(dfpsr_hip_batch *b, const dfpsr_hip_job *jobs, int32_t count, int32_t repeats,
+                                        const dfpsr_hip_tuning *base, dfpsr_hip_tuning *best, float *ms_out) {
+	if (!b || !best || repeats < 1 || repeats > 1000) return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_autotune: bad argument");
+	int rc = checkTuning(base);
+	if (rc) return rc;
+	dfpsr_hip_tuning t = {};
+	if (base) t = *base;
+	t.use_graph = 1;
+	float bestMs = 0.0f;
+	std::vector<double> samples((size_t)repeats);
+	for (int32_t tc = 0; tc < DFPSR_HIP_AUTOTUNE_TILE_CONFIGS; tc++) {
+		for (int32_t p = 0; p < DFPSR_HIP_AUTOTUNE_PERSISTENT_STEPS; p++) {
+			t.tile_config = tc;
+			t.persistent_per_cu = AUTOTUNE_PERSISTENT[p];
+			// Two untimed renders: the first (re)builds the tables and captures the graph.
+			for (int w = 0; w < 2; w++) {
+				rc = dfpsr_hip_batch_render(b, jobs, count, &t);
+				if (rc) return rc;
+			}
+			for (int32_t r = 0; r < repeats; r++) {
+				auto start = std::chrono::steady_clock::now();
+				rc = dfpsr_hip_batch_render(b, jobs, count, &t);
+				if (rc) return rc;
+				samples[(size_t)r] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+			}
+			std::sort(samples.begin(), samples.end());
+			const float ms = (float)samples[(size_t)repeats / 2];
+			if (ms_out) ms_out[tc * DFPSR_HIP_AUTOTUNE_PERSISTENT_STEPS + p] = ms;
+			if ((tc == 0 && p == 0) || ms < bestMs) {
+				bestMs = ms;
+				*best = t;
+			}
+		}
+	}
 	return DFPSR_HIP_OK;
 }
 

@@ -187,6 +187,19 @@ int dfpsr_hip_batch_stats(const dfpsr_hip_batch *batch, int64_t out[4]);
 int dfpsr_hip_batch_set_timing(dfpsr_hip_batch *batch, int32_t enable);
 int dfpsr_hip_batch_timing(const dfpsr_hip_batch *batch, float out[5]);
 
+/* Autotuner of the batched form.  Renders the jobs with every tile shape (tile_config 0..2)
+ * and persistent_per_cu in {0, 1, 2, 4, 8}, graph replay on, and writes the fastest setting
+ * to *best (clear, clear_color and clear_depth are taken from base, which may be NULL).
+ * Each setting is timed as the median host wall-clock time of `repeats` whole
+ * dfpsr_hip_batch_render calls after two untimed ones.  ms_out (may be NULL) receives the 15
+ * medians in milliseconds at [tile_config * 5 + persistent step].  The canvases end up
+ * holding the jobs' images, which are the same for every setting.  The bins are sized
+ * exactly by a count pass and a scan, so there is no bin depth to tune. */
+#define DFPSR_HIP_AUTOTUNE_TILE_CONFIGS 3
+#define DFPSR_HIP_AUTOTUNE_PERSISTENT_STEPS 5
+int dfpsr_hip_batch_autotune(dfpsr_hip_batch *batch, const dfpsr_hip_job *jobs, int32_t job_count, int32_t repeats,
+                             const dfpsr_hip_tuning *base, dfpsr_hip_tuning *best, float *ms_out);
+
 /* ---- Differentiable rendering (no counterpart in the reference) ------------------------- */
 
 /* One view of a differentiable frame.  All buffers are device memory with stride_pixels

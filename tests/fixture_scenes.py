@@ -102,6 +102,38 @@ def clip_scenes():
     return out
 
 
+def filter_scenes():
+    """The orthogonal camera over a depth canvas cleared to +inf (it keeps the lowest linear
+    depth, renderCore.cpp:372-392), diffuse + light maps under the Alpha filter, and two
+    instances of one model."""
+    out = []
+    sc = S.cube_soup(width=100, height=80, triangles=200, seed=24, spread=0.3, name="orthogonal")
+    sc.perspective = False
+    sc.width_slope = 1.5
+    sc.initial_depth = np.full((sc.height, sc.width), np.inf, np.float32)
+    out.append(sc)
+    sc = S.terrain(width=120, height=90, grid=24, seed=6, texture_size=64, resolutions=7, name="orthogonal_terrain")
+    sc.perspective = False
+    sc.width_slope = 14.0
+    sc.initial_depth = np.full((sc.height, sc.width), np.inf, np.float32)
+    out.append(sc)
+    n = 300
+    sc = S.cube_soup(width=120, height=90, triangles=n, seed=22, spread=0.3, name="alpha_lightmap")
+    rng = S.Rng(23)
+    sc.textures = [S.noise_checker_texture(32, 5, 6, check=4), S.noise_checker_texture(16, 6, 1, check=2)]
+    sc.parts = [S.Part(0, n // 2, diffuse=0, light=1), S.Part(n // 2, n, light=1)]
+    sc.texcoords[:, :3, :] = rng.uniform(n * 12, -1, 2).reshape(n, 3, 4)
+    sc.filter = 1
+    sc.colors[:, :, 3] = rng.uniform(n * 4).reshape(n, 4)
+    out.append(sc)
+    sc = S.cube_soup(width=96, height=96, triangles=150, seed=25, spread=0.3, name="instances")
+    t2 = S.identity_transform()
+    t2[0] = 0.4
+    sc.instances = np.stack([S.identity_transform(), t2])
+    out.append(sc)
+    return out
+
+
 def id_color(scene: S.Scene) -> S.Scene:
     """Same geometry with one constant colour per polygon and no textures: selects the
     "Single color" shader (RgbaMultiply.h:170-171), so the colour buffer is a triangle-id map
@@ -123,6 +155,6 @@ def id_color(scene: S.Scene) -> S.Scene:
 
 
 def fixture_scenes():
-    out = micro_scenes() + clip_scenes() + [variant(s) for s in range(40)]
+    out = micro_scenes() + clip_scenes() + [variant(s) for s in range(40)] + filter_scenes()
     out += [id_color(s) for s in clip_scenes()]
     return out

@@ -71,3 +71,17 @@ def test_renderer_tuning_does_not_change_results(setup):
         for _ in range(2):
             got = api.render_scene(sc, r)
             assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1].view(np.uint32), ref[1].view(np.uint32)), cfg
+
+
+def test_autotune_picks_a_swept_setting_and_keeps_images(setup):
+    api, jobs, g, plain = setup
+    b = api.Batch()
+    g.clear()
+    best, ms = b.autotune(g.jobs, 2, api.tuning(clear=True))
+    assert ms.shape == (3, 5) and np.all(ms > 0)
+    steps = [0, 1, 2, 4, 8]
+    assert 0 <= best.tile_config < 3 and best.persistent_per_cu in steps and best.use_graph == 1 and best.clear == 1
+    assert ms[best.tile_config, steps.index(best.persistent_per_cu)] == ms.min()
+    for i in range(len(jobs)):
+        c, d = g.image(i)
+        assert np.array_equal(c, plain[i][0]) and np.array_equal(d.view(np.uint32), plain[i][1].view(np.uint32)), i
