@@ -254,6 +254,7 @@ static RenderOptions optionsFrom(const dfpsr_hip_tuning *t) {
 	o.clear = t->clear != 0;
 	o.clearColor = t->clear_color;
 	o.clearDepth = t->clear_depth;
+	o.rasterPath = t->raster_path;
 	return o;
 }
 
@@ -261,6 +262,7 @@ static int checkTuning(const dfpsr_hip_tuning *t) {
 	if (!t) return fail(DFPSR_HIP_ERROR_ARGUMENT, "null tuning");
 	if (t->tile_config < 0 || t->tile_config >= TILE_CONFIGS) return fail(DFPSR_HIP_ERROR_ARGUMENT, "tile_config must be 0, 1 or 2");
 	if (t->persistent_per_cu < 0 || t->persistent_per_cu > 64) return fail(DFPSR_HIP_ERROR_ARGUMENT, "persistent_per_cu must be 0..64");
+	if (t->raster_path < 0 || t->raster_path > 1) return fail(DFPSR_HIP_ERROR_ARGUMENT, "raster_path must be 0 or 1");
 	return DFPSR_HIP_OK;
 }
 

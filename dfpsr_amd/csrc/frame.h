@@ -66,6 +66,7 @@ struct RenderOptions {
 	uint32_t clearColor = 0;
 	float clearDepth = 0.0f;
 	int32_t mode = 0;                // 0: the reference's RGBA8 shaders; 1: the differentiable float shader
+	int32_t rasterPath = 0;          // 0: visibility raster where the frame allows it; 1: always in order
 };
 
 class Frame {
@@ -113,6 +114,8 @@ private:
 	std::vector<int32_t> partTexture_;
 	std::vector<size_t> partTextureOffset_;
 	bool alphaAny_ = false;
+	bool orthogonalAny_ = false;  // a task with an orthogonal camera
+	bool oddCanvas_ = false;      // a canvas of odd height
 	int32_t commandCapacity_ = 0, pairCapacity_ = 0;
 	// All per-frame tables in one upload, skipped when identical to the previous frame's.
 	std::vector<uint8_t> tables_, uploadedTables_;
@@ -138,6 +141,11 @@ private:
 	int32_t textureSlot(const TextureDesc *desc);
 	int enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks);
 	void dropGraph();
+	// The visibility raster reproduces the in-order result when every command is Solid and
+	// perspective and no canvas has an odd height.
+	bool deferredRaster(const RenderOptions &o) const {
+		return o.mode == 0 && o.rasterPath == 0 && !alphaAny_ && !orthogonalAny_ && !oddCanvas_;
+	}
 };
 
 }  // namespace dfpsr

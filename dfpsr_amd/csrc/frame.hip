@@ -54,6 +54,8 @@ void Frame::reset(int32_t tileConfig) {
 	partTexture_.clear();
 	partTextureOffset_.clear();
 	alphaAny_ = false;
+	orthogonalAny_ = false;
+	oddCanvas_ = false;
 }
 
 int32_t Frame::addCanvas(const CanvasHost &c) {
@@ -72,6 +74,7 @@ int32_t Frame::addCanvas(const CanvasHost &c) {
 	d.visibility = c.visibility;
 	d.gradColor = c.gradColor;
 	d.diffStride = c.diffStride;
+	if (c.height & 1) oddCanvas_ = true;
 	canvases_.push_back(d);
 	canvasTileEnd_.push_back(d.tileBase + d.tilesX * d.tilesY);
 	return (int32_t)canvases_.size() - 1;
@@ -112,6 +115,7 @@ int Frame::addTask(const ModelView &m, const float *transform, const CameraParam
 		partTexture_.push_back(textureSlot(m.partLight[p]));
 	}
 	if (m.filter != 0) alphaAny_ = true;
+	if (!camera.perspective) orthogonalAny_ = true;
 	tasks_.push_back(t);
 	taskPointEnd_.push_back(t.pointBase + m.pointCount);
 	taskFanEnd_.push_back(t.fanBase + m.fanCount);
@@ -145,6 +149,12 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	const int32_t grid = persistent ? deviceCUs_ * o.persistentPerCU : tileTotal;
 	if (o.mode == 1) {
 		hipLaunchKernelGGL(k_raster_diff, dim3(grid), dim3(64), 0, stream, f, persistent);
+	} else if (deferredRaster(o)) {
+		switch (tileConfig_) {
+		case 1: hipLaunchKernelGGL((k_raster_deferred<32, 16>), dim3(grid), dim3(DEFER_THREADS), 0, stream, f, persistent); break;
+		case 2: hipLaunchKernelGGL((k_raster_deferred<32, 32>), dim3(grid), dim3(DEFER_THREADS), 0, stream, f, persistent); break;
+		default: hipLaunchKernelGGL((k_raster_deferred<16, 16>), dim3(grid), dim3(DEFER_THREADS), 0, stream, f, persistent); break;
+		}
 	} else switch (tileConfig_) {
 	case 1: hipLaunchKernelGGL((k_raster<32, 16>), dim3(grid), dim3(128), 0, stream, f, persistent); break;
 	case 2: hipLaunchKernelGGL((k_raster<32, 32>), dim3(grid), dim3(256), 0, stream, f, persistent); break;

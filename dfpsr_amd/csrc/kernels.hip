@@ -385,6 +385,34 @@ __device__ __forceinline__ bool quadVisibility(const QuadEval &e, const QuadPixe
 	return anyVis;
 }
 
+// Shader of one quad of one command for all four lanes: RgbaMultiply's getPixels_2x2 with
+// the vertex weights of the lanes (RgbaMultiply.h:74-175, shaderMethods.h:39-94).
+__device__ __forceinline__ void shadeCommand(const FrameDev &f, const Command &cmd, const TaskDev &task, const QuadEval &e,
+                                             bool affine, float *r, float *g, float *b, float *a) {
+	float wa[4], wb[4], wc[4];
+	for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
+	// Shader inputs of the source triangle (Shader.h:37-57, RgbaMultiply.h:37-58).
+	int32_t fp = task.fanPolygon[cmd.fan];
+	int32_t poly = fp >> 1, k = fp & 1;
+	const float *col = task.colors + poly * 16;
+	const float *tc = task.texcoords + poly * 16;
+	const int32_t ca = 0, cb = (1 + k) * 4, cc = (2 + k) * 4;
+	ShaderInputs si;
+	si.red = make3(col[ca], col[cb], col[cc]);
+	si.green = make3(col[ca + 1], col[cb + 1], col[cc + 1]);
+	si.blue = make3(col[ca + 2], col[cb + 2], col[cc + 2]);
+	si.alpha = make3(col[ca + 3], col[cb + 3], col[cc + 3]);
+	si.u1 = make3(tc[ca], tc[cb], tc[cc]);
+	si.v1 = make3(tc[ca + 1], tc[cb + 1], tc[cc + 1]);
+	si.u2 = make3(tc[ca + 2], tc[cb + 2], tc[cc + 2]);
+	si.v2 = make3(tc[ca + 3], tc[cb + 3], tc[cc + 3]);
+	int32_t part = task.polygonPart[poly];
+	si.diffuse = task.partTexture[part * 2];
+	si.light = task.partTexture[part * 2 + 1];
+	finishShaderInputs(si);
+	shadeQuad(si, f.textures, wa, wb, wc, r, g, b, a);
+}
+
 // The reference's shading of one quad (fillQuadSuper, fillerTemplates.h:154-187, with the
 // RgbaMultiply shaders of RgbaMultiply.h:74-175).
 __device__ void drawCommand(const FrameDev &f, const Command &cmd, const TaskDev &task, const QuadEval &e,
@@ -393,29 +421,8 @@ __device__ void drawCommand(const FrameDev &f, const Command &cmd, const TaskDev
 	bool vis[4];
 	if (!quadVisibility<0>(e, q, depthRead, affine, vis)) return;
 	if (colorWrite) {
-		float wa[4], wb[4], wc[4];
-		for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
-		// Shader inputs of the source triangle (Shader.h:37-57, RgbaMultiply.h:37-58).
-		int32_t fp = task.fanPolygon[cmd.fan];
-		int32_t poly = fp >> 1, k = fp & 1;
-		const float *col = task.colors + poly * 16;
-		const float *tc = task.texcoords + poly * 16;
-		const int32_t ca = 0, cb = (1 + k) * 4, cc = (2 + k) * 4;
-		ShaderInputs si;
-		si.red = make3(col[ca], col[cb], col[cc]);
-		si.green = make3(col[ca + 1], col[cb + 1], col[cc + 1]);
-		si.blue = make3(col[ca + 2], col[cb + 2], col[cc + 2]);
-		si.alpha = make3(col[ca + 3], col[cb + 3], col[cc + 3]);
-		si.u1 = make3(tc[ca], tc[cb], tc[cc]);
-		si.v1 = make3(tc[ca + 1], tc[cb + 1], tc[cc + 1]);
-		si.u2 = make3(tc[ca + 2], tc[cb + 2], tc[cc + 2]);
-		si.v2 = make3(tc[ca + 3], tc[cb + 3], tc[cc + 3]);
-		int32_t part = task.polygonPart[poly];
-		si.diffuse = task.partTexture[part * 2];
-		si.light = task.partTexture[part * 2 + 1];
-		finishShaderInputs(si);
 		float r[4], g[4], b[4], a[4];
-		shadeQuad(si, f.textures, wa, wb, wc, r, g, b, a);
+		shadeCommand(f, cmd, task, e, affine, r, g, b, a);
 		if (alphaFilter) {
 			float tr[4], tg[4], tb[4], ta[4];
 			for (int l = 0; l < 4; l++) {
@@ -543,6 +550,52 @@ __device__ void drawCommandDiff(const FrameDev &f, const Command &cmd, int32_t c
 
 // One tile: load the lanes' pixels, draw every binned command in increasing command index
 // (submission order), write back the pixels that changed.
+// Loads the next round of a tile's command list into LDS, sorted by command index: the
+// commands in (last, threshold] for the largest threshold that keeps the round within
+// TILE_SORT_CAP (all of them in one round when the list fits).  Every one of the workgroup's
+// NT threads calls it; returns the round's size.
+template <int NT>
+__device__ int32_t nextRound(const int32_t *src, int32_t n, int32_t done, int32_t &last, int32_t *list, int32_t *roundInfo) {
+	const int32_t t = threadIdx.x;
+	int32_t take;
+	if (done == 0 && n <= TILE_SORT_CAP) {
+		for (int32_t i = t; i < n; i += NT) list[i] = src[i];
+		take = n;
+		last = 0x7FFFFFFE;
+	} else {
+		if (t == 0) { roundInfo[0] = last + 1; roundInfo[1] = 0x7FFFFFFE; }
+		while (true) {
+			__syncthreads();
+			int32_t lo = roundInfo[0], hi = roundInfo[1];
+			if (lo >= hi) break;
+			int32_t mid = lo + ((hi - lo + 1) >> 1);
+			__syncthreads();
+			if (t == 0) roundInfo[2] = 0;
+			__syncthreads();
+			int32_t cnt = 0;
+			for (int32_t i = t; i < n; i += NT) { int32_t v = src[i]; cnt += (v > last && v <= mid) ? 1 : 0; }
+			if (cnt) atomicAdd(&roundInfo[2], cnt);
+			__syncthreads();
+			if (t == 0) { if (roundInfo[2] <= TILE_SORT_CAP) roundInfo[0] = mid; else roundInfo[1] = mid - 1; }
+		}
+		__syncthreads();
+		int32_t threshold = roundInfo[0];
+		__syncthreads();
+		if (t == 0) roundInfo[2] = 0;
+		__syncthreads();
+		for (int32_t i = t; i < n; i += NT) {
+			int32_t v = src[i];
+			if (v > last && v <= threshold) list[atomicAdd(&roundInfo[2], 1)] = v;
+		}
+		__syncthreads();
+		take = roundInfo[2];
+		last = threshold;
+	}
+	__syncthreads();
+	sortLds<NT>(list, take);
+	return take;
+}
+
 template <int TW, int TH, int MODE>
 __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowInterval *rows, int32_t *roundInfo) {
 	constexpr int32_t NL = (TW / 2) * (TH / 2);
@@ -566,46 +619,10 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 		if constexpr (MODE == 0) q.color[s] = (in && cv.color && f.alphaAny) ? cv.color[(size_t)y * cv.colorStride + x] : 0u;
 	}
 
-	// Commands in increasing index, in rounds of at most TILE_SORT_CAP: each round takes the
-	// commands in (last, threshold] with the largest threshold that keeps the round in LDS.
 	int32_t last = -1;
 	int32_t done = 0;
 	while (done < n) {
-		int32_t take;
-		if (done == 0 && n <= TILE_SORT_CAP) {
-			for (int32_t i = lane; i < n; i += NL) list[i] = src[i];
-			take = n;
-		} else {
-			if (lane == 0) { roundInfo[0] = last + 1; roundInfo[1] = 0x7FFFFFFE; }
-			while (true) {
-				__syncthreads();
-				int32_t lo = roundInfo[0], hi = roundInfo[1];
-				if (lo >= hi) break;
-				int32_t mid = lo + ((hi - lo + 1) >> 1);
-				__syncthreads();
-				if (lane == 0) roundInfo[2] = 0;
-				__syncthreads();
-				int32_t cnt = 0;
-				for (int32_t i = lane; i < n; i += NL) { int32_t v = src[i]; cnt += (v > last && v <= mid) ? 1 : 0; }
-				if (cnt) atomicAdd(&roundInfo[2], cnt);
-				__syncthreads();
-				if (lane == 0) { if (roundInfo[2] <= TILE_SORT_CAP) roundInfo[0] = mid; else roundInfo[1] = mid - 1; }
-			}
-			__syncthreads();
-			int32_t threshold = roundInfo[0];
-			__syncthreads();
-			if (lane == 0) roundInfo[2] = 0;
-			__syncthreads();
-			for (int32_t i = lane; i < n; i += NL) {
-				int32_t v = src[i];
-				if (v > last && v <= threshold) list[atomicAdd(&roundInfo[2], 1)] = v;
-			}
-			__syncthreads();
-			take = roundInfo[2];
-			last = threshold;
-		}
-		__syncthreads();
-		sortLds<NL>(list, take);
+		const int32_t take = nextRound<NL>(src, n, done, last, list, roundInfo);
 		for (int32_t i = 0; i < take; i++) {
 			const int32_t c = __builtin_amdgcn_readfirstlane(list[i]);
 			const Command cmd = f.commands[c];
@@ -688,6 +705,201 @@ __global__ void __launch_bounds__((TW / 2) * (TH / 2)) k_raster(FrameDev f, int3
 template __global__ void k_raster<16, 16>(FrameDev, int32_t);
 template __global__ void k_raster<32, 16>(FrameDev, int32_t);
 template __global__ void k_raster<32, 32>(FrameDev, int32_t);
+
+// ---------------------------------------------------------------------------------------
+// Visibility raster for frames whose commands are all Solid and perspective on canvases of
+// even height.
+//
+// In the reference's order a pixel ends with the colour and depth of the first command
+// reaching the largest depth above the pixel's starting depth: a command writes where its
+// 1/W is strictly greater than the depth held at that moment (fillerTemplates.h:110-152),
+// so the held depth is always the largest value written so far and a later equal value
+// loses.  That winner is found without walking the commands in order: every covered lane of
+// every command offers the key (depth order << 32 | ~command) to the pixel with an LDS
+// atomic max, and only the winning command is shaded for each pixel, with the same quad
+// evaluation, weights and mip selection as the in-order walk.  Without a depth canvas the key
+// is the command alone and the last covering command wins.  Commands run in chunks of
+// DEFER_CHUNK per tile; each chunk starts from the depths the previous one left.
+//
+// Frames with the Alpha filter (blending reads the colour held at that moment), orthogonal
+// cameras (a lower depth wins) or an odd-height canvas (the missing last row aliases lanes
+// onto the row above within one command) take the in-order kernel above.
+// ---------------------------------------------------------------------------------------
+
+constexpr int32_t DEFER_CHUNK = 64;
+
+// Keys order depths as floats compare: larger depth, larger key; -0 equals +0.
+__device__ __forceinline__ uint32_t depthOrder(float d) {
+	uint32_t u = __float_as_uint(d == 0.0f ? 0.0f : d);
+	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+template <int TW, int TH>
+__device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, int32_t *list, RowInterval *rows,
+                                   unsigned long long *keys, uint32_t *colorT, float *depthT, uint32_t *dirty,
+                                   int32_t *roundInfo) {
+	constexpr int32_t NQX = TW / 2, NQ = (TW / 2) * (TH / 2), NP = TW * TH;
+	constexpr int32_t WAVES = DEFER_THREADS / 64;
+	const int32_t n = f.tileCount[tile];
+	const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
+	const CanvasDev cv = f.canvases[ci];
+	const int32_t local = tile - cv.tileBase;
+	const int32_t tileX0 = (local % cv.tilesX) * TW;
+	const int32_t tileY0 = (local / cv.tilesX) * TH;
+	const int32_t t = threadIdx.x;
+	const int32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
+	const int32_t lane = t & 63;
+	const int32_t *src = f.tileList + f.tileOffset[tile];
+	const bool colorWrite = cv.color != nullptr;
+	const bool depthRead = cv.depth != nullptr;
+
+	for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
+		depthT[p] = (depthRead && x < cv.width && y < cv.height) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
+		dirty[p] = 0;
+	}
+
+	int32_t last = -1;
+	int32_t done = 0;
+	while (done < n) {
+		const int32_t take = nextRound<DEFER_THREADS>(src, n, done, last, list, roundInfo);
+		for (int32_t c0 = 0; c0 < take; c0 += DEFER_CHUNK) {
+			const int32_t cn = take - c0 < DEFER_CHUNK ? take - c0 : DEFER_CHUNK;
+			// Keys start at the held depth with the lowest command part above every command's,
+			// so only a strictly greater depth replaces it; a NaN depth is never replaced.
+			for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+				const float d = depthT[p];
+				keys[p] = !depthRead ? 0ull : (d != d ? ~0ull : (((unsigned long long)depthOrder(d) << 32) | 0xFFFFFFFFull));
+			}
+			__syncthreads();
+			// Pass 1: wave w offers the coverage of chunk commands w, w + WAVES, ...
+			for (int32_t j = wave; j < cn; j += WAVES) {
+				const int32_t c = __builtin_amdgcn_readfirstlane(list[c0 + j]);
+				const Command cmd = f.commands[c];
+				Rect wb = triangleBound(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]});
+				Rect clipR = cutRect(wb, canvasRect(cv));
+				Rect al = alignedRasterBound(wb, canvasRect(cv));
+				Rect rowClip = rectSides(clipR.l, al.t, clipR.right(), al.bottom());
+				RowInterval *cr = rows + j * TH;
+				if (lane < TH) {
+					const int32_t y = tileY0 + lane;
+					RowInterval ri{0, 0};
+					if (y >= al.t && y < al.bottom()) {
+						l2 A{cmd.fx[0], cmd.fy[0]}, B{cmd.fx[1], cmd.fy[1]}, C{cmd.fx[2], cmd.fy[2]};
+						bool degenerate = sameCorner(A, B) || sameCorner(B, C) || sameCorner(C, A);
+						EdgeSetup e[3] = {edgeSetup(A, B, rowClip), edgeSetup(B, C, rowClip), edgeSetup(C, A, rowClip)};
+						ri = rowInterval(e, degenerate, rowClip, y - al.t);
+					}
+					cr[lane] = ri;
+				}
+				// The rows were written by lanes of this wave only.
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+				for (int32_t qi = lane; qi < NQ; qi += 64) {
+					const int32_t px = tileX0 + 2 * (qi % NQX), py = tileY0 + 2 * (qi / NQX);
+					QuadEval e;
+					if (py < al.t || py >= al.bottom() || !evalQuad(cmd, cr[py - tileY0], cr[py + 1 - tileY0], px, py, cv.height, e))
+						continue;
+					for (int l = 0; l < 4; l++) {
+						if (e.clipSides && !e.clip[l]) continue;
+						const int32_t s = e.slot[l];
+						const int32_t p = (py - tileY0 + (s >> 1)) * TW + (px - tileX0 + (s & 1));
+						if (depthRead) {
+							const float d = e.l0[l];
+							if (d == d) atomicMax(&keys[p], ((unsigned long long)depthOrder(d) << 32) | (unsigned long long)(0xFFFFFFFEu - (uint32_t)j));
+						} else {
+							atomicMax(&keys[p], (unsigned long long)(j + 1));
+						}
+					}
+				}
+			}
+			__syncthreads();
+			// Pass 2: shade each quad once per distinct winning command.
+			for (int32_t s2 = t; s2 < NQ * 4; s2 += DEFER_THREADS) {
+				const int32_t qi = s2 % NQ, k = s2 / NQ;
+				const int32_t px = tileX0 + 2 * (qi % NQX), py = tileY0 + 2 * (qi / NQX);
+				int32_t win[4];
+				for (int s = 0; s < 4; s++) {
+					const unsigned long long key = keys[(py - tileY0 + (s >> 1)) * TW + (px - tileX0 + (s & 1))];
+					if (depthRead) {
+						const uint32_t low = (uint32_t)key;
+						win[s] = low == 0xFFFFFFFFu ? -1 : (int32_t)(0xFFFFFFFEu - low);
+					} else {
+						win[s] = (int32_t)key - 1;
+					}
+				}
+				// The k-th distinct winner in pixel order.
+				int32_t jw = -1, found = 0;
+				for (int s = 0; s < 4 && jw < 0; s++) {
+					if (win[s] < 0) continue;
+					bool seen = false;
+					for (int r = 0; r < s; r++) seen |= win[r] == win[s];
+					if (seen) continue;
+					if (found == k) jw = win[s];
+					found++;
+				}
+				if (jw < 0) continue;
+				const Command cmd = f.commands[list[c0 + jw]];
+				const RowInterval *cr = rows + jw * TH;
+				QuadEval e;
+				if (!evalQuad(cmd, cr[py - tileY0], cr[py + 1 - tileY0], px, py, cv.height, e)) continue;
+				float r[4], g[4], b[4], a[4];
+				if (colorWrite) shadeCommand(f, cmd, f.tasks[cmd.task], e, (cmd.flags & 1) != 0, r, g, b, a);
+				for (int l = 0; l < 4; l++) {
+					if (e.clipSides && !e.clip[l]) continue;
+					const int32_t s = e.slot[l];
+					if (win[s] != jw) continue;
+					const int32_t p = (py - tileY0 + (s >> 1)) * TW + (px - tileX0 + (s & 1));
+					if (colorWrite) colorT[p] = packSaturated(r[l], g[l], b[l], a[l]);
+					if (depthRead) depthT[p] = e.l0[l];
+					dirty[p] = 1;
+				}
+			}
+			__syncthreads();
+		}
+		done += take;
+		if (take == 0) break;
+	}
+
+	for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+		if (!dirty[p]) continue;
+		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
+		if (colorWrite) cv.color[(size_t)y * cv.colorStride + x] = colorT[p];
+		if (depthRead) cv.depth[(size_t)y * cv.depthStride + x] = depthT[p];
+	}
+}
+
+template <int TW, int TH>
+__global__ void __launch_bounds__(DEFER_THREADS) k_raster_deferred(FrameDev f, int32_t persistent) {
+	__shared__ int32_t list[TILE_SORT_CAP];
+	__shared__ RowInterval rows[DEFER_CHUNK * TH];
+	__shared__ unsigned long long keys[TW * TH];
+	__shared__ uint32_t colorT[TW * TH];
+	__shared__ float depthT[TW * TH];
+	__shared__ uint32_t dirty[TW * TH];
+	__shared__ int32_t roundInfo[3];
+	__shared__ int32_t slotLds;
+	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
+	const int32_t queued = f.counters[3];
+	int32_t slot = blockIdx.x;
+	while (true) {
+		if (persistent) {
+			if (threadIdx.x == 0) slotLds = atomicAdd(f.queueHead, 1);
+			__syncthreads();
+			slot = slotLds;
+			__syncthreads();
+		}
+		if (slot >= queued) break;
+		rasterTileDeferred<TW, TH>(f, f.tileQueue[slot], list, rows, keys, colorT, depthT, dirty, roundInfo);
+		if (!persistent) break;
+		__syncthreads();
+	}
+}
+
+template __global__ void k_raster_deferred<16, 16>(FrameDev, int32_t);
+template __global__ void k_raster_deferred<32, 16>(FrameDev, int32_t);
+template __global__ void k_raster_deferred<32, 32>(FrameDev, int32_t);
 
 // The differentiable forward: the same tile walk with the float shader, persistent or not.
 __global__ void __launch_bounds__(64) k_raster_diff(FrameDev f, int32_t persistent) {

@@ -150,7 +150,11 @@ int dfpsr_hip_renderer_timing(const dfpsr_hip_renderer *renderer, float out[5]);
  * tile its own workgroup, k > 0 launches k workgroups per CU that pull tiles from a queue;
  * use_graph = 1 captures the launch sequence into a hipGraph and replays it while the work
  * is unchanged; clear = 1 fills the colour canvas with clear_color and the depth canvas with
- * clear_depth first (the caller's image_fill).  Results never depend on these settings. */
+ * clear_depth first (the caller's image_fill).  raster_path = 0 draws frames whose commands are
+ * all Solid and perspective, on canvases of even height, with the visibility raster (each
+ * pixel's winning command is found with atomic max over (depth, command) keys and shaded
+ * once); raster_path = 1 always walks every tile's commands in order.  Results never depend
+ * on these settings. */
 typedef struct {
 	int32_t tile_config;
 	int32_t persistent_per_cu;
@@ -158,6 +162,7 @@ typedef struct {
 	int32_t clear;
 	uint32_t clear_color;
 	float clear_depth;
+	int32_t raster_path;
 } dfpsr_hip_tuning;
 
 /* Tuning used by renderer_end (default: 16x16 tiles, one workgroup per tile, no graph). */

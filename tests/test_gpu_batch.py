@@ -30,15 +30,16 @@ def setup():
     return api, jobs, g, plain
 
 
-TUNINGS = [(0, 0, False), (1, 0, False), (2, 0, False), (0, 4, False), (2, 2, True), (0, 8, True)]
+TUNINGS = [(0, 0, False, 0), (1, 0, False, 0), (2, 0, False, 0), (0, 4, False, 0), (2, 2, True, 0), (0, 8, True, 0),
+           (0, 0, False, 1), (1, 4, True, 1)]
 
 
-@pytest.mark.parametrize("tile_config,persistent,graph", TUNINGS)
-def test_batch_equals_plain_loop(setup, tile_config, persistent, graph):
+@pytest.mark.parametrize("tile_config,persistent,graph,path", TUNINGS)
+def test_batch_equals_plain_loop(setup, tile_config, persistent, graph, path):
     api, jobs, g, plain = setup
-    print(f"[gpu] batch tile_config={tile_config} persistent={persistent} graph={graph}", flush=True)
+    print(f"[gpu] batch tile_config={tile_config} persistent={persistent} graph={graph} raster_path={path}", flush=True)
     b = api.Batch()
-    t = api.tuning(tile_config, persistent, graph, clear=True)
+    t = api.tuning(tile_config, persistent, graph, clear=True, raster_path=path)
     for rep in range(2 if graph else 1):  # a replayed graph must give the same images
         g.clear()
         b.render(g.jobs, t)

@@ -22,8 +22,9 @@ pytestmark = pytest.mark.gpu
 
 # Tolerances written into the checks below.
 FORWARD_RTOL = 1e-5          # colour vs oracle: weights differ only by the reciprocal estimate ...
-FORWARD_FLIP_FRACTION = 1e-4  # ... except where u or v sits on an 8-bit sub-texel boundary and the
-FORWARD_FLIP_ATOL = 1.0 / 128  # tap weights move by one step of 1/256 (texels and multipliers < 2)
+FORWARD_FLIP_FRACTION = 1e-3  # ... except where u or v sits on an 8-bit sub-texel boundary and the
+FORWARD_FLIP_ATOL = 1.0 / 128  # tap weights move by one step of 1/256 (texels and multipliers < 2);
+                               # the same 99.9% bar as the RGBA8 path's colours
 ADJOINT_RTOL = 1e-3          # GPU float32 atomics vs the oracle's float64 adjoint
 FD_RTOL = 1e-3               # analytic gradient vs central finite differences (h = 1e-3)
 BATCHED_GRAD_RTOL = 1e-4     # 256-view batched adjoint vs the sum of per-view adjoints
