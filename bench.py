@@ -5,9 +5,11 @@ Workload: the 1920x1080 textured terrain frame -- a 256x256-point grid (130,050 
 a 1024x1024 RGBA8 texture with 11 mip levels, per-vertex Lambert colours, a 60 degree camera
 30 units up pitched down 35 degrees (dfpsr_amd/scenes.py: terrain()).  Synthetic, seeded.
 
-One step = one frame through the C ABI, exactly as a reference application draws one:
-clear both canvases (image_fill), renderer_begin, renderer_giveTask(model), renderer_end.
-Model, texture and canvases stay resident in HBM; renderer_end returns with the frame drawn.
+One step = one frame through the C ABI, as a reference application draws one: clear both
+canvases (image_fill of colour 0 and depth 0, done by the renderer's clear option inside the
+frame), renderer_begin, renderer_giveTask(model), renderer_end.  Model, texture and canvases
+stay resident in HBM; renderer_end returns with the frame drawn.  The renderer's tuning (tile
+shape, hipGraph replay) is the fastest of a short untimed sweep unless given by flags.
 
 Reported: shaded Mpixel/s (canvas pixels per second), Mtriangle/s (submitted triangles per
 second), frames/s, the whole frame's algorithmic GB/s, the roofline of the dominant kernel
@@ -191,6 +193,10 @@ def main() -> int:
     ap.add_argument("--cpu-baseline-only", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--tile-config", type=int, default=-1, help="renderer tile shape 0..2 (default: fastest in a sweep)")
+    ap.add_argument("--graph", type=int, default=-1, help="1/0: replay the frame as a hipGraph (default: fastest)")
+    ap.add_argument("--persistent", type=int, default=0, help="persistent raster workgroups per CU (0 = one per tile)")
+    ap.add_argument("--raster-path", type=int, default=0, help="0 visibility raster where allowed, 1 in-order")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES between two device buffers three times (calibrates HBM counters)")
     args = ap.parse_args()
@@ -218,27 +224,50 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     sc = scenes.terrain()
     gpu_scene = api.GpuScene(sc)
-    renderer = api.renderer_create()
     color = torch.zeros((HEIGHT, WIDTH), dtype=torch.int32, device="cuda")
     depth = torch.zeros((HEIGHT, WIDTH), dtype=torch.float32, device="cuda")
     color_im, depth_im = api.device_image(color), api.device_image(depth)
 
-    def step():
-        color.zero_()
-        depth.zero_()
-        api.renderer_begin(renderer, color_im, depth_im)
-        gpu_scene.give(renderer)
-        api.renderer_end(renderer)
+    def make_renderer(tile_config, persistent, graph, path):
+        r = api.renderer_create()
+        # clear = image_fill of both canvases (colour 0, depth 0 = infinitely far) inside the frame.
+        api.renderer_setTuning(r, api.tuning(tile_config=tile_config, persistent_per_cu=persistent, use_graph=graph,
+                                             clear=True, raster_path=path))
+        return r
+
+    def step(r):
+        api.renderer_begin(r, color_im, depth_im)
+        gpu_scene.give(r)
+        api.renderer_end(r)
+
+    # Untimed sweep of the renderer's tunings; the timed run uses the fastest (or the flags').
+    sweep = {}
+    for tile_config in (0, 1, 2):
+        for graph in (False, True):
+            r = make_renderer(tile_config, args.persistent, graph, args.raster_path)
+            for _ in range(3):
+                step(r)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(10):
+                step(r)
+            torch.cuda.synchronize()
+            sweep[(tile_config, graph)] = (time.perf_counter() - t0) / 10 * 1e3
+    best = min(sweep, key=sweep.get)
+    tile_config = best[0] if args.tile_config < 0 else args.tile_config
+    graph = best[1] if args.graph < 0 else bool(args.graph)
+    progress("plain sweep (ms/frame): " + ", ".join(f"tile{k[0]}{'+graph' if k[1] else ''}={v:.3f}" for k, v in sweep.items()))
+    renderer = make_renderer(tile_config, args.persistent, graph, args.raster_path)
 
     for _ in range(args.warmup):
-        step()
+        step(renderer)
     renderer.set_timing(True)
     raster_ms = []
     stage_ms = np.zeros(5)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        step(renderer)
         t = renderer.timing()
         raster_ms.append(float(t[3]))
         stage_ms += t
@@ -246,7 +275,6 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     stats = renderer.stats()
     progress(f"plain: {elapsed / args.steps * 1e3:.3f} ms per frame")
-
     ms_per_step = elapsed / args.steps * 1e3
     raster_avg_ms = float(np.mean(raster_ms))
     achieved = RASTER_BYTES / (raster_avg_ms * 1e-3) / 1e9
@@ -265,7 +293,12 @@ def main() -> int:
         "data": "synthetic (seeded terrain, noise+checker texture)",
         "config": {"workload": "1920x1080 textured terrain frame, 256x256 grid, 130050 triangles, 1024^2 RGBA8 "
                                "texture with 11 mips, renderer_begin/giveTask/end with canvas clears",
-                   "width": WIDTH, "height": HEIGHT, "triangles": TRIANGLES, "parallelism": "single GPU"},
+                   "width": WIDTH, "height": HEIGHT, "triangles": TRIANGLES, "parallelism": "single GPU",
+                   "tuning": {"tile": ["16x16", "32x16", "32x32"][tile_config], "graph": bool(graph),
+                              "persistent_per_cu": args.persistent,
+                              "raster": "in-order" if args.raster_path else "visibility"}},
+        "plain_sweep_ms": {f"{['16x16', '32x16', '32x32'][k[0]]}{'+graph' if k[1] else ''}": round(v, 4)
+                           for k, v in sweep.items()},
         "mtri_per_s": round(TRIANGLES * args.steps / elapsed / 1e6, 3),
         "frames_per_s": round(args.steps / elapsed, 2),
         "frame_gbs": round(FRAME_BYTES / (ms_per_step * 1e-3) / 1e9, 2),
@@ -273,12 +306,12 @@ def main() -> int:
                      zip(["project", "setup", "binning", "raster", "device_frame"], stage_ms)},
         "draw_commands": int(stats[0]),
         "tile_pairs": int(stats[1]),
-        "roofline": {"bound": "hbm", "kernel": "k_raster", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_raster_deferred" if not args.raster_path else "k_raster", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                      "algorithmic_bytes": RASTER_BYTES, "avg_launch_ms": round(raster_avg_ms, 5)},
         "cpu_baseline": None,
     }
-    del gpu_scene, renderer, color, depth
+    del gpu_scene, renderer, color, depth, color_im, depth_im
     if not args.no_batch:
         line["batched"] = batched_form()
         torch.cuda.empty_cache()

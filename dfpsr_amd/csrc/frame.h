@@ -73,7 +73,8 @@ class Frame {
 public:
 	hipStream_t stream = nullptr;
 	int64_t stats[4] = {0, 0, 0, 0};     // commands, tile pairs, tiles, passes
-	float stageMs[5] = {0, 0, 0, 0, 0};  // projection, setup, binning, raster, whole pass
+	float stageMs[5] = {0, 0, 0, 0, 0};  // projection, setup, binning, raster, whole pass (graph replay:
+	                                     // 0, 0, everything before the raster, raster, whole)
 	bool timing = false;
 
 	~Frame();
@@ -122,11 +123,14 @@ private:
 	DevBuf dTables_;
 	size_t offTasks_ = 0, offFanEnd_ = 0, offPointEnd_ = 0, offCanvases_ = 0, offTileEnd_ = 0, offTextures_ = 0;
 	bool prepared_ = false;
+	DevBuf dShaders_;
 	DevBuf dCanvasBase_, dProjected_, dFanCount_, dBlockSums_, dCommands_, dCounters_, dTileCount_, dTileOffset_, dTileFill_, dTileList_, dTileQueue_;
 	int32_t *hostCounters_ = nullptr;
 	hipEvent_t events_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-	hipGraph_t graph_ = nullptr;
+	hipGraph_t graph_ = nullptr;  // the whole sequence, or (timing on) everything before the raster
 	hipGraphExec_t graphExec_ = nullptr;
+	hipGraph_t rasterGraph_ = nullptr;  // timing on: the raster alone
+	hipGraphExec_t rasterGraphExec_ = nullptr;
 	std::vector<uint8_t> graphKey_;
 	int32_t deviceCUs_ = 0;
 	const float *ftex_ = nullptr;
@@ -139,7 +143,8 @@ private:
 	std::vector<uint8_t> bwdGraphKey_;
 
 	int32_t textureSlot(const TextureDesc *desc);
-	int enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks);
+	// parts: 1 = everything before the raster, 2 = the raster, 3 = both.
+	int enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks, bool events, int parts);
 	void dropGraph();
 	// The visibility raster reproduces the in-order result when every command is Solid and
 	// perspective and no canvas has an odd height.

@@ -32,8 +32,12 @@ int Frame::init() {
 void Frame::dropGraph() {
 	if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
 	if (graph_) (void)hipGraphDestroy(graph_);
+	if (rasterGraphExec_) (void)hipGraphExecDestroy(rasterGraphExec_);
+	if (rasterGraph_) (void)hipGraphDestroy(rasterGraph_);
 	graphExec_ = nullptr;
 	graph_ = nullptr;
+	rasterGraphExec_ = nullptr;
+	rasterGraph_ = nullptr;
 	graphKey_.clear();
 	if (bwdGraphExec_) (void)hipGraphExecDestroy(bwdGraphExec_);
 	if (bwdGraph_) (void)hipGraphDestroy(bwdGraph_);
@@ -125,10 +129,10 @@ int Frame::addTask(const ModelView &m, const float *transform, const CameraParam
 static inline int32_t blocksOf(int64_t n, int32_t size) { return (int32_t)((n + size - 1) / size); }
 static inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks) {
-	const bool events = timing && !o.graph;
+int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks, bool events, int parts) {
 	const int32_t tileTotal = f.tileTotal;
-	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, f.counters, 8, 0);
+	if (parts & 1) {
+	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, f.counters, COUNTERS, 0);
 	hipLaunchKernelGGL(k_fill, dim3(blocksOf(2 * (int64_t)tileTotal + 1, 256)), dim3(256), 0, stream, f.tileCount, 2 * tileTotal + 1, 0);
 	hipLaunchKernelGGL(k_fill, dim3(blocksOf(f.canvasCount, 256)), dim3(256), 0, stream, f.canvasCommandBase, f.canvasCount, 0x7FFFFFFF);
 	if (o.clear) hipLaunchKernelGGL(k_clear_canvases, dim3(tileTotal), dim3(256), 0, stream, f, o.clearColor, o.clearDepth);
@@ -141,12 +145,18 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	if (events) HIP_TRY(hipEventRecord(events_[2], stream));
 	hipLaunchKernelGGL(k_bin_count, dim3(blocksOf(f.commandCapacity, 256)), dim3(256), 0, stream, f);
 	hipLaunchKernelGGL(k_scan_tiles_local, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
-	hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, tileBlocks, f.counters + 1);
+	hipLaunchKernelGGL(k_scan_block_sums64, dim3(1), dim3(SCAN_BLOCK), 0, stream, (long long *)f.blockSums, tileBlocks,
+	                   f.counters + 1, f.counters + 3);
 	hipLaunchKernelGGL(k_scan_tiles_add, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
 	hipLaunchKernelGGL(k_bin_fill, dim3(blocksOf(f.commandCapacity, 256)), dim3(256), 0, stream, f);
 	if (events) HIP_TRY(hipEventRecord(events_[3], stream));
+	}
+	if (!(parts & 2)) {
+		HIP_TRY(hipGetLastError());
+		return 0;
+	}
 	const int32_t persistent = o.persistentPerCU > 0 ? 1 : 0;
-	const int32_t grid = persistent ? deviceCUs_ * o.persistentPerCU : tileTotal;
+	const int32_t grid = persistent ? deviceCUs_ * o.persistentPerCU : XCDS * blocksOf(tileTotal, XCDS);
 	if (o.mode == 1) {
 		hipLaunchKernelGGL(k_raster_diff, dim3(grid), dim3(64), 0, stream, f, persistent);
 	} else if (deferredRaster(o)) {
@@ -219,14 +229,15 @@ int Frame::run(const RenderOptions &o) {
 	if (pairCapacity_ < 4 * commandCapacity_) pairCapacity_ = 4 * commandCapacity_;
 	if (dProjected_.ensure((size_t)pointTotal * sizeof(Projected)) != hipSuccess ||
 	    dFanCount_.ensure((size_t)fanTotal * 4) != hipSuccess ||
-	    dBlockSums_.ensure((size_t)(fanBlocks > tileBlocks ? fanBlocks : tileBlocks) * 4) != hipSuccess ||
-	    dCounters_.ensure(8 * 4) != hipSuccess || dTileCount_.ensure((2 * (size_t)tileTotal + 1) * 4) != hipSuccess ||
+	    dBlockSums_.ensure((size_t)(fanBlocks > tileBlocks ? fanBlocks : tileBlocks) * 8) != hipSuccess ||
+	    dCounters_.ensure(COUNTERS * 4) != hipSuccess || dTileCount_.ensure((2 * (size_t)tileTotal + 1) * 4) != hipSuccess ||
 	    dTileOffset_.ensure(((size_t)tileTotal + 1) * 4) != hipSuccess || dTileQueue_.ensure((size_t)tileTotal * 4) != hipSuccess ||
 	    dCanvasBase_.ensure(nC * 4) != hipSuccess)
 		return fail(-4, "device allocation failed");
 
 	for (int pass = 1; pass <= 3; pass++) {
 		if (dCommands_.ensure((size_t)commandCapacity_ * sizeof(Command)) != hipSuccess ||
+		    dShaders_.ensure((size_t)commandCapacity_ * sizeof(ShaderInputs)) != hipSuccess ||
 		    dTileList_.ensure((size_t)pairCapacity_ * 4) != hipSuccess)
 			return fail(-4, "device allocation failed");
 		FrameDev f;
@@ -246,6 +257,7 @@ int Frame::run(const RenderOptions &o) {
 		f.fanCount = dFanCount_.as<int32_t>();
 		f.blockSums = dBlockSums_.as<int32_t>();
 		f.commands = dCommands_.as<Command>();
+		f.shaders = dShaders_.as<ShaderInputs>();
 		f.commandCapacity = commandCapacity_;
 		f.counters = dCounters_.as<int32_t>();
 		f.queueHead = dCounters_.as<int32_t>() + 4;
@@ -268,28 +280,46 @@ int Frame::run(const RenderOptions &o) {
 		lastRunValid_ = false;
 
 		if (o.graph) {
-			// The graph stays valid while the launch arguments and options are the same.
-			std::vector<uint8_t> key(sizeof(FrameDev) + sizeof(RenderOptions) + sizeof(int32_t));
+			// The graphs stay valid while the launch arguments, the options and the timing switch
+			// are the same.  Events recorded inside a graph cannot be timed, so with timing on the
+			// sequence is captured as two graphs -- everything before the raster, and the raster --
+			// with the events recorded between their launches.
+			const uint8_t split = timing ? 1 : 0;
+			std::vector<uint8_t> key(sizeof(FrameDev) + sizeof(RenderOptions) + sizeof(int32_t) + 1);
 			std::memcpy(key.data(), &f, sizeof(FrameDev));
 			std::memcpy(key.data() + sizeof(FrameDev), &o, sizeof(RenderOptions));
 			std::memcpy(key.data() + sizeof(FrameDev) + sizeof(RenderOptions), &tileConfig_, sizeof(int32_t));
+			key.back() = split;
 			if (!graphExec_ || key != graphKey_) {
 				dropGraph();
-				HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-				int rc = enqueue(f, o, fanBlocks, tileBlocks);
-				hipGraph_t g = nullptr;
-				hipError_t e = hipStreamEndCapture(stream, &g);
-				if (rc) return rc;
-				if (e != hipSuccess) return fail(-3, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-				graph_ = g;
-				HIP_TRY(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
+				for (int g = 0; g < (split ? 2 : 1); g++) {
+					HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+					int rc = enqueue(f, o, fanBlocks, tileBlocks, false, split ? (g == 0 ? 1 : 2) : 3);
+					hipGraph_t captured = nullptr;
+					hipError_t e = hipStreamEndCapture(stream, &captured);
+					if (rc) {
+						if (captured) (void)hipGraphDestroy(captured);
+						return rc;
+					}
+					if (e != hipSuccess) return fail(-3, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+					hipGraph_t &slotGraph = g == 0 ? graph_ : rasterGraph_;
+					hipGraphExec_t &slotExec = g == 0 ? graphExec_ : rasterGraphExec_;
+					slotGraph = captured;
+					HIP_TRY(hipGraphInstantiate(&slotExec, slotGraph, nullptr, nullptr, 0));
+				}
 				graphKey_ = key;
 			}
-			if (timing) HIP_TRY(hipEventRecord(events_[0], stream));
-			HIP_TRY(hipGraphLaunch(graphExec_, stream));
-			if (timing) HIP_TRY(hipEventRecord(events_[4], stream));
+			if (split) {
+				HIP_TRY(hipEventRecord(events_[0], stream));
+				HIP_TRY(hipGraphLaunch(graphExec_, stream));
+				HIP_TRY(hipEventRecord(events_[3], stream));
+				HIP_TRY(hipGraphLaunch(rasterGraphExec_, stream));
+				HIP_TRY(hipEventRecord(events_[4], stream));
+			} else {
+				HIP_TRY(hipGraphLaunch(graphExec_, stream));
+			}
 		} else {
-			int rc = enqueue(f, o, fanBlocks, tileBlocks);
+			int rc = enqueue(f, o, fanBlocks, tileBlocks, timing, 3);
 			if (rc) return rc;
 		}
 		HIP_TRY(hipMemcpyAsync(hostCounters_, f.counters, 16, hipMemcpyDeviceToHost, stream));
@@ -299,8 +329,14 @@ int Frame::run(const RenderOptions &o) {
 		stats[2] = hostCounters_[3];
 		stats[3] = pass;
 		if (timing) {
-			if (!o.graph)
+			if (o.graph) {
+				// The graph's stages before the raster are timed as one (reported as binning).
+				stageMs[0] = stageMs[1] = 0.0f;
+				HIP_TRY(hipEventElapsedTime(&stageMs[2], events_[0], events_[3]));
+				HIP_TRY(hipEventElapsedTime(&stageMs[3], events_[3], events_[4]));
+			} else {
 				for (int k = 0; k < 4; k++) HIP_TRY(hipEventElapsedTime(&stageMs[k], events_[k], events_[k + 1]));
+			}
 			HIP_TRY(hipEventElapsedTime(&stageMs[4], events_[0], events_[4]));
 		}
 		int32_t overflow = hostCounters_[2];

@@ -32,9 +32,31 @@ __global__ void k_project(FrameDev f) {
 // -> drawSubTriangle (renderCore.cpp:220-341).  Returns the number of draw commands the fan
 // triangle makes; with EMIT they are written to out[0..count).
 // ---------------------------------------------------------------------------------------
+// The finished shader inputs of fan triangle (poly, k) of a task (Shader.h:37-57,
+// RgbaMultiply.h:37-58): corner colours and texture coordinates, texture slots, the shader
+// variant flags.
+__device__ ShaderInputs triangleShader(const TaskDev &t, int32_t poly, int32_t k) {
+	const float *col = t.colors + poly * 16;
+	const float *tc = t.texcoords + poly * 16;
+	const int32_t ca = 0, cb = (1 + k) * 4, cc = (2 + k) * 4;
+	ShaderInputs si;
+	si.red = make3(col[ca], col[cb], col[cc]);
+	si.green = make3(col[ca + 1], col[cb + 1], col[cc + 1]);
+	si.blue = make3(col[ca + 2], col[cb + 2], col[cc + 2]);
+	si.alpha = make3(col[ca + 3], col[cb + 3], col[cc + 3]);
+	si.u1 = make3(tc[ca], tc[cb], tc[cc]);
+	si.v1 = make3(tc[ca + 1], tc[cb + 1], tc[cc + 1]);
+	si.u2 = make3(tc[ca + 2], tc[cb + 2], tc[cc + 2]);
+	si.v2 = make3(tc[ca + 3], tc[cb + 3], tc[cc + 3]);
+	int32_t part = t.polygonPart[poly];
+	si.diffuse = t.partTexture[part * 2];
+	si.light = t.partTexture[part * 2 + 1];
+	finishShaderInputs(si);
+	return si;
+}
+
 template <bool EMIT>
-__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, Command *out) {
-	int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
+__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, Command *out) {
 	const TaskDev &t = f.tasks[ti];
 	const CanvasDev &cv = f.canvases[t.canvas];
 	if (cv.color == nullptr && cv.depth == nullptr) return 0;
@@ -109,7 +131,7 @@ __device__ int32_t blockExclusiveScan(int32_t value, int32_t &exclusive) {
 
 __global__ void k_setup_count(FrameDev f) {
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
-	int32_t n = (g < f.fanTotal) ? setupTriangle<false>(f, g, nullptr) : 0;
+	int32_t n = (g < f.fanTotal) ? setupTriangle<false>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), nullptr) : 0;
 	if (g < f.fanTotal) f.fanCount[g] = n;
 	int32_t ex;
 	int32_t total = blockExclusiveScan(n, ex);
@@ -135,20 +157,65 @@ __global__ void k_scan_block_sums(int32_t *blockSums, int32_t blocks, int32_t *t
 	if (threadIdx.x == 0) *total = carry;
 }
 
+// The same over int64 values.
+__device__ long long blockExclusiveScan64(long long value, long long &exclusive) {
+	__shared__ long long s[SCAN_BLOCK];
+	int32_t tid = threadIdx.x;
+	s[tid] = value;
+	__syncthreads();
+	for (int32_t offset = 1; offset < SCAN_BLOCK; offset <<= 1) {
+		long long v = (tid >= offset) ? s[tid - offset] : 0;
+		__syncthreads();
+		s[tid] += v;
+		__syncthreads();
+	}
+	long long inclusive = s[tid];
+	long long total = s[SCAN_BLOCK - 1];
+	__syncthreads();
+	exclusive = inclusive - value;
+	return total;
+}
+
+// Exclusive scan of int64 block sums by one workgroup; writes the two 32-bit halves of the
+// grand total.
+__global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_t *totalLow, int32_t *totalHigh) {
+	__shared__ long long carry;
+	if (threadIdx.x == 0) carry = 0;
+	__syncthreads();
+	for (int32_t base = 0; base < blocks; base += SCAN_BLOCK) {
+		int32_t i = base + threadIdx.x;
+		long long v = (i < blocks) ? blockSums[i] : 0;
+		long long ex;
+		long long sum = blockExclusiveScan64(v, ex);
+		long long c = carry;
+		if (i < blocks) blockSums[i] = c + ex;
+		__syncthreads();
+		if (threadIdx.x == 0) carry = c + sum;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		*totalLow = (int32_t)(carry & 0xFFFFFFFFll);
+		*totalHigh = (int32_t)(carry >> 32);
+	}
+}
+
 __global__ void k_setup_emit(FrameDev f) {
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	int32_t n = (g < f.fanTotal) ? f.fanCount[g] : 0;
 	int32_t ex;
 	blockExclusiveScan(n, ex);
+	if (g >= f.fanTotal) return;
+	const int32_t offset = f.blockSums[blockIdx.x] + ex;
+	const int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
+	// The commands of a canvas's tasks are consecutive, so the offset at the first fan
+	// triangle of each task bounds its canvas's first command: one atomic per task.
+	if (g == f.tasks[ti].fanBase) atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
 	if (n == 0) return;
-	int32_t offset = f.blockSums[blockIdx.x] + ex;
 	if (offset + n > f.commandCapacity) {
 		atomicOr(&f.counters[2], 1);
 		return;
 	}
-	const int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
-	atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
-	setupTriangle<true>(f, g, f.commands + offset);
+	setupTriangle<true>(f, g, ti, f.commands + offset);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -178,10 +245,16 @@ DH TileRange commandTiles(const Command &c, const CanvasDev &cv, int32_t tw, int
 	return r;
 }
 
+// Counts each command's tiles and finishes its shader inputs for the raster.
 __global__ void k_bin_count(FrameDev f) {
 	int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c >= f.counters[0] || f.counters[2]) return;
 	const Command &cmd = f.commands[c];
+	if (f.mode == 0) {
+		const TaskDev &t = f.tasks[cmd.task];
+		const int32_t fp = t.fanPolygon[cmd.fan];
+		f.shaders[c] = triangleShader(t, fp >> 1, fp & 1);
+	}
 	const CanvasDev &cv = f.canvases[cmd.canvas];
 	TileRange r = commandTiles(cmd, cv, f.tileW, f.tileH);
 	if (!r.any) return;
@@ -189,20 +262,28 @@ __global__ void k_bin_count(FrameDev f) {
 		for (int32_t tx = r.tx0; tx <= r.tx1; tx++) atomicAdd(&f.tileCount[cv.tileBase + ty * cv.tilesX + tx], 1);
 }
 
+// Tile offsets and the ordered queue of non-empty tiles come from one scan of
+// (count | nonEmpty << 32); the block-local queue position waits in tileFill, which the
+// list fill uses as its cursors afterwards.
 __global__ void k_scan_tiles_local(FrameDev f) {
 	int32_t i = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	int32_t v = (i < f.tileTotal) ? f.tileCount[i] : 0;
-	int32_t ex;
-	int32_t total = blockExclusiveScan(v, ex);
-	if (i < f.tileTotal) f.tileOffset[i] = ex;
-	if (threadIdx.x == 0) f.blockSums[blockIdx.x] = total;
+	long long ex;
+	long long total = blockExclusiveScan64((long long)v | ((long long)(v > 0) << 32), ex);
+	if (i < f.tileTotal) {
+		f.tileOffset[i] = (int32_t)(ex & 0xFFFFFFFFll);
+		f.tileFill[i] = (int32_t)(ex >> 32);
+	}
+	if (threadIdx.x == 0) ((long long *)f.blockSums)[blockIdx.x] = total;
 }
 
 __global__ void k_scan_tiles_add(FrameDev f) {
 	int32_t i = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	if (i < f.tileTotal) {
-		f.tileOffset[i] += f.blockSums[blockIdx.x];
-		if (f.tileCount[i] > 0) f.tileQueue[atomicAdd(&f.counters[3], 1)] = i;
+		const long long base = ((const long long *)f.blockSums)[blockIdx.x];
+		f.tileOffset[i] += (int32_t)(base & 0xFFFFFFFFll);
+		if (f.tileCount[i] > 0) f.tileQueue[(int32_t)(base >> 32) + f.tileFill[i]] = i;
+		f.tileFill[i] = 0;
 	}
 	if (i == f.tileTotal - 1) {
 		int32_t end = f.tileOffset[i] + f.tileCount[i];
@@ -385,44 +466,27 @@ __device__ __forceinline__ bool quadVisibility(const QuadEval &e, const QuadPixe
 	return anyVis;
 }
 
-// Shader of one quad of one command for all four lanes: RgbaMultiply's getPixels_2x2 with
-// the vertex weights of the lanes (RgbaMultiply.h:74-175, shaderMethods.h:39-94).
-__device__ __forceinline__ void shadeCommand(const FrameDev &f, const Command &cmd, const TaskDev &task, const QuadEval &e,
-                                             bool affine, float *r, float *g, float *b, float *a) {
+// Shader of one quad of command c for all four lanes: RgbaMultiply's getPixels_2x2 with
+// the vertex weights of the lanes (RgbaMultiply.h:74-175, shaderMethods.h:39-94).  The
+// shader inputs were finished once per source triangle at setup.
+__device__ __forceinline__ void shadeCommand(const FrameDev &f, int32_t c, const QuadEval &e, bool affine,
+                                             float *r, float *g, float *b, float *a) {
 	float wa[4], wb[4], wc[4];
 	for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
-	// Shader inputs of the source triangle (Shader.h:37-57, RgbaMultiply.h:37-58).
-	int32_t fp = task.fanPolygon[cmd.fan];
-	int32_t poly = fp >> 1, k = fp & 1;
-	const float *col = task.colors + poly * 16;
-	const float *tc = task.texcoords + poly * 16;
-	const int32_t ca = 0, cb = (1 + k) * 4, cc = (2 + k) * 4;
-	ShaderInputs si;
-	si.red = make3(col[ca], col[cb], col[cc]);
-	si.green = make3(col[ca + 1], col[cb + 1], col[cc + 1]);
-	si.blue = make3(col[ca + 2], col[cb + 2], col[cc + 2]);
-	si.alpha = make3(col[ca + 3], col[cb + 3], col[cc + 3]);
-	si.u1 = make3(tc[ca], tc[cb], tc[cc]);
-	si.v1 = make3(tc[ca + 1], tc[cb + 1], tc[cc + 1]);
-	si.u2 = make3(tc[ca + 2], tc[cb + 2], tc[cc + 2]);
-	si.v2 = make3(tc[ca + 3], tc[cb + 3], tc[cc + 3]);
-	int32_t part = task.polygonPart[poly];
-	si.diffuse = task.partTexture[part * 2];
-	si.light = task.partTexture[part * 2 + 1];
-	finishShaderInputs(si);
+	const ShaderInputs si = f.shaders[c];
 	shadeQuad(si, f.textures, wa, wb, wc, r, g, b, a);
 }
 
 // The reference's shading of one quad (fillQuadSuper, fillerTemplates.h:154-187, with the
 // RgbaMultiply shaders of RgbaMultiply.h:74-175).
-__device__ void drawCommand(const FrameDev &f, const Command &cmd, const TaskDev &task, const QuadEval &e,
+__device__ void drawCommand(const FrameDev &f, const Command &cmd, int32_t c, const QuadEval &e,
                             QuadPixels<0> &q, bool colorWrite, bool depthRead, bool depthWrite, bool alphaFilter) {
 	const bool affine = (cmd.flags & 1) != 0;
 	bool vis[4];
 	if (!quadVisibility<0>(e, q, depthRead, affine, vis)) return;
 	if (colorWrite) {
 		float r[4], g[4], b[4], a[4];
-		shadeCommand(f, cmd, task, e, affine, r, g, b, a);
+		shadeCommand(f, c, e, affine, r, g, b, a);
 		if (alphaFilter) {
 			float tr[4], tg[4], tb[4], ta[4];
 			for (int l = 0; l < 4; l++) {
@@ -650,7 +714,7 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 			__syncthreads();
 			QuadEval e;
 			if (py >= al.t && py < al.bottom() && evalQuad(cmd, rows[py - tileY0], rows[py + 1 - tileY0], px, py, cv.height, e)) {
-				if constexpr (MODE == 0) drawCommand(f, cmd, task, e, q, colorWrite, depthRead, depthWrite, alphaFilter);
+				if constexpr (MODE == 0) drawCommand(f, cmd, c, e, q, colorWrite, depthRead, depthWrite, alphaFilter);
 				else drawCommandDiff(f, cmd, c - f.canvasCommandBase[ci], task, e, q, depthRead, depthWrite);
 			}
 		}
@@ -678,6 +742,37 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 // queue slot b (the grid covers every tile; surplus workgroups exit).  persistent = 1: a fixed
 // grid whose workgroups pull slots from an atomic head until the queue is empty, so every
 // workgroup reaches the exit once all tiles are handed out.
+// Runs body(tile) over the ordered queue of non-empty tiles, XCD by XCD.  Without
+// persistence the grid has 8 * ceil(tiles / 8) workgroups and workgroup b takes slot
+// b / 8 of its XCD's range.  Persistent workgroups pull slots from their own XCD's head and
+// then from the other XCDs' heads; every head ends past its range, so all of them exit.
+template <typename Body>
+__device__ void forEachQueuedTile(const FrameDev &f, int32_t persistent, int32_t *slotLds, Body body) {
+	const int32_t queued = f.counters[3];
+	const int32_t x = (int32_t)(blockIdx.x % XCDS);
+	if (!persistent) {
+		int32_t begin, end;
+		xcdRange(queued, x, begin, end);
+		const int32_t slot = begin + (int32_t)(blockIdx.x / XCDS);
+		if (slot < end) body(f.tileQueue[slot]);
+		return;
+	}
+	for (int32_t r = 0; r < XCDS; r++) {
+		const int32_t xr = (x + r) % XCDS;
+		int32_t begin, end;
+		xcdRange(queued, xr, begin, end);
+		while (true) {
+			if (threadIdx.x == 0) *slotLds = begin + atomicAdd(&f.counters[16 + xr], 1);
+			__syncthreads();
+			const int32_t slot = *slotLds;
+			__syncthreads();
+			if (slot >= end) break;
+			body(f.tileQueue[slot]);
+			__syncthreads();
+		}
+	}
+}
+
 template <int TW, int TH>
 __global__ void __launch_bounds__((TW / 2) * (TH / 2)) k_raster(FrameDev f, int32_t persistent) {
 	if (f.mode == 1) return;  // the differentiable frames use k_raster_diff
@@ -686,20 +781,7 @@ __global__ void __launch_bounds__((TW / 2) * (TH / 2)) k_raster(FrameDev f, int3
 	__shared__ int32_t roundInfo[3];
 	__shared__ int32_t slotLds;
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
-	const int32_t queued = f.counters[3];
-	int32_t slot = blockIdx.x;
-	while (true) {
-		if (persistent) {
-			if (threadIdx.x == 0) slotLds = atomicAdd(f.queueHead, 1);
-			__syncthreads();
-			slot = slotLds;
-			__syncthreads();
-		}
-		if (slot >= queued) break;
-		rasterTile<TW, TH, 0>(f, f.tileQueue[slot], list, rows, roundInfo);
-		if (!persistent) break;
-		__syncthreads();
-	}
+	forEachQueuedTile(f, persistent, &slotLds, [&](int32_t tile) { rasterTile<TW, TH, 0>(f, tile, list, rows, roundInfo); });
 }
 
 template __global__ void k_raster<16, 16>(FrameDev, int32_t);
@@ -726,7 +808,7 @@ template __global__ void k_raster<32, 32>(FrameDev, int32_t);
 // onto the row above within one command) take the in-order kernel above.
 // ---------------------------------------------------------------------------------------
 
-constexpr int32_t DEFER_CHUNK = 64;
+constexpr int32_t DEFER_CHUNK = 32;
 
 // Keys order depths as floats compare: larger depth, larger key; -0 equals +0.
 __device__ __forceinline__ uint32_t depthOrder(float d) {
@@ -736,10 +818,10 @@ __device__ __forceinline__ uint32_t depthOrder(float d) {
 
 template <int TW, int TH>
 __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, int32_t *list, RowInterval *rows,
-                                   unsigned long long *keys, uint32_t *colorT, float *depthT, uint32_t *dirty,
-                                   int32_t *roundInfo) {
+                                   unsigned long long *keys, int32_t *roundInfo) {
 	constexpr int32_t NQX = TW / 2, NQ = (TW / 2) * (TH / 2), NP = TW * TH;
 	constexpr int32_t WAVES = DEFER_THREADS / 64;
+	constexpr int32_t PPT = NP / DEFER_THREADS;  // pixels per thread
 	const int32_t n = f.tileCount[tile];
 	const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
 	const CanvasDev cv = f.canvases[ci];
@@ -753,10 +835,16 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, int32_t *lis
 	const bool colorWrite = cv.color != nullptr;
 	const bool depthRead = cv.depth != nullptr;
 
-	for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+	// Thread t holds pixels t, t + 256, ... of the tile (row-major) in registers.
+	float depth[PPT];
+	uint32_t color[PPT];
+	bool dirty[PPT];
+	for (int32_t i = 0; i < PPT; i++) {
+		const int32_t p = t + i * DEFER_THREADS;
 		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
-		depthT[p] = (depthRead && x < cv.width && y < cv.height) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
-		dirty[p] = 0;
+		depth[i] = (depthRead && x < cv.width && y < cv.height) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
+		color[i] = 0u;
+		dirty[i] = false;
 	}
 
 	int32_t last = -1;
@@ -767,12 +855,13 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, int32_t *lis
 			const int32_t cn = take - c0 < DEFER_CHUNK ? take - c0 : DEFER_CHUNK;
 			// Keys start at the held depth with the lowest command part above every command's,
 			// so only a strictly greater depth replaces it; a NaN depth is never replaced.
-			for (int32_t p = t; p < NP; p += DEFER_THREADS) {
-				const float d = depthT[p];
-				keys[p] = !depthRead ? 0ull : (d != d ? ~0ull : (((unsigned long long)depthOrder(d) << 32) | 0xFFFFFFFFull));
+			for (int32_t i = 0; i < PPT; i++) {
+				const float d = depth[i];
+				keys[t + i * DEFER_THREADS] =
+				    !depthRead ? 0ull : (d != d ? ~0ull : (((unsigned long long)depthOrder(d) << 32) | 0xFFFFFFFFull));
 			}
 			__syncthreads();
-			// Pass 1: wave w offers the coverage of chunk commands w, w + WAVES, ...
+			// Pass 1: wave w offers the coverage of chunk commands w, w + WAVES, ... (a lane per quad).
 			for (int32_t j = wave; j < cn; j += WAVES) {
 				const int32_t c = __builtin_amdgcn_readfirstlane(list[c0 + j]);
 				const Command cmd = f.commands[c];
@@ -815,46 +904,40 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, int32_t *lis
 				}
 			}
 			__syncthreads();
-			// Pass 2: shade each quad once per distinct winning command.
-			for (int32_t s2 = t; s2 < NQ * 4; s2 += DEFER_THREADS) {
-				const int32_t qi = s2 % NQ, k = s2 / NQ;
-				const int32_t px = tileX0 + 2 * (qi % NQX), py = tileY0 + 2 * (qi / NQX);
-				int32_t win[4];
-				for (int s = 0; s < 4; s++) {
-					const unsigned long long key = keys[(py - tileY0 + (s >> 1)) * TW + (px - tileX0 + (s & 1))];
-					if (depthRead) {
-						const uint32_t low = (uint32_t)key;
-						win[s] = low == 0xFFFFFFFFu ? -1 : (int32_t)(0xFFFFFFFEu - low);
-					} else {
-						win[s] = (int32_t)key - 1;
-					}
-				}
-				// The k-th distinct winner in pixel order.
-				int32_t jw = -1, found = 0;
-				for (int s = 0; s < 4 && jw < 0; s++) {
-					if (win[s] < 0) continue;
-					bool seen = false;
-					for (int r = 0; r < s; r++) seen |= win[r] == win[s];
-					if (seen) continue;
-					if (found == k) jw = win[s];
-					found++;
+			// Pass 2: each pixel evaluates its winning command's quad and shades its own lane.
+			for (int32_t i = 0; i < PPT; i++) {
+				const int32_t p = t + i * DEFER_THREADS;
+				const unsigned long long key = keys[p];
+				int32_t jw;
+				if (depthRead) {
+					const uint32_t low = (uint32_t)key;
+					jw = low == 0xFFFFFFFFu ? -1 : (int32_t)(0xFFFFFFFEu - low);
+				} else {
+					jw = (int32_t)key - 1;
 				}
 				if (jw < 0) continue;
-				const Command cmd = f.commands[list[c0 + jw]];
+				const int32_t xl = p % TW, yl = p / TW;
+				const int32_t px = tileX0 + (xl & ~1), py = tileY0 + (yl & ~1);
+				const int32_t s = (yl & 1) * 2 + (xl & 1);
+				const int32_t cw = list[c0 + jw];
+				const Command cmd = f.commands[cw];
 				const RowInterval *cr = rows + jw * TH;
 				QuadEval e;
 				if (!evalQuad(cmd, cr[py - tileY0], cr[py + 1 - tileY0], px, py, cv.height, e)) continue;
-				float r[4], g[4], b[4], a[4];
-				if (colorWrite) shadeCommand(f, cmd, f.tasks[cmd.task], e, (cmd.flags & 1) != 0, r, g, b, a);
-				for (int l = 0; l < 4; l++) {
-					if (e.clipSides && !e.clip[l]) continue;
-					const int32_t s = e.slot[l];
-					if (win[s] != jw) continue;
-					const int32_t p = (py - tileY0 + (s >> 1)) * TW + (px - tileX0 + (s & 1));
-					if (colorWrite) colorT[p] = packSaturated(r[l], g[l], b[l], a[l]);
-					if (depthRead) depthT[p] = e.l0[l];
-					dirty[p] = 1;
+				int l = -1;
+				for (int k = 0; k < 4; k++)
+					if ((!e.clipSides || e.clip[k]) && e.slot[k] == s) l = k;
+				if (l < 0) continue;
+				if (colorWrite) {
+					const bool affine = (cmd.flags & 1) != 0;
+					float wa[4], wb[4], wc[4];
+					for (int k = 0; k < 4; k++) laneWeights(e, k, affine, wa[k], wb[k], wc[k]);
+					float r, g, b, a;
+					shadeLane(f.shaders[cw], f.textures, wa, wb, wc, l, r, g, b, a);
+					color[i] = packSaturated(r, g, b, a);
 				}
+				depth[i] = pickLane(e.l0, l);
+				dirty[i] = true;
 			}
 			__syncthreads();
 		}
@@ -862,39 +945,25 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, int32_t *lis
 		if (take == 0) break;
 	}
 
-	for (int32_t p = t; p < NP; p += DEFER_THREADS) {
-		if (!dirty[p]) continue;
+	for (int32_t i = 0; i < PPT; i++) {
+		if (!dirty[i]) continue;
+		const int32_t p = t + i * DEFER_THREADS;
 		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
-		if (colorWrite) cv.color[(size_t)y * cv.colorStride + x] = colorT[p];
-		if (depthRead) cv.depth[(size_t)y * cv.depthStride + x] = depthT[p];
+		if (colorWrite) cv.color[(size_t)y * cv.colorStride + x] = color[i];
+		if (depthRead) cv.depth[(size_t)y * cv.depthStride + x] = depth[i];
 	}
 }
 
+// Five waves per SIMD is the most the kernel holds without spilling registers.
 template <int TW, int TH>
-__global__ void __launch_bounds__(DEFER_THREADS) k_raster_deferred(FrameDev f, int32_t persistent) {
+__global__ void __launch_bounds__(DEFER_THREADS) __attribute__((amdgpu_waves_per_eu(5, 5))) k_raster_deferred(FrameDev f, int32_t persistent) {
 	__shared__ int32_t list[TILE_SORT_CAP];
 	__shared__ RowInterval rows[DEFER_CHUNK * TH];
 	__shared__ unsigned long long keys[TW * TH];
-	__shared__ uint32_t colorT[TW * TH];
-	__shared__ float depthT[TW * TH];
-	__shared__ uint32_t dirty[TW * TH];
 	__shared__ int32_t roundInfo[3];
 	__shared__ int32_t slotLds;
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
-	const int32_t queued = f.counters[3];
-	int32_t slot = blockIdx.x;
-	while (true) {
-		if (persistent) {
-			if (threadIdx.x == 0) slotLds = atomicAdd(f.queueHead, 1);
-			__syncthreads();
-			slot = slotLds;
-			__syncthreads();
-		}
-		if (slot >= queued) break;
-		rasterTileDeferred<TW, TH>(f, f.tileQueue[slot], list, rows, keys, colorT, depthT, dirty, roundInfo);
-		if (!persistent) break;
-		__syncthreads();
-	}
+	forEachQueuedTile(f, persistent, &slotLds, [&](int32_t tile) { rasterTileDeferred<TW, TH>(f, tile, list, rows, keys, roundInfo); });
 }
 
 template __global__ void k_raster_deferred<16, 16>(FrameDev, int32_t);
@@ -908,20 +977,7 @@ __global__ void __launch_bounds__(64) k_raster_diff(FrameDev f, int32_t persiste
 	__shared__ int32_t roundInfo[3];
 	__shared__ int32_t slotLds;
 	if (f.counters[2]) return;
-	const int32_t queued = f.counters[3];
-	int32_t slot = blockIdx.x;
-	while (true) {
-		if (persistent) {
-			if (threadIdx.x == 0) slotLds = atomicAdd(f.queueHead, 1);
-			__syncthreads();
-			slot = slotLds;
-			__syncthreads();
-		}
-		if (slot >= queued) break;
-		rasterTile<16, 16, 1>(f, f.tileQueue[slot], list, rows, roundInfo);
-		if (!persistent) break;
-		__syncthreads();
-	}
+	forEachQueuedTile(f, persistent, &slotLds, [&](int32_t tile) { rasterTile<16, 16, 1>(f, tile, list, rows, roundInfo); });
 }
 
 // The adjoint of the differentiable forward with visibility fixed: each pixel re-derives the

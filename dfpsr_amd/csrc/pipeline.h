@@ -70,16 +70,18 @@ struct FrameDev {
 	const TextureDesc *textures;
 	Projected *projected;         // pointTotal
 	int32_t *fanCount;            // fanTotal: commands made by each fan triangle
-	int32_t *blockSums;           // scan scratch
+	int32_t *blockSums;           // scan scratch (per block: int32 for triangles, int64 for tiles)
 	Command *commands;
+	ShaderInputs *shaders;        // per command: the finished shader inputs of its source triangle
 	int32_t commandCapacity;
-	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flags, [3] queued tiles
+	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flags, [3] queued tiles,
+	                              // [4] raster head, [5] backward head, [16..23] per-XCD raster heads
 	int32_t *tileCount;           // tileTotal + 1
 	int32_t *tileOffset;          // tileTotal + 1 (exclusive scan of tileCount)
 	int32_t *tileFill;            // tileTotal
 	int32_t *tileList;            // pair capacity
 	int32_t pairCapacity;
-	int32_t *tileQueue;           // tiles with at least one command (counters[3] of them)
+	int32_t *tileQueue;           // tiles with at least one command, in tile order (counters[3] of them)
 	int32_t *canvasCommandBase;   // per canvas: index of its first draw command (visibility ids count from it)
 	int32_t *queueHead;           // next queue slot for persistent raster workgroups
 	int32_t tileW, tileH;
@@ -107,6 +109,7 @@ DH int32_t findSegment(const int32_t *inclusiveEnd, int32_t count, int32_t index
 __global__ void k_project(FrameDev f);
 __global__ void k_setup_count(FrameDev f);
 __global__ void k_scan_block_sums(int32_t *blockSums, int32_t blocks, int32_t *total);
+__global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_t *totalLow, int32_t *totalHigh);
 __global__ void k_setup_emit(FrameDev f);
 __global__ void k_bin_count(FrameDev f);
 __global__ void k_scan_tiles_local(FrameDev f);
@@ -122,5 +125,16 @@ __global__ void k_clear_canvases(FrameDev f, uint32_t color, float depth);
 __global__ void k_downsample(uint32_t *texels, TextureDesc t, uint32_t targetLevel);
 
 constexpr int32_t SCAN_BLOCK = 256;
+constexpr int32_t COUNTERS = 32;
+
+// Workgroups are dispatched to gfx950's eight XCDs round-robin (workgroup b runs on XCD
+// b % 8, each XCD with its own L2).  The raster gives XCD x the x-th eighth of the ordered
+// tile queue, so neighbouring tiles -- and the texels they sample -- share one L2.
+constexpr int32_t XCDS = 8;
+DH void xcdRange(int32_t queued, int32_t x, int32_t &begin, int32_t &end) {
+	const int32_t per = (queued + XCDS - 1) / XCDS;
+	begin = x * per < queued ? x * per : queued;
+	end = begin + per < queued ? begin + per : queued;
+}
 
 }  // namespace dfpsr

@@ -226,4 +226,49 @@ DH void shadeQuad(const ShaderInputs &s, const TextureDesc *textures, const floa
 	}
 }
 
+// Lane l of shadeQuad: the same operations for one pixel of the quad.  The mip level still
+// comes from the texture coordinates of lanes 0, 1 and 2 (texture_getMipLevelIndex).
+DH float pickLane(const float *v, int l) { return l == 0 ? v[0] : (l == 1 ? v[1] : (l == 2 ? v[2] : v[3])); }
+
+DH void shadeLane(const ShaderInputs &s, const TextureDesc *textures, const float *wa, const float *wb, const float *wc, int l,
+                  float &r, float &g, float &b, float &a) {
+	const float la = pickLane(wa, l), lb = pickLane(wb, l), lc = pickLane(wc, l);
+	if (s.diffuse >= 0 && s.light < 0 && s.colorless) {
+		const TextureDesc &t = textures[s.diffuse];
+		float u[3], v[3];
+		for (int k = 0; k < 3; k++) { u[k] = interp(s.u1, wa[k], wb[k], wc[k]); v[k] = interp(s.v1, wa[k], wb[k], wc[k]); }
+		uint32_t mip = mipLevelIndex(t, u[0], u[1], u[2], v[0], v[1], v[2]);
+		unpackColor(sampleBilinear(t, interp(s.u1, la, lb, lc), interp(s.v1, la, lb, lc), mip, false), r, g, b, a);
+		return;
+	}
+	if (s.light >= 0 && s.diffuse < 0 && s.colorless) {
+		const TextureDesc &t = textures[s.light];
+		unpackColor(sampleBilinear(t, interp(s.u2, la, lb, lc), interp(s.v2, la, lb, lc), 0u, true), r, g, b, a);
+		return;
+	}
+	if (s.fading) {
+		r = interp(s.red, la, lb, lc);
+		g = interp(s.green, la, lb, lc);
+		b = interp(s.blue, la, lb, lc);
+		a = interp(s.alpha, la, lb, lc);
+	} else {
+		r = s.red.x; g = s.green.x; b = s.blue.x; a = s.alpha.x;
+	}
+	if (s.diffuse >= 0) {
+		const TextureDesc &t = textures[s.diffuse];
+		float u[3], v[3];
+		for (int k = 0; k < 3; k++) { u[k] = interp(s.u1, wa[k], wb[k], wc[k]); v[k] = interp(s.v1, wa[k], wb[k], wc[k]); }
+		uint32_t mip = mipLevelIndex(t, u[0], u[1], u[2], v[0], v[1], v[2]);
+		float sr, sg, sb, sa;
+		unpackColor(sampleBilinear(t, interp(s.u1, la, lb, lc), interp(s.v1, la, lb, lc), mip, false), sr, sg, sb, sa);
+		r = r * sr; g = g * sg; b = b * sb; a = a * sa;
+	}
+	if (s.light >= 0) {
+		const TextureDesc &t = textures[s.light];
+		float sr, sg, sb, sa;
+		unpackColor(sampleBilinear(t, interp(s.u2, la, lb, lc), interp(s.v2, la, lb, lc), 0u, true), sr, sg, sb, sa);
+		r = r * sr; g = g * sg; b = b * sb; a = a * sa;
+	}
+}
+
 }  // namespace dfpsr

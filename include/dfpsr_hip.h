@@ -139,7 +139,9 @@ int dfpsr_hip_renderer_stats(const dfpsr_hip_renderer *renderer, int64_t out[4])
 /* Per-stage device timing with HIP events on the renderer's stream (no counterpart in the
  * reference, which has no profiler hooks).  When enabled, renderer_end records the device
  * milliseconds of its last pass: [0] projection, [1] triangle setup, [2] tile binning,
- * [3] tile raster, [4] the whole pass. */
+ * [3] tile raster, [4] the whole pass.  With graph replay the frame is launched as two graphs
+ * (everything before the raster, then the raster) and [0], [1] read 0 while [2] covers all
+ * stages before the raster. */
 int dfpsr_hip_renderer_set_timing(dfpsr_hip_renderer *renderer, int32_t enable);
 int dfpsr_hip_renderer_timing(const dfpsr_hip_renderer *renderer, float out[5]);
 

@@ -86,3 +86,19 @@ def test_autotune_picks_a_swept_setting_and_keeps_images(setup):
     for i in range(len(jobs)):
         c, d = g.image(i)
         assert np.array_equal(c, plain[i][0]) and np.array_equal(d.view(np.uint32), plain[i][1].view(np.uint32)), i
+
+
+def test_stage_timing_inside_graph_replay(setup):
+    """With timing on, a graph-replayed frame is launched as two graphs with events between them:
+    stages before the raster as one figure, then the raster, then the whole."""
+    api, jobs, g, plain = setup
+    sc = S.terrain(width=320, height=240, grid=48, texture_size=128, resolutions=8)
+    ref = api.render_scene(sc)
+    r = api.renderer_create()
+    api.renderer_setTuning(r, api.tuning(use_graph=True))
+    r.set_timing(True)
+    for _ in range(3):
+        got = api.render_scene(sc, r)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1].view(np.uint32), ref[1].view(np.uint32))
+        t = r.timing()
+        assert t[0] == 0 and t[1] == 0 and t[2] > 0 and t[3] > 0 and t[4] >= t[3]
