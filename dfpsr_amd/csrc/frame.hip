@@ -135,7 +135,7 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, f.counters, COUNTERS, 0);
 	hipLaunchKernelGGL(k_fill, dim3(blocksOf(2 * (int64_t)tileTotal + 1, 256)), dim3(256), 0, stream, f.tileCount, 2 * tileTotal + 1, 0);
 	hipLaunchKernelGGL(k_fill, dim3(blocksOf(f.canvasCount, 256)), dim3(256), 0, stream, f.canvasCommandBase, f.canvasCount, 0x7FFFFFFF);
-	if (o.clear) hipLaunchKernelGGL(k_clear_canvases, dim3(tileTotal), dim3(256), 0, stream, f, o.clearColor, o.clearDepth);
+	if (o.clear && !f.clearInRaster) hipLaunchKernelGGL(k_clear_canvases, dim3(tileTotal), dim3(256), 0, stream, f, o.clearColor, o.clearDepth);
 	if (events) HIP_TRY(hipEventRecord(events_[0], stream));
 	hipLaunchKernelGGL(k_project, dim3(blocksOf(f.pointTotal, 256)), dim3(256), 0, stream, f);
 	if (events) HIP_TRY(hipEventRecord(events_[1], stream));
@@ -238,6 +238,7 @@ int Frame::run(const RenderOptions &o) {
 	for (int pass = 1; pass <= 3; pass++) {
 		if (dCommands_.ensure((size_t)commandCapacity_ * sizeof(Command)) != hipSuccess ||
 		    dShaders_.ensure((size_t)commandCapacity_ * sizeof(ShaderInputs)) != hipSuccess ||
+		    dTriangleRows_.ensure((size_t)commandCapacity_ * sizeof(TriangleRows)) != hipSuccess ||
 		    dTileList_.ensure((size_t)pairCapacity_ * 4) != hipSuccess)
 			return fail(-4, "device allocation failed");
 		FrameDev f;
@@ -258,6 +259,7 @@ int Frame::run(const RenderOptions &o) {
 		f.blockSums = dBlockSums_.as<int32_t>();
 		f.commands = dCommands_.as<Command>();
 		f.shaders = dShaders_.as<ShaderInputs>();
+		f.triangleRows = dTriangleRows_.as<TriangleRows>();
 		f.commandCapacity = commandCapacity_;
 		f.counters = dCounters_.as<int32_t>();
 		f.queueHead = dCounters_.as<int32_t>() + 4;
@@ -271,6 +273,9 @@ int Frame::run(const RenderOptions &o) {
 		f.tileW = tileConfigW(tileConfig_);
 		f.tileH = tileConfigH(tileConfig_);
 		f.alphaAny = alphaAny_ ? 1 : 0;
+		f.clearInRaster = (o.clear && deferredRaster(o)) ? 1 : 0;
+		f.clearColor = o.clearColor;
+		f.clearDepth = o.clearDepth;
 		f.mode = o.mode;
 		f.ftexTexels = ftex_;
 		f.ftexLog2w = ftexLog2w_;

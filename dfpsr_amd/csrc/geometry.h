@@ -421,6 +421,26 @@ DH EdgeSetup edgeSetup(const l2 &s, const l2 &e, const Rect &clip) {
 
 struct RowInterval { int32_t left, right; };
 
+// Truncating int64 division.  Both operands are exact in double when |n| < 2^52 (edge values
+// of triangles inside the clip frustum stay far below that), so the double quotient is within
+// one of the truncated quotient and one correction makes it exact; larger operands take the
+// integer division.
+DH int64_t divTrunc64(int64_t n, int64_t d) {
+	const int64_t lim = (int64_t)1 << 52;
+	if (n >= lim || n <= -lim || d >= lim || d <= -lim) return n / d;
+	int64_t q = (int64_t)((double)n / (double)d);
+	const int64_t dir = ((n < 0) == (d < 0)) ? 1 : -1;  // sign of the true quotient
+	const int64_t ad = d < 0 ? -d : d;
+	for (int k = 0; k < 2; k++) {
+		const int64_t r = n - q * d;
+		const int64_t ar = r < 0 ? -r : r;
+		if (r != 0 && ((r < 0) != (n < 0))) q -= dir;       // one step too far from zero
+		else if (ar >= ad) q += dir;                         // one step short
+		else break;
+	}
+	return q;
+}
+
 // Row r (0-based from clip.t) of rasterizeTriangle (ITriangle2D.cpp:150-169).
 DH RowInterval rowInterval(const EdgeSetup *edges, bool degenerate, const Rect &clip, int32_t r) {
 	const int32_t L = clip.l, R = clip.right();
@@ -432,11 +452,11 @@ DH RowInterval rowInterval(const EdgeSetup *edges, bool degenerate, const Rect &
 		const EdgeSetup &e = edges[k];
 		if (e.kind == 1) {
 			int64_t limit = e.limitTop - e.offsetY * (int64_t)r;
-			int32_t side = tmin(tmax(L, (int32_t)((limit + 1) / e.offsetX + 1)), R);
+			int32_t side = tmin(tmax(L, (int32_t)(divTrunc64(limit + 1, e.offsetX) + 1)), R);
 			row.left = tmax(row.left, side);
 		} else if (e.kind == 2) {
 			int64_t limit = e.limitTop - e.offsetY * (int64_t)r;
-			int32_t side = tmin(tmax(L, (int32_t)(limit / e.offsetX + 1)), R);
+			int32_t side = tmin(tmax(L, (int32_t)(divTrunc64(limit, e.offsetX) + 1)), R);
 			row.right = tmin(row.right, side);
 		} else if (e.kind == 3) {
 			int64_t value = e.valueTop + e.offsetY * (int64_t)r;
@@ -447,5 +467,40 @@ DH RowInterval rowInterval(const EdgeSetup *edges, bool degenerate, const Rect &
 }
 
 DH bool sameCorner(const l2 &a, const l2 &b) { return a.x == b.x && a.y == b.y; }
+
+// What every row of a triangle needs (ITriangle2D.cpp:70-176): the three edges of
+// rasterizeTriangle, the clipped columns and the aligned row bound.  Set up once per draw
+// command; rows are then independent of each other.
+struct TriangleRows {
+	EdgeSetup edges[3];
+	int32_t clipL, clipR, alT, alB;
+	int32_t degenerate, pad;
+};
+
+DH TriangleRows triangleRows(const l2 &A, const l2 &B, const l2 &C, const Rect &canvas) {
+	const Rect wb = triangleBound(A, B, C);
+	const Rect clipR = cutRect(wb, canvas);
+	const Rect al = alignedRasterBound(wb, canvas);
+	const Rect rowClip = rectSides(clipR.l, al.t, clipR.right(), al.bottom());
+	TriangleRows tr;
+	tr.edges[0] = edgeSetup(A, B, rowClip);
+	tr.edges[1] = edgeSetup(B, C, rowClip);
+	tr.edges[2] = edgeSetup(C, A, rowClip);
+	tr.clipL = clipR.l;
+	tr.clipR = clipR.right();
+	tr.alT = al.t;
+	tr.alB = al.bottom();
+	tr.degenerate = (sameCorner(A, B) || sameCorner(B, C) || sameCorner(C, A)) ? 1 : 0;
+	tr.pad = 0;
+	return tr;
+}
+
+// Row y of the triangle; empty outside the aligned bound.
+DH RowInterval triangleRow(const TriangleRows &tr, int32_t y) {
+	RowInterval ri{0, 0};
+	if (y >= tr.alT && y < tr.alB)
+		ri = rowInterval(tr.edges, tr.degenerate != 0, rectSides(tr.clipL, tr.alT, tr.clipR, tr.alB), y - tr.alT);
+	return ri;
+}
 
 }  // namespace dfpsr

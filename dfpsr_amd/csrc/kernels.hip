@@ -245,11 +245,13 @@ DH TileRange commandTiles(const Command &c, const CanvasDev &cv, int32_t tw, int
 	return r;
 }
 
-// Counts each command's tiles and finishes its shader inputs for the raster.
+// Counts each command's tiles and sets up its rows and shader inputs for the raster.
 __global__ void k_bin_count(FrameDev f) {
 	int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c >= f.counters[0] || f.counters[2]) return;
 	const Command &cmd = f.commands[c];
+	f.triangleRows[c] = triangleRows(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]},
+	                                 canvasRect(f.canvases[cmd.canvas]));
 	if (f.mode == 0) {
 		const TaskDev &t = f.tasks[cmd.task];
 		const int32_t fp = t.fanPolygon[cmd.fan];
@@ -368,9 +370,16 @@ struct QuadEval {
 	bool clipSides;
 };
 
+// n steps of both rows' weights by d, one rounded addition at a time as the reference steps
+// them (n <= 0: none).
+__device__ __forceinline__ void stepPair(f3 &wU, f3 &wL, const f3 &d, int32_t n) {
+#pragma clang loop unroll(disable)
+	for (int32_t i = 0; i < n; i++) { wU = add3(wU, d); wL = add3(wL, d); }
+}
+
 // upper/lower: row intervals of rows py and py + 1 (py even).  Returns false when the command
 // does not touch the quad at px (even).
-__device__ __forceinline__ bool evalQuad(const Command &cmd, RowInterval upper, RowInterval lower, int32_t px, int32_t py,
+__device__ __forceinline__ bool evalQuad(const Projection &cmd, RowInterval upper, RowInterval lower, int32_t px, int32_t py,
                                          int32_t canvasHeight, QuadEval &e) {
 	int32_t outerStart = tmin(upper.left, lower.left);
 	int32_t outerEnd = tmax(upper.right, lower.right);
@@ -388,29 +397,30 @@ __device__ __forceinline__ bool evalQuad(const Command &cmd, RowInterval upper, 
 
 	const f3 dx = cmd.dx;
 	const f3 dx2 = scale3(dx, 2.0f);
-	f3 wU = weightAtPixel(Projection{cmd.start, cmd.dx, cmd.dy}, outerBlockStart, py);
+	f3 wU = weightAtPixel(cmd, outerBlockStart, py);
 	f3 wL = add3(wU, cmd.dy);
 	e.clipSides = true;
 	bool splitSpan = innerBlockEnd > innerBlockStart;
 	if (splitSpan && px >= innerBlockStart && px < innerBlockEnd) {
 		// Unclipped inner span: lanes start at the span and step per lane.
-		for (int32_t x = outerBlockStart; x < innerBlockStart; x += 2) { wU = add3(wU, dx2); wL = add3(wL, dx2); }
+		stepPair(wU, wL, dx2, (innerBlockStart - outerBlockStart) >> 1);
 		e.clipSides = false;
 		e.l0[0] = wU.x; e.l0[1] = wU.x + dx.x; e.l0[2] = wL.x; e.l0[3] = wL.x + dx.x;
 		e.l1[0] = wU.y; e.l1[1] = wU.y + dx.y; e.l1[2] = wL.y; e.l1[3] = wL.y + dx.y;
 		e.l2[0] = wU.z; e.l2[1] = wU.z + dx.z; e.l2[2] = wL.z; e.l2[3] = wL.z + dx.z;
+#pragma clang loop unroll(disable)
 		for (int32_t x = innerBlockStart; x < px; x += 2) {
 			for (int l = 0; l < 4; l++) { e.l0[l] = e.l0[l] + dx2.x; e.l1[l] = e.l1[l] + dx2.y; e.l2[l] = e.l2[l] + dx2.z; }
 		}
 	} else {
 		if (!splitSpan || px < innerBlockStart) {
-			for (int32_t x = outerBlockStart; x < px; x += 2) { wU = add3(wU, dx2); wL = add3(wL, dx2); }
+			stepPair(wU, wL, dx2, (px - outerBlockStart) >> 1);
 		} else {
-			for (int32_t x = outerBlockStart; x < innerBlockStart; x += 2) { wU = add3(wU, dx2); wL = add3(wL, dx2); }
+			stepPair(wU, wL, dx2, (innerBlockStart - outerBlockStart) >> 1);
 			float quadCount = (float)((innerBlockEnd - innerBlockStart) / 2);
 			wU = add3(wU, scale3(dx2, quadCount));
 			wL = add3(wL, scale3(dx2, quadCount));
-			for (int32_t x = innerBlockEnd; x < px; x += 2) { wU = add3(wU, dx2); wL = add3(wL, dx2); }
+			stepPair(wU, wL, dx2, (px - innerBlockEnd) >> 1);
 		}
 		e.l0[0] = wU.x; e.l0[1] = wU.x + dx.x; e.l0[2] = wL.x; e.l0[3] = wL.x + dx.x;
 		e.l1[0] = wU.y; e.l1[1] = wU.y + dx.y; e.l1[2] = wL.y; e.l1[3] = wL.y + dx.y;
@@ -430,6 +440,11 @@ __device__ __forceinline__ bool evalQuad(const Command &cmd, RowInterval upper, 
 		e.clip[0] = e.clip[1] = e.clip[2] = e.clip[3] = true;
 	}
 	return true;
+}
+
+__device__ __forceinline__ bool evalQuad(const Command &cmd, RowInterval upper, RowInterval lower, int32_t px, int32_t py,
+                                         int32_t canvasHeight, QuadEval &e) {
+	return evalQuad(Projection{cmd.start, cmd.dx, cmd.dy}, upper, lower, px, py, canvasHeight, e);
 }
 
 // Vertex weights of a lane (fillRowSuper, fillerTemplates.h:222-238): perspective divides the
@@ -469,12 +484,12 @@ __device__ __forceinline__ bool quadVisibility(const QuadEval &e, const QuadPixe
 // Shader of one quad of command c for all four lanes: RgbaMultiply's getPixels_2x2 with
 // the vertex weights of the lanes (RgbaMultiply.h:74-175, shaderMethods.h:39-94).  The
 // shader inputs were finished once per source triangle at setup.
-__device__ __forceinline__ void shadeCommand(const FrameDev &f, int32_t c, const QuadEval &e, bool affine,
+__device__ __forceinline__ void shadeCommand(const FrameDev &f, int32_t c, const QuadEval &e, bool affine, uint32_t mask,
                                              float *r, float *g, float *b, float *a) {
 	float wa[4], wb[4], wc[4];
 	for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
 	const ShaderInputs si = f.shaders[c];
-	shadeQuad(si, f.textures, wa, wb, wc, r, g, b, a);
+	shadeQuad(si, f.textures, wa, wb, wc, mask, r, g, b, a);
 }
 
 // The reference's shading of one quad (fillQuadSuper, fillerTemplates.h:154-187, with the
@@ -486,7 +501,8 @@ __device__ void drawCommand(const FrameDev &f, const Command &cmd, int32_t c, co
 	if (!quadVisibility<0>(e, q, depthRead, affine, vis)) return;
 	if (colorWrite) {
 		float r[4], g[4], b[4], a[4];
-		shadeCommand(f, c, e, affine, r, g, b, a);
+		const uint32_t mask = (vis[0] ? 1u : 0u) | (vis[1] ? 2u : 0u) | (vis[2] ? 4u : 0u) | (vis[3] ? 8u : 0u);
+		shadeCommand(f, c, e, affine, mask, r, g, b, a);
 		if (alphaFilter) {
 			float tr[4], tg[4], tb[4], ta[4];
 			for (int l = 0; l < 4; l++) {
@@ -697,23 +713,13 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 			const bool depthWrite = cv.depth != nullptr && (solid || !colorWrite);
 			const bool alphaFilter = colorWrite && !solid;
 			// Rows of this command inside the tile: rasterizeTriangle over the aligned bound.
-			Rect wb = triangleBound(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]});
-			Rect clipR = cutRect(wb, canvasRect(cv));
-			Rect al = alignedRasterBound(wb, canvasRect(cv));
-			Rect rowClip = rectSides(clipR.l, al.t, clipR.right(), al.bottom());
+			const TriangleRows &tr = f.triangleRows[c];
+			const int32_t alT = tr.alT, alB = tr.alB;
 			__syncthreads();
-			if (lane < TH) {
-				int32_t y = tileY0 + lane;
-				if (y >= al.t && y < al.bottom()) {
-					l2 A{cmd.fx[0], cmd.fy[0]}, B{cmd.fx[1], cmd.fy[1]}, C{cmd.fx[2], cmd.fy[2]};
-					bool degenerate = sameCorner(A, B) || sameCorner(B, C) || sameCorner(C, A);
-					EdgeSetup e[3] = {edgeSetup(A, B, rowClip), edgeSetup(B, C, rowClip), edgeSetup(C, A, rowClip)};
-					rows[lane] = rowInterval(e, degenerate, rowClip, y - al.t);
-				}
-			}
+			if (lane < TH) rows[lane] = triangleRow(tr, tileY0 + lane);
 			__syncthreads();
 			QuadEval e;
-			if (py >= al.t && py < al.bottom() && evalQuad(cmd, rows[py - tileY0], rows[py + 1 - tileY0], px, py, cv.height, e)) {
+			if (py >= alT && py < alB && evalQuad(cmd, rows[py - tileY0], rows[py + 1 - tileY0], px, py, cv.height, e)) {
 				if constexpr (MODE == 0) drawCommand(f, cmd, c, e, q, colorWrite, depthRead, depthWrite, alphaFilter);
 				else drawCommandDiff(f, cmd, c - f.canvasCommandBase[ci], task, e, q, depthRead, depthWrite);
 			}
@@ -748,13 +754,13 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 // then from the other XCDs' heads; every head ends past its range, so all of them exit.
 template <typename Body>
 __device__ void forEachQueuedTile(const FrameDev &f, int32_t persistent, int32_t *slotLds, Body body) {
-	const int32_t queued = f.counters[3];
+	const int32_t queued = f.clearInRaster ? f.tileTotal : f.counters[3];
 	const int32_t x = (int32_t)(blockIdx.x % XCDS);
 	if (!persistent) {
 		int32_t begin, end;
 		xcdRange(queued, x, begin, end);
 		const int32_t slot = begin + (int32_t)(blockIdx.x / XCDS);
-		if (slot < end) body(f.tileQueue[slot]);
+		if (slot < end) body(f.clearInRaster ? slot : f.tileQueue[slot]);
 		return;
 	}
 	for (int32_t r = 0; r < XCDS; r++) {
@@ -767,7 +773,7 @@ __device__ void forEachQueuedTile(const FrameDev &f, int32_t persistent, int32_t
 			const int32_t slot = *slotLds;
 			__syncthreads();
 			if (slot >= end) break;
-			body(f.tileQueue[slot]);
+			body(f.clearInRaster ? slot : f.tileQueue[slot]);
 			__syncthreads();
 		}
 	}
@@ -800,15 +806,16 @@ template __global__ void k_raster<32, 32>(FrameDev, int32_t);
 // every command offers the key (depth order << 32 | ~command) to the pixel with an LDS
 // atomic max, and only the winning command is shaded for each pixel, with the same quad
 // evaluation, weights and mip selection as the in-order walk.  Without a depth canvas the key
-// is the command alone and the last covering command wins.  Commands run in chunks of
-// DEFER_CHUNK per tile; each chunk starts from the depths the previous one left.
+// is the command alone and the last covering command wins.  Commands run in chunks per tile;
+// each chunk starts from the depths the previous one left.  Work is spread densely: one
+// thread per (command, row) for the exact rows, per (command, covered quad) for the keys and
+// per pixel for the shading of its winner.
 //
 // Frames with the Alpha filter (blending reads the colour held at that moment), orthogonal
 // cameras (a lower depth wins) or an odd-height canvas (the missing last row aliases lanes
 // onto the row above within one command) take the in-order kernel above.
 // ---------------------------------------------------------------------------------------
 
-constexpr int32_t DEFER_CHUNK = 32;
 
 // Keys order depths as floats compare: larger depth, larger key; -0 equals +0.
 __device__ __forceinline__ uint32_t depthOrder(float d) {
@@ -816,12 +823,50 @@ __device__ __forceinline__ uint32_t depthOrder(float d) {
 	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// Per chunk command: what every row of it needs (rasterizeTriangle's edges, clip and aligned
+// bound) and the weight planes, set up once.
+struct ChunkCommand {
+	TriangleRows rows;
+	Projection plane;
+	int32_t flags, command;
+};
+
+// LDS of one visibility-raster workgroup.
+template <int TW, int TH> struct DeferLds {
+	static constexpr int32_t CHUNK = TH == 16 ? 64 : 32;   // commands per chunk
+	static constexpr int32_t PAIRS = CHUNK * (TH / 2);      // (command, row pair) slots of a chunk
+	int32_t list[TILE_SORT_CAP];                 // the round's commands, sorted
+	ChunkCommand cmds[CHUNK];
+	RowInterval rows[CHUNK * TH];                // chunk command j, tile row r
+	int32_t pairStart[PAIRS];                    // first quad column (pixel x) of each pair in the tile
+	int32_t pairOffset[PAIRS + 1];               // exclusive prefix of quads per pair
+	unsigned long long keys[TW * TH];
+	uint32_t color[TW * TH];                     // the tile's canvas pixels as they stand
+	float depth[TW * TH];
+	uint8_t dirty[TW * TH];
+	int32_t roundInfo[4];
+	int32_t slot;
+};
+
+// Exclusive prefix over per-thread sums of the workgroup; returns the total.
+__device__ __forceinline__ int32_t workgroupPrefix(int32_t value, int32_t &before) { return blockExclusiveScan(value, before); }
+
+// Index of the segment holding item it in an exclusive prefix of count + 1 entries.
+__device__ __forceinline__ int32_t segmentOf(const int32_t *offset, int32_t count, int32_t it) {
+	int32_t lo = 0, hi = count - 1;
+	while (lo < hi) {
+		const int32_t mid = (lo + hi + 1) >> 1;
+		if (offset[mid] <= it) lo = mid; else hi = mid - 1;
+	}
+	return lo;
+}
+
 template <int TW, int TH>
-__device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, int32_t *list, RowInterval *rows,
-                                   unsigned long long *keys, int32_t *roundInfo) {
-	constexpr int32_t NQX = TW / 2, NQ = (TW / 2) * (TH / 2), NP = TW * TH;
-	constexpr int32_t WAVES = DEFER_THREADS / 64;
-	constexpr int32_t PPT = NP / DEFER_THREADS;  // pixels per thread
+__device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW, TH> &s) {
+	using L = DeferLds<TW, TH>;
+	constexpr int32_t NP = TW * TH, NQX = TW / 2, RP = TH / 2;
+	constexpr int32_t CHUNK = L::CHUNK, PAIRS = L::PAIRS;
+	constexpr int32_t PAIRS_PER_THREAD = PAIRS / DEFER_THREADS;
 	const int32_t n = f.tileCount[tile];
 	const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
 	const CanvasDev cv = f.canvases[ci];
@@ -829,85 +874,111 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, int32_t *lis
 	const int32_t tileX0 = (local % cv.tilesX) * TW;
 	const int32_t tileY0 = (local / cv.tilesX) * TH;
 	const int32_t t = threadIdx.x;
-	const int32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
-	const int32_t lane = t & 63;
 	const int32_t *src = f.tileList + f.tileOffset[tile];
 	const bool colorWrite = cv.color != nullptr;
 	const bool depthRead = cv.depth != nullptr;
 
-	// Thread t holds pixels t, t + 256, ... of the tile (row-major) in registers.
-	float depth[PPT];
-	uint32_t color[PPT];
-	bool dirty[PPT];
-	for (int32_t i = 0; i < PPT; i++) {
-		const int32_t p = t + i * DEFER_THREADS;
-		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
-		depth[i] = (depthRead && x < cv.width && y < cv.height) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
-		color[i] = 0u;
-		dirty[i] = false;
+	for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+		if (f.clearInRaster) {
+			s.depth[p] = f.clearDepth;
+			s.color[p] = f.clearColor;
+			s.dirty[p] = 1;
+		} else {
+			const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
+			s.depth[p] = (depthRead && x < cv.width && y < cv.height) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
+			s.dirty[p] = 0;
+		}
 	}
 
 	int32_t last = -1;
 	int32_t done = 0;
 	while (done < n) {
-		const int32_t take = nextRound<DEFER_THREADS>(src, n, done, last, list, roundInfo);
-		for (int32_t c0 = 0; c0 < take; c0 += DEFER_CHUNK) {
-			const int32_t cn = take - c0 < DEFER_CHUNK ? take - c0 : DEFER_CHUNK;
+		const int32_t take = nextRound<DEFER_THREADS>(src, n, done, last, s.list, s.roundInfo);
+		for (int32_t c0 = 0; c0 < take; c0 += CHUNK) {
+			const int32_t cn = take - c0 < CHUNK ? take - c0 : CHUNK;
 			// Keys start at the held depth with the lowest command part above every command's,
 			// so only a strictly greater depth replaces it; a NaN depth is never replaced.
-			for (int32_t i = 0; i < PPT; i++) {
-				const float d = depth[i];
-				keys[t + i * DEFER_THREADS] =
-				    !depthRead ? 0ull : (d != d ? ~0ull : (((unsigned long long)depthOrder(d) << 32) | 0xFFFFFFFFull));
+			for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+				const float d = s.depth[p];
+				s.keys[p] = !depthRead ? 0ull : (d != d ? ~0ull : (((unsigned long long)depthOrder(d) << 32) | 0xFFFFFFFFull));
+			}
+			// The chunk commands' row setups and weight planes, copied word by word.
+			constexpr int32_t ROW_WORDS = (int32_t)(sizeof(TriangleRows) / 4);
+			for (int32_t idx = t; idx < cn * ROW_WORDS; idx += DEFER_THREADS) {
+				const int32_t j = idx / ROW_WORDS, w = idx % ROW_WORDS;
+				((uint32_t *)&s.cmds[j].rows)[w] = ((const uint32_t *)&f.triangleRows[s.list[c0 + j]])[w];
+			}
+			for (int32_t j = t; j < cn; j += DEFER_THREADS) {
+				const int32_t c = s.list[c0 + j];
+				const Command &cmd = f.commands[c];
+				Projection &pl = s.cmds[j].plane;
+				pl.start.x = cmd.start.x; pl.start.y = cmd.start.y; pl.start.z = cmd.start.z;
+				pl.dx.x = cmd.dx.x; pl.dx.y = cmd.dx.y; pl.dx.z = cmd.dx.z;
+				pl.dy.x = cmd.dy.x; pl.dy.y = cmd.dy.y; pl.dy.z = cmd.dy.z;
+				s.cmds[j].flags = cmd.flags;
+				s.cmds[j].command = c;
 			}
 			__syncthreads();
-			// Pass 1: wave w offers the coverage of chunk commands w, w + WAVES, ... (a lane per quad).
-			for (int32_t j = wave; j < cn; j += WAVES) {
-				const int32_t c = __builtin_amdgcn_readfirstlane(list[c0 + j]);
-				const Command cmd = f.commands[c];
-				Rect wb = triangleBound(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]});
-				Rect clipR = cutRect(wb, canvasRect(cv));
-				Rect al = alignedRasterBound(wb, canvasRect(cv));
-				Rect rowClip = rectSides(clipR.l, al.t, clipR.right(), al.bottom());
-				RowInterval *cr = rows + j * TH;
-				if (lane < TH) {
-					const int32_t y = tileY0 + lane;
-					RowInterval ri{0, 0};
-					if (y >= al.t && y < al.bottom()) {
-						l2 A{cmd.fx[0], cmd.fy[0]}, B{cmd.fx[1], cmd.fy[1]}, C{cmd.fx[2], cmd.fy[2]};
-						bool degenerate = sameCorner(A, B) || sameCorner(B, C) || sameCorner(C, A);
-						EdgeSetup e[3] = {edgeSetup(A, B, rowClip), edgeSetup(B, C, rowClip), edgeSetup(C, A, rowClip)};
-						ri = rowInterval(e, degenerate, rowClip, y - al.t);
+			// Rows of every chunk command in the tile, one (command, row) per thread: the exact
+			// intervals of rasterizeTriangle inside the aligned bound, empty outside it.
+			for (int32_t idx = t; idx < cn * TH; idx += DEFER_THREADS) {
+				s.rows[idx] = triangleRow(s.cmds[idx / TH].rows, tileY0 + idx % TH);
+			}
+			__syncthreads();
+			// Quads of each (command, row pair) inside the tile: the outer span of the row pair
+			// (fillShapeSuper), cut to the tile's columns.
+			int32_t pairCount[PAIRS_PER_THREAD];
+			int32_t threadSum = 0;
+			for (int32_t k = 0; k < PAIRS_PER_THREAD; k++) {
+				const int32_t pr = t * PAIRS_PER_THREAD + k;
+				const int32_t j = pr / RP, rp = pr % RP;
+				int32_t count = 0, x0 = 0;
+				if (j < cn) {
+					const RowInterval upper = s.rows[j * TH + 2 * rp], lower = s.rows[j * TH + 2 * rp + 1];
+					if (upper.right > upper.left || lower.right > lower.left) {
+						const int32_t outerBlockStart = (int32_t)((uint32_t)tmin(upper.left, lower.left) & ~1u);
+						const int32_t outerBlockEnd = (int32_t)(((uint32_t)tmax(upper.right, lower.right) + 1u) & ~1u);
+						x0 = tmax(outerBlockStart, tileX0);
+						const int32_t x1 = tmin(outerBlockEnd, tileX0 + TW);
+						count = x1 > x0 ? (x1 - x0) >> 1 : 0;
 					}
-					cr[lane] = ri;
 				}
-				// The rows were written by lanes of this wave only.
-				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-				__builtin_amdgcn_wave_barrier();
-				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-				for (int32_t qi = lane; qi < NQ; qi += 64) {
-					const int32_t px = tileX0 + 2 * (qi % NQX), py = tileY0 + 2 * (qi / NQX);
-					QuadEval e;
-					if (py < al.t || py >= al.bottom() || !evalQuad(cmd, cr[py - tileY0], cr[py + 1 - tileY0], px, py, cv.height, e))
-						continue;
-					for (int l = 0; l < 4; l++) {
-						if (e.clipSides && !e.clip[l]) continue;
-						const int32_t s = e.slot[l];
-						const int32_t p = (py - tileY0 + (s >> 1)) * TW + (px - tileX0 + (s & 1));
-						if (depthRead) {
-							const float d = e.l0[l];
-							if (d == d) atomicMax(&keys[p], ((unsigned long long)depthOrder(d) << 32) | (unsigned long long)(0xFFFFFFFEu - (uint32_t)j));
-						} else {
-							atomicMax(&keys[p], (unsigned long long)(j + 1));
-						}
+				s.pairStart[pr] = x0;
+				pairCount[k] = count;
+				threadSum += count;
+			}
+			int32_t before;
+			const int32_t items = workgroupPrefix(threadSum, before);
+			for (int32_t k = 0; k < PAIRS_PER_THREAD; k++) {
+				s.pairOffset[t * PAIRS_PER_THREAD + k] = before;
+				before += pairCount[k];
+			}
+			if (t == 0) s.pairOffset[PAIRS] = items;
+			__syncthreads();
+			// Pass 1: one (command, quad) item per thread offers the covered lanes' keys.
+			for (int32_t it = t; it < items; it += DEFER_THREADS) {
+				const int32_t pr = segmentOf(s.pairOffset, PAIRS, it);
+				const int32_t j = pr / RP, rp = pr % RP;
+				const int32_t px = s.pairStart[pr] + 2 * (it - s.pairOffset[pr]);
+				const int32_t py = tileY0 + 2 * rp;
+				QuadEval e;
+				if (!evalQuad(s.cmds[j].plane, s.rows[j * TH + 2 * rp], s.rows[j * TH + 2 * rp + 1], px, py, cv.height, e)) continue;
+				for (int l = 0; l < 4; l++) {
+					if (e.clipSides && !e.clip[l]) continue;
+					const int32_t sl = e.slot[l];
+					const int32_t p = (py - tileY0 + (sl >> 1)) * TW + (px - tileX0 + (sl & 1));
+					if (depthRead) {
+						const float d = e.l0[l];
+						if (d == d) atomicMax(&s.keys[p], ((unsigned long long)depthOrder(d) << 32) | (unsigned long long)(0xFFFFFFFEu - (uint32_t)j));
+					} else {
+						atomicMax(&s.keys[p], (unsigned long long)(j + 1));
 					}
 				}
 			}
 			__syncthreads();
 			// Pass 2: each pixel evaluates its winning command's quad and shades its own lane.
-			for (int32_t i = 0; i < PPT; i++) {
-				const int32_t p = t + i * DEFER_THREADS;
-				const unsigned long long key = keys[p];
+			for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+				const unsigned long long key = s.keys[p];
 				int32_t jw;
 				if (depthRead) {
 					const uint32_t low = (uint32_t)key;
@@ -917,27 +988,25 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, int32_t *lis
 				}
 				if (jw < 0) continue;
 				const int32_t xl = p % TW, yl = p / TW;
-				const int32_t px = tileX0 + (xl & ~1), py = tileY0 + (yl & ~1);
-				const int32_t s = (yl & 1) * 2 + (xl & 1);
-				const int32_t cw = list[c0 + jw];
-				const Command cmd = f.commands[cw];
-				const RowInterval *cr = rows + jw * TH;
+				const int32_t qx = xl & ~1, qy = yl & ~1;
+				const int32_t sl = (yl & 1) * 2 + (xl & 1);
+				const ChunkCommand &cc = s.cmds[jw];
 				QuadEval e;
-				if (!evalQuad(cmd, cr[py - tileY0], cr[py + 1 - tileY0], px, py, cv.height, e)) continue;
+				if (!evalQuad(cc.plane, s.rows[jw * TH + qy], s.rows[jw * TH + qy + 1], tileX0 + qx, tileY0 + qy, cv.height, e)) continue;
 				int l = -1;
 				for (int k = 0; k < 4; k++)
-					if ((!e.clipSides || e.clip[k]) && e.slot[k] == s) l = k;
+					if ((!e.clipSides || e.clip[k]) && e.slot[k] == sl) l = k;
 				if (l < 0) continue;
 				if (colorWrite) {
-					const bool affine = (cmd.flags & 1) != 0;
+					const bool affine = (cc.flags & 1) != 0;
 					float wa[4], wb[4], wc[4];
 					for (int k = 0; k < 4; k++) laneWeights(e, k, affine, wa[k], wb[k], wc[k]);
 					float r, g, b, a;
-					shadeLane(f.shaders[cw], f.textures, wa, wb, wc, l, r, g, b, a);
-					color[i] = packSaturated(r, g, b, a);
+					shadeLane(f.shaders[cc.command], f.textures, wa, wb, wc, l, r, g, b, a);
+					s.color[p] = packSaturated(r, g, b, a);
 				}
-				depth[i] = pickLane(e.l0, l);
-				dirty[i] = true;
+				s.depth[p] = pickLane(e.l0, l);
+				s.dirty[p] = 1;
 			}
 			__syncthreads();
 		}
@@ -945,25 +1014,20 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, int32_t *lis
 		if (take == 0) break;
 	}
 
-	for (int32_t i = 0; i < PPT; i++) {
-		if (!dirty[i]) continue;
-		const int32_t p = t + i * DEFER_THREADS;
+	for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+		if (!s.dirty[p]) continue;
 		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
-		if (colorWrite) cv.color[(size_t)y * cv.colorStride + x] = color[i];
-		if (depthRead) cv.depth[(size_t)y * cv.depthStride + x] = depth[i];
+		if (x >= cv.width || y >= cv.height) continue;
+		if (colorWrite) cv.color[(size_t)y * cv.colorStride + x] = s.color[p];
+		if (depthRead) cv.depth[(size_t)y * cv.depthStride + x] = s.depth[p];
 	}
 }
 
-// Five waves per SIMD is the most the kernel holds without spilling registers.
 template <int TW, int TH>
-__global__ void __launch_bounds__(DEFER_THREADS) __attribute__((amdgpu_waves_per_eu(5, 5))) k_raster_deferred(FrameDev f, int32_t persistent) {
-	__shared__ int32_t list[TILE_SORT_CAP];
-	__shared__ RowInterval rows[DEFER_CHUNK * TH];
-	__shared__ unsigned long long keys[TW * TH];
-	__shared__ int32_t roundInfo[3];
-	__shared__ int32_t slotLds;
+__global__ void __launch_bounds__(DEFER_THREADS) k_raster_deferred(FrameDev f, int32_t persistent) {
+	__shared__ DeferLds<TW, TH> s;
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
-	forEachQueuedTile(f, persistent, &slotLds, [&](int32_t tile) { rasterTileDeferred<TW, TH>(f, tile, list, rows, keys, roundInfo); });
+	forEachQueuedTile(f, persistent, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH>(f, tile, s); });
 }
 
 template __global__ void k_raster_deferred<16, 16>(FrameDev, int32_t);
@@ -1005,17 +1069,10 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 		const int32_t c = (id >> 2) + f.canvasCommandBase[ci], lane = id & 3;
 		const Command cmd = f.commands[c];
 		const TaskDev &task = f.tasks[cmd.task];
-		Rect wb = triangleBound(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]});
-		Rect full = rectSides(0, 0, cv.width, cv.height);
-		Rect clipR = cutRect(wb, full);
-		Rect al = alignedRasterBound(wb, full);
-		Rect rowClip = rectSides(clipR.l, al.t, clipR.right(), al.bottom());
-		l2 A{cmd.fx[0], cmd.fy[0]}, B{cmd.fx[1], cmd.fy[1]}, C{cmd.fx[2], cmd.fy[2]};
-		bool degenerate = sameCorner(A, B) || sameCorner(B, C) || sameCorner(C, A);
-		EdgeSetup es[3] = {edgeSetup(A, B, rowClip), edgeSetup(B, C, rowClip), edgeSetup(C, A, rowClip)};
+		const TriangleRows &tr = f.triangleRows[c];
 		const int32_t py = y & ~1, px = x & ~1;
-		RowInterval upper = rowInterval(es, degenerate, rowClip, py - al.t);
-		RowInterval lower = rowInterval(es, degenerate, rowClip, py + 1 - al.t);
+		RowInterval upper = triangleRow(tr, py);
+		RowInterval lower = triangleRow(tr, py + 1);
 		QuadEval e;
 		if (!evalQuad(cmd, upper, lower, px, py, cv.height, e)) continue;
 		const bool affine = (cmd.flags & 1) != 0;

@@ -73,6 +73,7 @@ struct FrameDev {
 	int32_t *blockSums;           // scan scratch (per block: int32 for triangles, int64 for tiles)
 	Command *commands;
 	ShaderInputs *shaders;        // per command: the finished shader inputs of its source triangle
+	TriangleRows *triangleRows;   // per command: edges and bounds of its rows
 	int32_t commandCapacity;
 	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flags, [3] queued tiles,
 	                              // [4] raster head, [5] backward head, [16..23] per-XCD raster heads
@@ -86,6 +87,11 @@ struct FrameDev {
 	int32_t *queueHead;           // next queue slot for persistent raster workgroups
 	int32_t tileW, tileH;
 	int32_t alphaAny;             // any task uses the Alpha filter: tiles read colour first
+	// The visibility raster fills the canvases itself (image_fill) instead of a separate clear:
+	// it then visits every tile, and empty ones only take the fill values.
+	int32_t clearInRaster;
+	uint32_t clearColor;
+	float clearDepth;
 	// Differentiable frames (mode 1): float RGBA texture (level 0), per-point RGB multipliers
 	// and the gradient buffers of the adjoint.
 	int32_t mode;

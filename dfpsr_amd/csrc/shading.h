@@ -173,20 +173,23 @@ DH float interp(const f3 &d, float wa, float wb, float wc) { return ((d.x * wa) 
 
 struct Quad4 { float v[4]; };
 
-// getPixels_2x2 (RgbaMultiply.h:74-106) for the four lanes of one quad; returns packed RGBA.
+// getPixels_2x2 (RgbaMultiply.h:74-106) for the lanes of one quad in mask (bit l = lane l;
+// the others are left unset).  The mip level always comes from lanes 0, 1 and 2.
 DH void shadeQuad(const ShaderInputs &s, const TextureDesc *textures, const float *wa, const float *wb, const float *wc,
-                  float *r, float *g, float *b, float *a) {
+                  uint32_t mask, float *r, float *g, float *b, float *a) {
 	if (s.diffuse >= 0 && s.light < 0 && s.colorless) {
 		const TextureDesc &t = textures[s.diffuse];
 		float u[4], v[4];
 		for (int l = 0; l < 4; l++) { u[l] = interp(s.u1, wa[l], wb[l], wc[l]); v[l] = interp(s.v1, wa[l], wb[l], wc[l]); }
 		uint32_t mip = mipLevelIndex(t, u[0], u[1], u[2], v[0], v[1], v[2]);
-		for (int l = 0; l < 4; l++) unpackColor(sampleBilinear(t, u[l], v[l], mip, false), r[l], g[l], b[l], a[l]);
+		for (int l = 0; l < 4; l++)
+			if ((mask >> l) & 1u) unpackColor(sampleBilinear(t, u[l], v[l], mip, false), r[l], g[l], b[l], a[l]);
 		return;
 	}
 	if (s.light >= 0 && s.diffuse < 0 && s.colorless) {
 		const TextureDesc &t = textures[s.light];
 		for (int l = 0; l < 4; l++) {
+			if (!((mask >> l) & 1u)) continue;
 			float u = interp(s.u2, wa[l], wb[l], wc[l]);
 			float v = interp(s.v2, wa[l], wb[l], wc[l]);
 			unpackColor(sampleBilinear(t, u, v, 0u, true), r[l], g[l], b[l], a[l]);
@@ -209,6 +212,7 @@ DH void shadeQuad(const ShaderInputs &s, const TextureDesc *textures, const floa
 		for (int l = 0; l < 4; l++) { u[l] = interp(s.u1, wa[l], wb[l], wc[l]); v[l] = interp(s.v1, wa[l], wb[l], wc[l]); }
 		uint32_t mip = mipLevelIndex(t, u[0], u[1], u[2], v[0], v[1], v[2]);
 		for (int l = 0; l < 4; l++) {
+			if (!((mask >> l) & 1u)) continue;
 			float sr, sg, sb, sa;
 			unpackColor(sampleBilinear(t, u[l], v[l], mip, false), sr, sg, sb, sa);
 			r[l] = r[l] * sr; g[l] = g[l] * sg; b[l] = b[l] * sb; a[l] = a[l] * sa;
@@ -217,6 +221,7 @@ DH void shadeQuad(const ShaderInputs &s, const TextureDesc *textures, const floa
 	if (s.light >= 0) {
 		const TextureDesc &t = textures[s.light];
 		for (int l = 0; l < 4; l++) {
+			if (!((mask >> l) & 1u)) continue;
 			float u = interp(s.u2, wa[l], wb[l], wc[l]);
 			float v = interp(s.v2, wa[l], wb[l], wc[l]);
 			float sr, sg, sb, sa;
