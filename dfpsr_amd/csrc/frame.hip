@@ -1,6 +1,7 @@
 // The launch sequence of one frame (renderer_end, or one batch of jobs):
-//   reset counters -> [clear canvases] -> projection -> triangle setup (count, scan, emit)
-//   -> tile binning (count, scan + tile queue, fill) -> tile raster.
+//   [clear canvases, unless the raster fills them] -> frame begin (counters, projection)
+//   -> triangle setup (count, scan, emit with per-command row setup, shader inputs and tile
+//   counts) -> tile offsets (scan + ordered tile queue) -> list fill -> tile raster.
 // Buffers that depend on data (draw commands, tile pairs) start from an estimate; a pass that
 // runs out of room raises a flag, leaves the canvases untouched, and the frame is run again
 // with the exact sizes.  With RenderOptions::graph the sequence is captured once into a
@@ -132,18 +133,18 @@ static inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks, bool events, int parts) {
 	const int32_t tileTotal = f.tileTotal;
 	if (parts & 1) {
-	hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, f.counters, COUNTERS, 0);
-	hipLaunchKernelGGL(k_fill, dim3(blocksOf(2 * (int64_t)tileTotal + 1, 256)), dim3(256), 0, stream, f.tileCount, 2 * tileTotal + 1, 0);
-	hipLaunchKernelGGL(k_fill, dim3(blocksOf(f.canvasCount, 256)), dim3(256), 0, stream, f.canvasCommandBase, f.canvasCount, 0x7FFFFFFF);
 	if (o.clear && !f.clearInRaster) hipLaunchKernelGGL(k_clear_canvases, dim3(tileTotal), dim3(256), 0, stream, f, o.clearColor, o.clearDepth);
 	if (events) HIP_TRY(hipEventRecord(events_[0], stream));
-	hipLaunchKernelGGL(k_project, dim3(blocksOf(f.pointTotal, 256)), dim3(256), 0, stream, f);
+	int64_t beginThreads = 2 * (int64_t)tileTotal + 1;
+	if (beginThreads < f.pointTotal) beginThreads = f.pointTotal;
+	if (beginThreads < f.canvasCount) beginThreads = f.canvasCount;
+	if (beginThreads < COUNTERS) beginThreads = COUNTERS;
+	hipLaunchKernelGGL(k_frame_begin, dim3(blocksOf(beginThreads, 256)), dim3(256), 0, stream, f);
 	if (events) HIP_TRY(hipEventRecord(events_[1], stream));
 	hipLaunchKernelGGL(k_setup_count, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
 	hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, fanBlocks, f.counters + 0);
 	hipLaunchKernelGGL(k_setup_emit, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
 	if (events) HIP_TRY(hipEventRecord(events_[2], stream));
-	hipLaunchKernelGGL(k_bin_count, dim3(blocksOf(f.commandCapacity, 256)), dim3(256), 0, stream, f);
 	hipLaunchKernelGGL(k_scan_tiles_local, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
 	hipLaunchKernelGGL(k_scan_block_sums64, dim3(1), dim3(SCAN_BLOCK), 0, stream, (long long *)f.blockSums, tileBlocks,
 	                   f.counters + 1, f.counters + 3);
@@ -161,9 +162,9 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 		hipLaunchKernelGGL(k_raster_diff, dim3(grid), dim3(64), 0, stream, f, persistent);
 	} else if (deferredRaster(o)) {
 		switch (tileConfig_) {
-		case 1: hipLaunchKernelGGL((k_raster_deferred<32, 16>), dim3(grid), dim3(DEFER_THREADS), 0, stream, f, persistent); break;
-		case 2: hipLaunchKernelGGL((k_raster_deferred<32, 32>), dim3(grid), dim3(DEFER_THREADS), 0, stream, f, persistent); break;
-		default: hipLaunchKernelGGL((k_raster_deferred<16, 16>), dim3(grid), dim3(DEFER_THREADS), 0, stream, f, persistent); break;
+		case 1: hipLaunchKernelGGL((k_raster_deferred<32, 16>), dim3(grid), dim3(32 * 16 / 4), 0, stream, f, persistent); break;
+		case 2: hipLaunchKernelGGL((k_raster_deferred<32, 32>), dim3(grid), dim3(32 * 32 / 4), 0, stream, f, persistent); break;
+		default: hipLaunchKernelGGL((k_raster_deferred<16, 16>), dim3(grid), dim3(16 * 16 / 4), 0, stream, f, persistent); break;
 		}
 	} else switch (tileConfig_) {
 	case 1: hipLaunchKernelGGL((k_raster<32, 16>), dim3(grid), dim3(128), 0, stream, f, persistent); break;
@@ -254,6 +255,7 @@ int Frame::run(const RenderOptions &o) {
 		f.canvasTileEnd = (const int32_t *)(base + offTileEnd_);
 		f.tileTotal = tileTotal;
 		f.textures = (const TextureDesc *)(base + offTextures_);
+		f.textureCount = (int32_t)textures_.size();
 		f.projected = dProjected_.as<Projected>();
 		f.fanCount = dFanCount_.as<int32_t>();
 		f.blockSums = dBlockSums_.as<int32_t>();

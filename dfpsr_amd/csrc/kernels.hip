@@ -12,9 +12,7 @@ namespace dfpsr {
 // ---------------------------------------------------------------------------------------
 // Projection of every model point once per task (modelAPI.cpp:237-242).
 // ---------------------------------------------------------------------------------------
-__global__ void k_project(FrameDev f) {
-	int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= f.pointTotal) return;
+__device__ void projectPoint(const FrameDev &f, int32_t i) {
 	int32_t ti = findSegment(f.taskPointEnd, f.taskCount, i);
 	const TaskDev &t = f.tasks[ti];
 	int32_t local = i - t.pointBase;
@@ -199,30 +197,6 @@ __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_
 	}
 }
 
-__global__ void k_setup_emit(FrameDev f) {
-	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
-	int32_t n = (g < f.fanTotal) ? f.fanCount[g] : 0;
-	int32_t ex;
-	blockExclusiveScan(n, ex);
-	if (g >= f.fanTotal) return;
-	const int32_t offset = f.blockSums[blockIdx.x] + ex;
-	const int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
-	// The commands of a canvas's tasks are consecutive, so the offset at the first fan
-	// triangle of each task bounds its canvas's first command: one atomic per task.
-	if (g == f.tasks[ti].fanBase) atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
-	if (n == 0) return;
-	if (offset + n > f.commandCapacity) {
-		atomicOr(&f.counters[2], 1);
-		return;
-	}
-	setupTriangle<true>(f, g, ti, f.commands + offset);
-}
-
-// ---------------------------------------------------------------------------------------
-// Binning into screen tiles.  Lists are filled with atomics and sorted by command index in
-// the raster kernel, which restores submission order per pixel.
-// ---------------------------------------------------------------------------------------
-
 struct TileRange { int32_t tx0, tx1, ty0, ty1; bool any; };
 
 DH Rect canvasRect(const CanvasDev &cv) { return rectSides(0, 0, cv.width, cv.height); }
@@ -245,24 +219,47 @@ DH TileRange commandTiles(const Command &c, const CanvasDev &cv, int32_t tw, int
 	return r;
 }
 
-// Counts each command's tiles and sets up its rows and shader inputs for the raster.
-__global__ void k_bin_count(FrameDev f) {
-	int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= f.counters[0] || f.counters[2]) return;
+// Sets up draw command c for the raster -- its rows and finished shader inputs -- and counts
+// it into every tile it can touch.
+__device__ void binCommand(const FrameDev &f, int32_t c) {
 	const Command &cmd = f.commands[c];
-	f.triangleRows[c] = triangleRows(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]},
-	                                 canvasRect(f.canvases[cmd.canvas]));
+	const CanvasDev &cv = f.canvases[cmd.canvas];
+	f.triangleRows[c] = triangleRows(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]}, canvasRect(cv));
 	if (f.mode == 0) {
 		const TaskDev &t = f.tasks[cmd.task];
 		const int32_t fp = t.fanPolygon[cmd.fan];
 		f.shaders[c] = triangleShader(t, fp >> 1, fp & 1);
 	}
-	const CanvasDev &cv = f.canvases[cmd.canvas];
 	TileRange r = commandTiles(cmd, cv, f.tileW, f.tileH);
 	if (!r.any) return;
 	for (int32_t ty = r.ty0; ty <= r.ty1; ty++)
 		for (int32_t tx = r.tx0; tx <= r.tx1; tx++) atomicAdd(&f.tileCount[cv.tileBase + ty * cv.tilesX + tx], 1);
 }
+
+__global__ void k_setup_emit(FrameDev f) {
+	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+	int32_t n = (g < f.fanTotal) ? f.fanCount[g] : 0;
+	int32_t ex;
+	blockExclusiveScan(n, ex);
+	if (g >= f.fanTotal) return;
+	const int32_t offset = f.blockSums[blockIdx.x] + ex;
+	const int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
+	// The commands of a canvas's tasks are consecutive, so the offset at the first fan
+	// triangle of each task bounds its canvas's first command: one atomic per task.
+	if (g == f.tasks[ti].fanBase) atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
+	if (n == 0) return;
+	if (offset + n > f.commandCapacity) {
+		atomicOr(&f.counters[2], 1);
+		return;
+	}
+	setupTriangle<true>(f, g, ti, f.commands + offset);
+	for (int32_t k = 0; k < n; k++) binCommand(f, offset + k);
+}
+
+// ---------------------------------------------------------------------------------------
+// Binning into screen tiles.  Lists are filled with atomics and sorted by command index in
+// the raster kernel, which restores submission order per pixel.
+// ---------------------------------------------------------------------------------------
 
 // Tile offsets and the ordered queue of non-empty tiles come from one scan of
 // (count | nonEmpty << 32); the block-local queue position waits in tileFill, which the
@@ -634,11 +631,11 @@ __device__ void drawCommandDiff(const FrameDev &f, const Command &cmd, int32_t c
 // commands in (last, threshold] for the largest threshold that keeps the round within
 // TILE_SORT_CAP (all of them in one round when the list fits).  Every one of the workgroup's
 // NT threads calls it; returns the round's size.
-template <int NT>
+template <int NT, int CAP = TILE_SORT_CAP>
 __device__ int32_t nextRound(const int32_t *src, int32_t n, int32_t done, int32_t &last, int32_t *list, int32_t *roundInfo) {
 	const int32_t t = threadIdx.x;
 	int32_t take;
-	if (done == 0 && n <= TILE_SORT_CAP) {
+	if (done == 0 && n <= CAP) {
 		for (int32_t i = t; i < n; i += NT) list[i] = src[i];
 		take = n;
 		last = 0x7FFFFFFE;
@@ -656,7 +653,7 @@ __device__ int32_t nextRound(const int32_t *src, int32_t n, int32_t done, int32_
 			for (int32_t i = t; i < n; i += NT) { int32_t v = src[i]; cnt += (v > last && v <= mid) ? 1 : 0; }
 			if (cnt) atomicAdd(&roundInfo[2], cnt);
 			__syncthreads();
-			if (t == 0) { if (roundInfo[2] <= TILE_SORT_CAP) roundInfo[0] = mid; else roundInfo[1] = mid - 1; }
+			if (t == 0) { if (roundInfo[2] <= CAP) roundInfo[0] = mid; else roundInfo[1] = mid - 1; }
 		}
 		__syncthreads();
 		int32_t threshold = roundInfo[0];
@@ -823,33 +820,81 @@ __device__ __forceinline__ uint32_t depthOrder(float d) {
 	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// Per chunk command: what every row of it needs (rasterizeTriangle's edges, clip and aligned
-// bound) and the weight planes, set up once.
-struct ChunkCommand {
-	TriangleRows rows;
-	Projection plane;
-	int32_t flags, command;
+// Threads of a visibility-raster workgroup: four pixels of the tile per thread.
+template <int TW, int TH> struct DeferShape {
+	static constexpr int32_t NT = TW * TH / 4;
+	static constexpr int32_t CHUNK = 32;                   // commands per chunk
+	static constexpr int32_t LIST = NT >= 256 ? TILE_SORT_CAP : 512;  // commands sorted per round
 };
+
+constexpr int32_t LDS_TEXTURES = 16;  // texture tables up to this size are read from LDS
+
+// The words of a draw command from its weight planes to its flags (Command::start .. flags).
+struct CommandTail {
+	Projection plane;
+	int32_t task, fan, canvas, flags;
+};
+constexpr int32_t COMMAND_TAIL_OFFSET = 48;  // bytes of Command before start
+static_assert(offsetof(Command, start) == COMMAND_TAIL_OFFSET && offsetof(Command, flags) == COMMAND_TAIL_OFFSET + 48 &&
+              sizeof(CommandTail) == 52, "CommandTail mirrors the tail of Command");
 
 // LDS of one visibility-raster workgroup.
 template <int TW, int TH> struct DeferLds {
-	static constexpr int32_t CHUNK = TH == 16 ? 64 : 32;   // commands per chunk
-	static constexpr int32_t PAIRS = CHUNK * (TH / 2);      // (command, row pair) slots of a chunk
-	int32_t list[TILE_SORT_CAP];                 // the round's commands, sorted
-	ChunkCommand cmds[CHUNK];
-	RowInterval rows[CHUNK * TH];                // chunk command j, tile row r
+	using S = DeferShape<TW, TH>;
+	static constexpr int32_t PAIRS = S::CHUNK * (TH / 2);  // (command, row pair) slots of a chunk
+	int32_t list[S::LIST];                       // the round's commands, sorted
+	CommandTail tail[S::CHUNK];                  // the chunk's weight planes and flags
+	RowInterval rows[S::CHUNK * TH];             // chunk command j, tile row r
 	int32_t pairStart[PAIRS];                    // first quad column (pixel x) of each pair in the tile
 	int32_t pairOffset[PAIRS + 1];               // exclusive prefix of quads per pair
 	unsigned long long keys[TW * TH];
 	uint32_t color[TW * TH];                     // the tile's canvas pixels as they stand
 	float depth[TW * TH];
 	uint8_t dirty[TW * TH];
+	int32_t scan[S::NT];
+	TextureDesc textures[LDS_TEXTURES];
 	int32_t roundInfo[4];
 	int32_t slot;
 };
 
-// Exclusive prefix over per-thread sums of the workgroup; returns the total.
-__device__ __forceinline__ int32_t workgroupPrefix(int32_t value, int32_t &before) { return blockExclusiveScan(value, before); }
+// Copies W words per chunk command (from byte offset `offset` of element list[c0 + j] of an
+// array with `stride` bytes per element) into dst[j]: all of a thread's loads are issued
+// before any of them is waited on.
+template <int NT, int CHUNK, int W>
+__device__ __forceinline__ void stageWords(uint32_t *dst, const uint8_t *src, int32_t stride, int32_t offset,
+                                           const int32_t *list, int32_t c0, int32_t cn) {
+	constexpr int32_t K = (CHUNK * W + NT - 1) / NT;
+	uint32_t v[K];
+#pragma unroll
+	for (int k = 0; k < K; k++) {
+		const int32_t idx = (int32_t)threadIdx.x + k * NT;
+		const int32_t j = idx / W, w = idx % W;
+		v[k] = j < cn ? ((const uint32_t *)(src + (size_t)list[c0 + j] * stride + offset))[w] : 0u;
+	}
+#pragma unroll
+	for (int k = 0; k < K; k++) {
+		const int32_t idx = (int32_t)threadIdx.x + k * NT;
+		if (idx / W < cn) dst[idx] = v[k];
+	}
+}
+
+// Exclusive prefix over one value per thread of an NT-thread workgroup; returns the total.
+template <int NT>
+__device__ __forceinline__ int32_t groupPrefix(int32_t value, int32_t &before, int32_t *scan) {
+	const int32_t t = threadIdx.x;
+	scan[t] = value;
+	__syncthreads();
+	for (int32_t offset = 1; offset < NT; offset <<= 1) {
+		const int32_t v = t >= offset ? scan[t - offset] : 0;
+		__syncthreads();
+		scan[t] += v;
+		__syncthreads();
+	}
+	const int32_t inclusive = scan[t], total = scan[NT - 1];
+	__syncthreads();
+	before = inclusive - value;
+	return total;
+}
 
 // Index of the segment holding item it in an exclusive prefix of count + 1 entries.
 __device__ __forceinline__ int32_t segmentOf(const int32_t *offset, int32_t count, int32_t it) {
@@ -862,11 +907,12 @@ __device__ __forceinline__ int32_t segmentOf(const int32_t *offset, int32_t coun
 }
 
 template <int TW, int TH>
-__device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW, TH> &s) {
-	using L = DeferLds<TW, TH>;
-	constexpr int32_t NP = TW * TH, NQX = TW / 2, RP = TH / 2;
-	constexpr int32_t CHUNK = L::CHUNK, PAIRS = L::PAIRS;
-	constexpr int32_t PAIRS_PER_THREAD = PAIRS / DEFER_THREADS;
+__device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW, TH> &s, const TextureDesc *textures) {
+	using S = DeferShape<TW, TH>;
+	constexpr int32_t NT = S::NT, CHUNK = S::CHUNK;
+	constexpr int32_t NP = TW * TH, RP = TH / 2;
+	constexpr int32_t PAIRS = DeferLds<TW, TH>::PAIRS;
+	constexpr int32_t PAIRS_PER_THREAD = PAIRS / NT > 0 ? PAIRS / NT : 1;
 	const int32_t n = f.tileCount[tile];
 	const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
 	const CanvasDev cv = f.canvases[ci];
@@ -878,7 +924,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	const bool colorWrite = cv.color != nullptr;
 	const bool depthRead = cv.depth != nullptr;
 
-	for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+	for (int32_t p = t; p < NP; p += NT) {
 		if (f.clearInRaster) {
 			s.depth[p] = f.clearDepth;
 			s.color[p] = f.clearColor;
@@ -893,37 +939,22 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	int32_t last = -1;
 	int32_t done = 0;
 	while (done < n) {
-		const int32_t take = nextRound<DEFER_THREADS>(src, n, done, last, s.list, s.roundInfo);
+		const int32_t take = nextRound<NT, S::LIST>(src, n, done, last, s.list, s.roundInfo);
 		for (int32_t c0 = 0; c0 < take; c0 += CHUNK) {
 			const int32_t cn = take - c0 < CHUNK ? take - c0 : CHUNK;
 			// Keys start at the held depth with the lowest command part above every command's,
 			// so only a strictly greater depth replaces it; a NaN depth is never replaced.
-			for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+			for (int32_t p = t; p < NP; p += NT) {
 				const float d = s.depth[p];
 				s.keys[p] = !depthRead ? 0ull : (d != d ? ~0ull : (((unsigned long long)depthOrder(d) << 32) | 0xFFFFFFFFull));
 			}
-			// The chunk commands' row setups and weight planes, copied word by word.
-			constexpr int32_t ROW_WORDS = (int32_t)(sizeof(TriangleRows) / 4);
-			for (int32_t idx = t; idx < cn * ROW_WORDS; idx += DEFER_THREADS) {
-				const int32_t j = idx / ROW_WORDS, w = idx % ROW_WORDS;
-				((uint32_t *)&s.cmds[j].rows)[w] = ((const uint32_t *)&f.triangleRows[s.list[c0 + j]])[w];
-			}
-			for (int32_t j = t; j < cn; j += DEFER_THREADS) {
-				const int32_t c = s.list[c0 + j];
-				const Command &cmd = f.commands[c];
-				Projection &pl = s.cmds[j].plane;
-				pl.start.x = cmd.start.x; pl.start.y = cmd.start.y; pl.start.z = cmd.start.z;
-				pl.dx.x = cmd.dx.x; pl.dx.y = cmd.dx.y; pl.dx.z = cmd.dx.z;
-				pl.dy.x = cmd.dy.x; pl.dy.y = cmd.dy.y; pl.dy.z = cmd.dy.z;
-				s.cmds[j].flags = cmd.flags;
-				s.cmds[j].command = c;
-			}
-			__syncthreads();
-			// Rows of every chunk command in the tile, one (command, row) per thread: the exact
-			// intervals of rasterizeTriangle inside the aligned bound, empty outside it.
-			for (int32_t idx = t; idx < cn * TH; idx += DEFER_THREADS) {
-				s.rows[idx] = triangleRow(s.cmds[idx / TH].rows, tileY0 + idx % TH);
-			}
+			// The chunk's weight planes into LDS, and the rows of every chunk command in the tile,
+			// one (command, row) per thread: the exact intervals of rasterizeTriangle inside the
+			// aligned bound, empty outside it.
+			stageWords<NT, CHUNK, (int32_t)(sizeof(CommandTail) / 4)>((uint32_t *)s.tail, (const uint8_t *)f.commands,
+			                                                           (int32_t)sizeof(Command), COMMAND_TAIL_OFFSET, s.list, c0, cn);
+			for (int32_t idx = t; idx < cn * TH; idx += NT)
+				s.rows[idx] = triangleRow(f.triangleRows[s.list[c0 + idx / TH]], tileY0 + idx % TH);
 			__syncthreads();
 			// Quads of each (command, row pair) inside the tile: the outer span of the row pair
 			// (fillShapeSuper), cut to the tile's columns.
@@ -933,7 +964,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				const int32_t pr = t * PAIRS_PER_THREAD + k;
 				const int32_t j = pr / RP, rp = pr % RP;
 				int32_t count = 0, x0 = 0;
-				if (j < cn) {
+				if (pr < PAIRS && j < cn) {
 					const RowInterval upper = s.rows[j * TH + 2 * rp], lower = s.rows[j * TH + 2 * rp + 1];
 					if (upper.right > upper.left || lower.right > lower.left) {
 						const int32_t outerBlockStart = (int32_t)((uint32_t)tmin(upper.left, lower.left) & ~1u);
@@ -943,26 +974,27 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 						count = x1 > x0 ? (x1 - x0) >> 1 : 0;
 					}
 				}
-				s.pairStart[pr] = x0;
+				if (pr < PAIRS) s.pairStart[pr] = x0;
 				pairCount[k] = count;
 				threadSum += count;
 			}
 			int32_t before;
-			const int32_t items = workgroupPrefix(threadSum, before);
+			const int32_t items = groupPrefix<NT>(threadSum, before, s.scan);
 			for (int32_t k = 0; k < PAIRS_PER_THREAD; k++) {
-				s.pairOffset[t * PAIRS_PER_THREAD + k] = before;
+				const int32_t pr = t * PAIRS_PER_THREAD + k;
+				if (pr < PAIRS) s.pairOffset[pr] = before;
 				before += pairCount[k];
 			}
 			if (t == 0) s.pairOffset[PAIRS] = items;
 			__syncthreads();
 			// Pass 1: one (command, quad) item per thread offers the covered lanes' keys.
-			for (int32_t it = t; it < items; it += DEFER_THREADS) {
+			for (int32_t it = t; it < items; it += NT) {
 				const int32_t pr = segmentOf(s.pairOffset, PAIRS, it);
 				const int32_t j = pr / RP, rp = pr % RP;
 				const int32_t px = s.pairStart[pr] + 2 * (it - s.pairOffset[pr]);
 				const int32_t py = tileY0 + 2 * rp;
 				QuadEval e;
-				if (!evalQuad(s.cmds[j].plane, s.rows[j * TH + 2 * rp], s.rows[j * TH + 2 * rp + 1], px, py, cv.height, e)) continue;
+				if (!evalQuad(s.tail[j].plane, s.rows[j * TH + 2 * rp], s.rows[j * TH + 2 * rp + 1], px, py, cv.height, e)) continue;
 				for (int l = 0; l < 4; l++) {
 					if (e.clipSides && !e.clip[l]) continue;
 					const int32_t sl = e.slot[l];
@@ -977,7 +1009,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 			}
 			__syncthreads();
 			// Pass 2: each pixel evaluates its winning command's quad and shades its own lane.
-			for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+			for (int32_t p = t; p < NP; p += NT) {
 				const unsigned long long key = s.keys[p];
 				int32_t jw;
 				if (depthRead) {
@@ -990,19 +1022,19 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				const int32_t xl = p % TW, yl = p / TW;
 				const int32_t qx = xl & ~1, qy = yl & ~1;
 				const int32_t sl = (yl & 1) * 2 + (xl & 1);
-				const ChunkCommand &cc = s.cmds[jw];
 				QuadEval e;
-				if (!evalQuad(cc.plane, s.rows[jw * TH + qy], s.rows[jw * TH + qy + 1], tileX0 + qx, tileY0 + qy, cv.height, e)) continue;
+				if (!evalQuad(s.tail[jw].plane, s.rows[jw * TH + qy], s.rows[jw * TH + qy + 1], tileX0 + qx, tileY0 + qy, cv.height, e))
+					continue;
 				int l = -1;
 				for (int k = 0; k < 4; k++)
 					if ((!e.clipSides || e.clip[k]) && e.slot[k] == sl) l = k;
 				if (l < 0) continue;
 				if (colorWrite) {
-					const bool affine = (cc.flags & 1) != 0;
+					const bool affine = (s.tail[jw].flags & 1) != 0;
 					float wa[4], wb[4], wc[4];
 					for (int k = 0; k < 4; k++) laneWeights(e, k, affine, wa[k], wb[k], wc[k]);
 					float r, g, b, a;
-					shadeLane(f.shaders[cc.command], f.textures, wa, wb, wc, l, r, g, b, a);
+					shadeLane(f.shaders[s.list[c0 + jw]], textures, wa, wb, wc, l, r, g, b, a);
 					s.color[p] = packSaturated(r, g, b, a);
 				}
 				s.depth[p] = pickLane(e.l0, l);
@@ -1014,7 +1046,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 		if (take == 0) break;
 	}
 
-	for (int32_t p = t; p < NP; p += DEFER_THREADS) {
+	for (int32_t p = t; p < NP; p += NT) {
 		if (!s.dirty[p]) continue;
 		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
 		if (x >= cv.width || y >= cv.height) continue;
@@ -1024,10 +1056,19 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 }
 
 template <int TW, int TH>
-__global__ void __launch_bounds__(DEFER_THREADS) k_raster_deferred(FrameDev f, int32_t persistent) {
+__global__ void __launch_bounds__(TW * TH / 4) k_raster_deferred(FrameDev f, int32_t persistent) {
 	__shared__ DeferLds<TW, TH> s;
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
-	forEachQueuedTile(f, persistent, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH>(f, tile, s); });
+	// Small texture tables are read from LDS by the shading.
+	const bool ldsTextures = f.textureCount <= LDS_TEXTURES;
+	if (ldsTextures) {
+		constexpr int32_t W = (int32_t)(sizeof(TextureDesc) / 4);
+		for (int32_t i = threadIdx.x; i < f.textureCount * W; i += TW * TH / 4)
+			((uint32_t *)s.textures)[i] = ((const uint32_t *)f.textures)[i];
+		__syncthreads();
+	}
+	const TextureDesc *textures = ldsTextures ? s.textures : f.textures;
+	forEachQueuedTile(f, persistent, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH>(f, tile, s, textures); });
 }
 
 template __global__ void k_raster_deferred<16, 16>(FrameDev, int32_t);
@@ -1108,6 +1149,16 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 				for (int ch = 0; ch < 3; ch++) unsafeAtomicAdd(f.gradMultipliers + d.point[c3] * 3 + ch, gm[ch] * w3[c3]);
 		}
 	}
+}
+
+// The start of a frame: zeroed counters, tile counts and fill cursors, unset canvas command
+// bases, and the projection of every model point (modelAPI.cpp:237-242), in one launch.
+__global__ void k_frame_begin(FrameDev f) {
+	const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < COUNTERS) f.counters[i] = 0;
+	if (i < 2 * f.tileTotal + 1) f.tileCount[i] = 0;
+	if (i < f.canvasCount) f.canvasCommandBase[i] = 0x7FFFFFFF;
+	if (i < f.pointTotal) projectPoint(f, i);
 }
 
 __global__ void k_fill(int32_t *p, int32_t n, int32_t value) {

@@ -68,6 +68,7 @@ struct FrameDev {
 	const int32_t *canvasTileEnd; // inclusive prefix of tiles per canvas
 	int32_t tileTotal;
 	const TextureDesc *textures;
+	int32_t textureCount;
 	Projected *projected;         // pointTotal
 	int32_t *fanCount;            // fanTotal: commands made by each fan triangle
 	int32_t *blockSums;           // scan scratch (per block: int32 for triangles, int64 for tiles)
@@ -112,18 +113,16 @@ DH int32_t findSegment(const int32_t *inclusiveEnd, int32_t count, int32_t index
 }
 
 // Kernel entry points (kernels.hip).
-__global__ void k_project(FrameDev f);
+__global__ void k_frame_begin(FrameDev f);
 __global__ void k_setup_count(FrameDev f);
 __global__ void k_scan_block_sums(int32_t *blockSums, int32_t blocks, int32_t *total);
 __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_t *totalLow, int32_t *totalHigh);
 __global__ void k_setup_emit(FrameDev f);
-__global__ void k_bin_count(FrameDev f);
 __global__ void k_scan_tiles_local(FrameDev f);
 __global__ void k_scan_tiles_add(FrameDev f);
 __global__ void k_bin_fill(FrameDev f);
 template <int TW, int TH> __global__ void k_raster(FrameDev f, int32_t persistent);
 template <int TW, int TH> __global__ void k_raster_deferred(FrameDev f, int32_t persistent);
-constexpr int32_t DEFER_THREADS = 256;  // threads of a visibility-raster workgroup (four waves)
 __global__ void k_raster_diff(FrameDev f, int32_t persistent);
 __global__ void k_backward(FrameDev f, int32_t *head);
 __global__ void k_fill(int32_t *p, int32_t n, int32_t value);
