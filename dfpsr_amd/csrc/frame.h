@@ -126,6 +126,7 @@ private:
 	DevBuf dShaders_, dTriangleRows_;
 	DevBuf dCanvasBase_, dProjected_, dFanCount_, dBlockSums_, dCommands_, dCounters_, dTileCount_, dTileOffset_, dTileFill_, dTileList_, dTileQueue_;
 	int32_t *hostCounters_ = nullptr;
+	int32_t *hostCountersDevice_ = nullptr;
 	hipEvent_t events_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 	hipGraph_t graph_ = nullptr;  // the whole sequence, or (timing on) everything before the raster
 	hipGraphExec_t graphExec_ = nullptr;

@@ -23,6 +23,7 @@ Frame::~Frame() {
 int Frame::init() {
 	HIP_TRY(hipStreamCreate(&stream));  // blocking: ordered after work on the legacy default stream
 	HIP_TRY(hipHostMalloc((void **)&hostCounters_, 8 * sizeof(int32_t), hipHostMallocDefault));
+	HIP_TRY(hipHostGetDevicePointer((void **)&hostCountersDevice_, hostCounters_, 0));
 	for (hipEvent_t &e : events_) HIP_TRY(hipEventCreate(&e));
 	int device = 0;
 	HIP_TRY(hipGetDevice(&device));
@@ -265,6 +266,7 @@ int Frame::run(const RenderOptions &o) {
 		f.commandCapacity = commandCapacity_;
 		f.counters = dCounters_.as<int32_t>();
 		f.queueHead = dCounters_.as<int32_t>() + 4;
+		f.hostCounters = hostCountersDevice_;
 		f.tileCount = dTileCount_.as<int32_t>();
 		f.tileFill = dTileCount_.as<int32_t>() + tileTotal + 1;
 		f.tileOffset = dTileOffset_.as<int32_t>();
@@ -329,8 +331,11 @@ int Frame::run(const RenderOptions &o) {
 			int rc = enqueue(f, o, fanBlocks, tileBlocks, timing, 3);
 			if (rc) return rc;
 		}
-		HIP_TRY(hipMemcpyAsync(hostCounters_, f.counters, 16, hipMemcpyDeviceToHost, stream));
+		// The raster kernel copies counters[0..3] into pinned host memory as it starts; the
+		// sentinel shows whether it did.
+		hostCounters_[2] = -1;
 		HIP_TRY(hipStreamSynchronize(stream));
+		if (hostCounters_[2] == -1) return fail(-3, "renderer_end: the frame did not report its counters");
 		stats[0] = hostCounters_[0];
 		stats[1] = hostCounters_[1];
 		stats[2] = hostCounters_[3];

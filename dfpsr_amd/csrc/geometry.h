@@ -471,7 +471,7 @@ DH bool sameCorner(const l2 &a, const l2 &b) { return a.x == b.x && a.y == b.y; 
 // What every row of a triangle needs (ITriangle2D.cpp:70-176): the three edges of
 // rasterizeTriangle, the clipped columns and the aligned row bound.  Set up once per draw
 // command; rows are then independent of each other.
-struct TriangleRows {
+struct __attribute__((aligned(16))) TriangleRows {
 	EdgeSetup edges[3];
 	int32_t clipL, clipR, alT, alB;
 	int32_t degenerate, pad;

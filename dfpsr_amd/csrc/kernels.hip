@@ -745,6 +745,12 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 // queue slot b (the grid covers every tile; surplus workgroups exit).  persistent = 1: a fixed
 // grid whose workgroups pull slots from an atomic head until the queue is empty, so every
 // workgroup reaches the exit once all tiles are handed out.
+// The raster kernels start after every counter is final: workgroup 0 copies them to the
+// frame's pinned host memory, which the host reads after the stream completes.
+__device__ __forceinline__ void reportCounters(const FrameDev &f) {
+	if (blockIdx.x == 0 && threadIdx.x < 4) f.hostCounters[threadIdx.x] = f.counters[threadIdx.x];
+}
+
 // Runs body(tile) over the ordered queue of non-empty tiles, XCD by XCD.  Without
 // persistence the grid has 8 * ceil(tiles / 8) workgroups and workgroup b takes slot
 // b / 8 of its XCD's range.  Persistent workgroups pull slots from their own XCD's head and
@@ -783,6 +789,7 @@ __global__ void __launch_bounds__((TW / 2) * (TH / 2)) k_raster(FrameDev f, int3
 	__shared__ RowInterval rows[TH];
 	__shared__ int32_t roundInfo[3];
 	__shared__ int32_t slotLds;
+	reportCounters(f);
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
 	forEachQueuedTile(f, persistent, &slotLds, [&](int32_t tile) { rasterTile<TW, TH, 0>(f, tile, list, rows, roundInfo); });
 }
@@ -824,30 +831,37 @@ __device__ __forceinline__ uint32_t depthOrder(float d) {
 template <int TW, int TH> struct DeferShape {
 	static constexpr int32_t NT = TW * TH / 4;
 	static constexpr int32_t CHUNK = 32;                   // commands per chunk
-	static constexpr int32_t LIST = NT >= 256 ? TILE_SORT_CAP : 512;  // commands sorted per round
+	static constexpr int32_t LIST = 256;                    // commands sorted per round
 };
 
 constexpr int32_t LDS_TEXTURES = 16;  // texture tables up to this size are read from LDS
 
-// The words of a draw command from its weight planes to its flags (Command::start .. flags).
-struct CommandTail {
+// The bytes of a draw command from its weight planes to its end (Command::start ..).
+struct __attribute__((aligned(16))) CommandTail {
 	Projection plane;
 	int32_t task, fan, canvas, flags;
+	int32_t pad[3];
 };
 constexpr int32_t COMMAND_TAIL_OFFSET = 48;  // bytes of Command before start
 static_assert(offsetof(Command, start) == COMMAND_TAIL_OFFSET && offsetof(Command, flags) == COMMAND_TAIL_OFFSET + 48 &&
-              sizeof(CommandTail) == 52, "CommandTail mirrors the tail of Command");
+              sizeof(CommandTail) == sizeof(Command) - COMMAND_TAIL_OFFSET, "CommandTail mirrors the tail of Command");
+static_assert(sizeof(TriangleRows) % 16 == 0 && sizeof(ShaderInputs) % 16 == 0 && sizeof(CommandTail) % 16 == 0,
+              "staged records are whole 16-byte units");
 
 // LDS of one visibility-raster workgroup.
 template <int TW, int TH> struct DeferLds {
 	using S = DeferShape<TW, TH>;
 	static constexpr int32_t PAIRS = S::CHUNK * (TH / 2);  // (command, row pair) slots of a chunk
 	int32_t list[S::LIST];                       // the round's commands, sorted
-	CommandTail tail[S::CHUNK];                  // the chunk's weight planes and flags
+	union {
+		TriangleRows tr[S::CHUNK];               // the chunk's commands: row setup (until the rows are made),
+		unsigned long long keys[TW * TH];        // then the pixels' (depth, command) keys
+	};
+	ShaderInputs si[S::CHUNK];                   // shader inputs,
+	CommandTail tail[S::CHUNK];                  // weight planes and flags
 	RowInterval rows[S::CHUNK * TH];             // chunk command j, tile row r
 	int32_t pairStart[PAIRS];                    // first quad column (pixel x) of each pair in the tile
 	int32_t pairOffset[PAIRS + 1];               // exclusive prefix of quads per pair
-	unsigned long long keys[TW * TH];
 	uint32_t color[TW * TH];                     // the tile's canvas pixels as they stand
 	float depth[TW * TH];
 	uint8_t dirty[TW * TH];
@@ -857,26 +871,28 @@ template <int TW, int TH> struct DeferLds {
 	int32_t slot;
 };
 
-// Copies W words per chunk command (from byte offset `offset` of element list[c0 + j] of an
-// array with `stride` bytes per element) into dst[j]: all of a thread's loads are issued
-// before any of them is waited on.
-template <int NT, int CHUNK, int W>
-__device__ __forceinline__ void stageWords(uint32_t *dst, const uint8_t *src, int32_t stride, int32_t offset,
+// Copies the 16-byte units [offset, offset + sizeof(T)) of element list[c0 + j] of a global
+// array with `stride` bytes per element into dst[j], for every chunk command j, with loads that
+// write LDS directly (no registers hold the data).  Each wave instruction fills 64
+// consecutive units of dst.  The caller waits for them (stageWait) before reading dst.
+template <int NT, typename T>
+__device__ __forceinline__ void stageToLds(T *dst, const uint8_t *src, int32_t stride, int32_t offset,
                                            const int32_t *list, int32_t c0, int32_t cn) {
-	constexpr int32_t K = (CHUNK * W + NT - 1) / NT;
-	uint32_t v[K];
-#pragma unroll
-	for (int k = 0; k < K; k++) {
-		const int32_t idx = (int32_t)threadIdx.x + k * NT;
-		const int32_t j = idx / W, w = idx % W;
-		v[k] = j < cn ? ((const uint32_t *)(src + (size_t)list[c0 + j] * stride + offset))[w] : 0u;
-	}
-#pragma unroll
-	for (int k = 0; k < K; k++) {
-		const int32_t idx = (int32_t)threadIdx.x + k * NT;
-		if (idx / W < cn) dst[idx] = v[k];
+	constexpr int32_t UNITS = (int32_t)(sizeof(T) / 16);
+	const int32_t total = cn * UNITS;
+	const int32_t lane = (int32_t)(threadIdx.x & 63);
+	for (int32_t base = (int32_t)(threadIdx.x & ~63u); base < total; base += NT) {
+		const int32_t q = base + lane;
+		if (q < total) {
+			const int32_t j = q / UNITS, u = q % UNITS;
+			const uint8_t *g = src + (size_t)list[c0 + j] * stride + offset + u * 16;
+			__builtin_amdgcn_global_load_lds((const void *)g, (void *)((uint8_t *)dst + (size_t)base * 16), 16, 0, 0);
+		}
 	}
 }
+
+// Waits for this thread's LDS-direct loads; a barrier then publishes them to the workgroup.
+__device__ __forceinline__ void stageWait() { __builtin_amdgcn_s_waitcnt(0); }
 
 // Exclusive prefix over one value per thread of an NT-thread workgroup; returns the total.
 template <int NT>
@@ -942,20 +958,22 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 		const int32_t take = nextRound<NT, S::LIST>(src, n, done, last, s.list, s.roundInfo);
 		for (int32_t c0 = 0; c0 < take; c0 += CHUNK) {
 			const int32_t cn = take - c0 < CHUNK ? take - c0 : CHUNK;
+			// The chunk's row setups, shader inputs and weight planes into LDS.
+			stageToLds<NT>(s.tr, (const uint8_t *)f.triangleRows, (int32_t)sizeof(TriangleRows), 0, s.list, c0, cn);
+			if (colorWrite) stageToLds<NT>(s.si, (const uint8_t *)f.shaders, (int32_t)sizeof(ShaderInputs), 0, s.list, c0, cn);
+			stageToLds<NT>(s.tail, (const uint8_t *)f.commands, (int32_t)sizeof(Command), COMMAND_TAIL_OFFSET, s.list, c0, cn);
+			stageWait();
+			__syncthreads();
+			// Rows of every chunk command in the tile, one (command, row) per thread: the exact
+			// intervals of rasterizeTriangle inside the aligned bound, empty outside it.
+			for (int32_t idx = t; idx < cn * TH; idx += NT) s.rows[idx] = triangleRow(s.tr[idx / TH], tileY0 + idx % TH);
+			__syncthreads();
 			// Keys start at the held depth with the lowest command part above every command's,
 			// so only a strictly greater depth replaces it; a NaN depth is never replaced.
 			for (int32_t p = t; p < NP; p += NT) {
 				const float d = s.depth[p];
 				s.keys[p] = !depthRead ? 0ull : (d != d ? ~0ull : (((unsigned long long)depthOrder(d) << 32) | 0xFFFFFFFFull));
 			}
-			// The chunk's weight planes into LDS, and the rows of every chunk command in the tile,
-			// one (command, row) per thread: the exact intervals of rasterizeTriangle inside the
-			// aligned bound, empty outside it.
-			stageWords<NT, CHUNK, (int32_t)(sizeof(CommandTail) / 4)>((uint32_t *)s.tail, (const uint8_t *)f.commands,
-			                                                           (int32_t)sizeof(Command), COMMAND_TAIL_OFFSET, s.list, c0, cn);
-			for (int32_t idx = t; idx < cn * TH; idx += NT)
-				s.rows[idx] = triangleRow(f.triangleRows[s.list[c0 + idx / TH]], tileY0 + idx % TH);
-			__syncthreads();
 			// Quads of each (command, row pair) inside the tile: the outer span of the row pair
 			// (fillShapeSuper), cut to the tile's columns.
 			int32_t pairCount[PAIRS_PER_THREAD];
@@ -1034,7 +1052,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 					float wa[4], wb[4], wc[4];
 					for (int k = 0; k < 4; k++) laneWeights(e, k, affine, wa[k], wb[k], wc[k]);
 					float r, g, b, a;
-					shadeLane(f.shaders[s.list[c0 + jw]], textures, wa, wb, wc, l, r, g, b, a);
+					shadeLane(s.si[jw], textures, wa, wb, wc, l, r, g, b, a);
 					s.color[p] = packSaturated(r, g, b, a);
 				}
 				s.depth[p] = pickLane(e.l0, l);
@@ -1058,6 +1076,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 template <int TW, int TH>
 __global__ void __launch_bounds__(TW * TH / 4) k_raster_deferred(FrameDev f, int32_t persistent) {
 	__shared__ DeferLds<TW, TH> s;
+	reportCounters(f);
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
 	// Small texture tables are read from LDS by the shading.
 	const bool ldsTextures = f.textureCount <= LDS_TEXTURES;
@@ -1081,6 +1100,7 @@ __global__ void __launch_bounds__(64) k_raster_diff(FrameDev f, int32_t persiste
 	__shared__ RowInterval rows[16];
 	__shared__ int32_t roundInfo[3];
 	__shared__ int32_t slotLds;
+	reportCounters(f);
 	if (f.counters[2]) return;
 	forEachQueuedTile(f, persistent, &slotLds, [&](int32_t tile) { rasterTile<16, 16, 1>(f, tile, list, rows, roundInfo); });
 }

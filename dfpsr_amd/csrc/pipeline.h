@@ -50,7 +50,7 @@ struct CanvasDev {
 };
 
 // One draw command: a (sub-)triangle that passed culling, clipping and the front-face test.
-struct Command {
+struct __attribute__((aligned(16))) Command {
 	int64_t fx[3], fy[3];  // corners in sub-pixels (256 per pixel)
 	f3 start, dx, dy;      // weight planes
 	int32_t task, fan;     // source: task and fan triangle within the task's model
@@ -86,6 +86,7 @@ struct FrameDev {
 	int32_t *tileQueue;           // tiles with at least one command, in tile order (counters[3] of them)
 	int32_t *canvasCommandBase;   // per canvas: index of its first draw command (visibility ids count from it)
 	int32_t *queueHead;           // next queue slot for persistent raster workgroups
+	int32_t *hostCounters;        // pinned host copy of counters[0..3], written by the raster kernel
 	int32_t tileW, tileH;
 	int32_t alphaAny;             // any task uses the Alpha filter: tiles read colour first
 	// The visibility raster fills the canvases itself (image_fill) instead of a separate clear:

@@ -144,7 +144,7 @@ __device__ __forceinline__ float refinedReciprocal(float x) {
 // The shader a triangle selects (RgbaMultiply.h:37-58, :110-175): colours pre-scaled by 255
 // and by 1/255 per bound texture, "vertex fading" when a channel differs across corners,
 // "colorless" when every channel is about 1.
-struct ShaderInputs {
+struct __attribute__((aligned(16))) ShaderInputs {
 	f3 red, green, blue, alpha;  // scaled corner colours
 	f3 u1, v1, u2, v2;
 	int32_t diffuse, light;      // texture slots, -1 when absent
