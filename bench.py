@@ -261,20 +261,26 @@ def main() -> int:
 
     for _ in range(args.warmup):
         step(renderer)
+    # Timed region: the K frames as an application draws them.
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(renderer)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats = renderer.stats()
+    progress(f"plain: {elapsed / args.steps * 1e3:.3f} ms per frame (untimed by events)")
+    # The same K frames again with the renderer's HIP events on its own stream around each stage;
+    # reading them back after every frame costs host time, so this pass is not the one timed above.
     renderer.set_timing(True)
     raster_ms = []
     stage_ms = np.zeros(5)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     for _ in range(args.steps):
         step(renderer)
         t = renderer.timing()
         raster_ms.append(float(t[3]))
         stage_ms += t
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    stats = renderer.stats()
-    progress(f"plain: {elapsed / args.steps * 1e3:.3f} ms per frame")
+    renderer.set_timing(False)
     ms_per_step = elapsed / args.steps * 1e3
     raster_avg_ms = float(np.mean(raster_ms))
     achieved = RASTER_BYTES / (raster_avg_ms * 1e-3) / 1e9

@@ -30,6 +30,42 @@ __device__ void projectPoint(const FrameDev &f, int32_t i) {
 // -> drawSubTriangle (renderCore.cpp:220-341).  Returns the number of draw commands the fan
 // triangle makes; with EMIT they are written to out[0..count).
 // ---------------------------------------------------------------------------------------
+DH Rect canvasRect(const CanvasDev &cv) { return rectSides(0, 0, cv.width, cv.height); }
+
+// Writes draw command `index` with everything the raster needs of it -- the command, its row
+// setup and (for the RGBA8 shaders) its finished shader inputs -- and counts it into every tile
+// it can touch: the columns of cut(wholeBound, canvas), the rows of the aligned raster bound
+// (ITriangle2D.cpp:70-84).
+__device__ void emitCommand(const FrameDev &f, int32_t index, const Command &cmd, const CanvasDev &cv,
+                            const ShaderInputs &si) {
+	f.commands[index] = cmd;
+	const TriangleRows tr = triangleRows(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]}, canvasRect(cv));
+	f.triangleRows[index] = tr;
+	if (f.mode == 0) f.shaders[index] = si;
+	if (tr.clipR <= tr.clipL) return;  // the bound misses the canvas
+	const int32_t tx0 = tr.clipL / f.tileW, tx1 = (tr.clipR - 1) / f.tileW;
+	const int32_t ty0 = tr.alT / f.tileH;
+	int32_t ty1 = (tr.alB - 2) / f.tileH;
+	if (ty1 > cv.tilesY - 1) ty1 = cv.tilesY - 1;
+	for (int32_t ty = ty0; ty <= ty1; ty++)
+		for (int32_t tx = tx0; tx <= tx1; tx++) atomicAdd(&f.tileCount[cv.tileBase + ty * cv.tilesX + tx], 1);
+}
+
+// The tiles a command can touch (as counted by emitCommand).
+struct TileRange { int32_t tx0, tx1, ty0, ty1; bool any; };
+
+DH TileRange commandTiles(const TriangleRows &tr, const CanvasDev &cv, int32_t tw, int32_t th) {
+	TileRange r;
+	r.any = tr.clipR > tr.clipL;
+	if (!r.any) { r.tx0 = r.tx1 = r.ty0 = r.ty1 = 0; return r; }
+	r.tx0 = tr.clipL / tw;
+	r.tx1 = (tr.clipR - 1) / tw;
+	r.ty0 = tr.alT / th;
+	r.ty1 = (tr.alB - 2) / th;
+	if (r.ty1 > cv.tilesY - 1) r.ty1 = cv.tilesY - 1;
+	return r;
+}
+
 // The finished shader inputs of fan triangle (poly, k) of a task (Shader.h:37-57,
 // RgbaMultiply.h:37-58): corner colours and texture coordinates, texture slots, the shader
 // variant flags.
@@ -54,7 +90,7 @@ __device__ ShaderInputs triangleShader(const TaskDev &t, int32_t poly, int32_t k
 }
 
 template <bool EMIT>
-__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, Command *out) {
+__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset) {
 	const TaskDev &t = f.tasks[ti];
 	const CanvasDev &cv = f.canvases[t.canvas];
 	if (cv.color == nullptr && cv.depth == nullptr) return 0;
@@ -72,15 +108,18 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, Comma
 	}
 	int32_t count = 0;
 	const bool perspective = cam.perspective != 0;
+	ShaderInputs si;
+	if (EMIT && f.mode == 0) si = triangleShader(t, poly, k);
 	if (triangleVisibility(P[0].cs, P[1].cs, P[2].cs, cam.clip) == VIS_FULL) {
 		if (frontFacing(P[0].flat, P[1].flat, P[2].flat)) {
 			if (EMIT) {
-				Command &c = out[0];
+				Command c;
 				const f2 is[3] = {P[0].is, P[1].is, P[2].is};
 				Projection pr = weightPlanes(is, make3(P[0].cs.z, P[1].cs.z, P[2].cs.z), make3(0.0f, 1.0f, 0.0f), make3(0.0f, 0.0f, 1.0f), perspective);
 				for (int q = 0; q < 3; q++) { c.fx[q] = P[q].flat.x; c.fy[q] = P[q].flat.y; }
 				c.start = pr.start; c.dx = pr.dx; c.dy = pr.dy;
 				c.task = ti; c.fan = local; c.canvas = t.canvas; c.flags = perspective ? 0 : 1;
+				emitCommand(f, offset, c, cv, si);
 			}
 			count = 1;
 		}
@@ -93,13 +132,14 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, Comma
 			Projected S[3] = {cameraToScreen(cam, va.cs), cameraToScreen(cam, vb.cs), cameraToScreen(cam, vc.cs)};
 			if (frontFacing(S[0].flat, S[1].flat, S[2].flat)) {
 				if (EMIT) {
-					Command &c = out[count];
+					Command c;
 					const f2 is[3] = {S[0].is, S[1].is, S[2].is};
 					Projection pr = weightPlanes(is, make3(S[0].cs.z, S[1].cs.z, S[2].cs.z), make3(va.subB, vb.subB, vc.subB),
 					                             make3(va.subC, vb.subC, vc.subC), perspective);
 					for (int q = 0; q < 3; q++) { c.fx[q] = S[q].flat.x; c.fy[q] = S[q].flat.y; }
 					c.start = pr.start; c.dx = pr.dx; c.dy = pr.dy;
 					c.task = ti; c.fan = local; c.canvas = t.canvas; c.flags = perspective ? 0 : 1;
+					emitCommand(f, offset + count, c, cv, si);
 				}
 				count++;
 			}
@@ -129,7 +169,7 @@ __device__ int32_t blockExclusiveScan(int32_t value, int32_t &exclusive) {
 
 __global__ void k_setup_count(FrameDev f) {
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
-	int32_t n = (g < f.fanTotal) ? setupTriangle<false>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), nullptr) : 0;
+	int32_t n = (g < f.fanTotal) ? setupTriangle<false>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), 0) : 0;
 	if (g < f.fanTotal) f.fanCount[g] = n;
 	int32_t ex;
 	int32_t total = blockExclusiveScan(n, ex);
@@ -197,45 +237,6 @@ __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_
 	}
 }
 
-struct TileRange { int32_t tx0, tx1, ty0, ty1; bool any; };
-
-DH Rect canvasRect(const CanvasDev &cv) { return rectSides(0, 0, cv.width, cv.height); }
-
-// Pixel region a command can touch: columns of cut(wholeBound, canvas) rounded out to quads,
-// rows of the aligned raster bound (ITriangle2D.cpp:70-84).
-DH TileRange commandTiles(const Command &c, const CanvasDev &cv, int32_t tw, int32_t th) {
-	TileRange r;
-	Rect wb = triangleBound(l2{c.fx[0], c.fy[0]}, l2{c.fx[1], c.fy[1]}, l2{c.fx[2], c.fy[2]});
-	Rect full = canvasRect(cv);
-	r.any = overlaps(wb, full);
-	if (!r.any) { r.tx0 = r.tx1 = r.ty0 = r.ty1 = 0; return r; }
-	Rect clip = cutRect(wb, full);
-	Rect al = alignedRasterBound(wb, full);
-	r.tx0 = clip.l / tw;
-	r.tx1 = (clip.right() - 1) / tw;
-	r.ty0 = al.t / th;
-	r.ty1 = (al.bottom() - 2) / th;
-	if (r.ty1 > cv.tilesY - 1) r.ty1 = cv.tilesY - 1;
-	return r;
-}
-
-// Sets up draw command c for the raster -- its rows and finished shader inputs -- and counts
-// it into every tile it can touch.
-__device__ void binCommand(const FrameDev &f, int32_t c) {
-	const Command &cmd = f.commands[c];
-	const CanvasDev &cv = f.canvases[cmd.canvas];
-	f.triangleRows[c] = triangleRows(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]}, canvasRect(cv));
-	if (f.mode == 0) {
-		const TaskDev &t = f.tasks[cmd.task];
-		const int32_t fp = t.fanPolygon[cmd.fan];
-		f.shaders[c] = triangleShader(t, fp >> 1, fp & 1);
-	}
-	TileRange r = commandTiles(cmd, cv, f.tileW, f.tileH);
-	if (!r.any) return;
-	for (int32_t ty = r.ty0; ty <= r.ty1; ty++)
-		for (int32_t tx = r.tx0; tx <= r.tx1; tx++) atomicAdd(&f.tileCount[cv.tileBase + ty * cv.tilesX + tx], 1);
-}
-
 __global__ void k_setup_emit(FrameDev f) {
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	int32_t n = (g < f.fanTotal) ? f.fanCount[g] : 0;
@@ -252,8 +253,7 @@ __global__ void k_setup_emit(FrameDev f) {
 		atomicOr(&f.counters[2], 1);
 		return;
 	}
-	setupTriangle<true>(f, g, ti, f.commands + offset);
-	for (int32_t k = 0; k < n; k++) binCommand(f, offset + k);
+	setupTriangle<true>(f, g, ti, offset);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -294,9 +294,8 @@ __global__ void k_scan_tiles_add(FrameDev f) {
 __global__ void k_bin_fill(FrameDev f) {
 	int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c >= f.counters[0] || f.counters[2]) return;
-	const Command &cmd = f.commands[c];
-	const CanvasDev &cv = f.canvases[cmd.canvas];
-	TileRange r = commandTiles(cmd, cv, f.tileW, f.tileH);
+	const CanvasDev &cv = f.canvases[f.commands[c].canvas];
+	TileRange r = commandTiles(f.triangleRows[c], cv, f.tileW, f.tileH);
 	if (!r.any) return;
 	for (int32_t ty = r.ty0; ty <= r.ty1; ty++) {
 		for (int32_t tx = r.tx0; tx <= r.tx1; tx++) {
