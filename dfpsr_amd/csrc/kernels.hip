@@ -812,7 +812,7 @@ template __global__ void k_raster<32, 32>(FrameDev, int32_t);
 // is the command alone and the last covering command wins.  Commands run in chunks per tile;
 // each chunk starts from the depths the previous one left.  Work is spread densely: one
 // thread per (command, row) for the exact rows, per (command, covered quad) for the keys and
-// per pixel for the shading of its winner.
+// per (quad, winning command) for the shading.
 //
 // Frames with the Alpha filter (blending reads the colour held at that moment), orthogonal
 // cameras (a lower depth wins) or an odd-height canvas (the missing last row aliases lanes
@@ -861,6 +861,8 @@ template <int TW, int TH> struct DeferLds {
 	RowInterval rows[S::CHUNK * TH];             // chunk command j, tile row r
 	int32_t pairStart[PAIRS];                    // first quad column (pixel x) of each pair in the tile
 	int32_t pairOffset[PAIRS + 1];               // exclusive prefix of quads per pair
+	uint32_t quadWinners[(TW / 2) * (TH / 2)];   // distinct winning chunk commands of a quad, 8 bits each
+	int32_t quadOffset[(TW / 2) * (TH / 2) + 1]; // exclusive prefix of distinct winners per quad
 	uint32_t color[TW * TH];                     // the tile's canvas pixels as they stand
 	float depth[TW * TH];
 	uint8_t dirty[TW * TH];
@@ -911,6 +913,13 @@ __device__ __forceinline__ int32_t groupPrefix(int32_t value, int32_t &before, i
 	return total;
 }
 
+// The chunk command that won a pixel's key, or -1.
+__device__ __forceinline__ int32_t keyWinner(unsigned long long key, bool depthRead) {
+	if (!depthRead) return (int32_t)key - 1;
+	const uint32_t low = (uint32_t)key;
+	return low == 0xFFFFFFFFu ? -1 : (int32_t)(0xFFFFFFFEu - low);
+}
+
 // Index of the segment holding item it in an exclusive prefix of count + 1 entries.
 __device__ __forceinline__ int32_t segmentOf(const int32_t *offset, int32_t count, int32_t it) {
 	int32_t lo = 0, hi = count - 1;
@@ -928,6 +937,8 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	constexpr int32_t NP = TW * TH, RP = TH / 2;
 	constexpr int32_t PAIRS = DeferLds<TW, TH>::PAIRS;
 	constexpr int32_t PAIRS_PER_THREAD = PAIRS / NT > 0 ? PAIRS / NT : 1;
+	constexpr int32_t NQX = TW / 2, NQ = (TW / 2) * (TH / 2);
+	constexpr int32_t QUADS_PER_THREAD = (NQ + NT - 1) / NT;
 	const int32_t n = f.tileCount[tile];
 	const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
 	const CanvasDev cv = f.canvases[ci];
@@ -1025,37 +1036,67 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				}
 			}
 			__syncthreads();
-			// Pass 2: each pixel evaluates its winning command's quad and shades its own lane.
-			for (int32_t p = t; p < NP; p += NT) {
-				const unsigned long long key = s.keys[p];
-				int32_t jw;
-				if (depthRead) {
-					const uint32_t low = (uint32_t)key;
-					jw = low == 0xFFFFFFFFu ? -1 : (int32_t)(0xFFFFFFFEu - low);
-				} else {
-					jw = (int32_t)key - 1;
+			// Winners: the distinct winning commands of each quad, in pixel order.
+			int32_t quadCount[QUADS_PER_THREAD];
+			threadSum = 0;
+			for (int32_t k = 0; k < QUADS_PER_THREAD; k++) {
+				const int32_t qi = t * QUADS_PER_THREAD + k;
+				int32_t count = 0;
+				uint32_t packed = 0;
+				if (qi < NQ) {
+					const int32_t p0 = (qi / NQX) * 2 * TW + (qi % NQX) * 2;
+					for (int sl = 0; sl < 4; sl++) {
+						const int32_t w = keyWinner(s.keys[p0 + (sl >> 1) * TW + (sl & 1)], depthRead);
+						if (w < 0) continue;
+						bool seen = false;
+						for (int r = 0; r < count; r++) seen |= ((packed >> (8 * r)) & 0xFFu) == (uint32_t)w;
+						if (!seen) { packed |= (uint32_t)w << (8 * count); count++; }
+					}
+					s.quadWinners[qi] = packed;
 				}
-				if (jw < 0) continue;
-				const int32_t xl = p % TW, yl = p / TW;
-				const int32_t qx = xl & ~1, qy = yl & ~1;
-				const int32_t sl = (yl & 1) * 2 + (xl & 1);
+				quadCount[k] = count;
+				threadSum += count;
+			}
+			const int32_t shades = groupPrefix<NT>(threadSum, before, s.scan);
+			for (int32_t k = 0; k < QUADS_PER_THREAD; k++) {
+				const int32_t qi = t * QUADS_PER_THREAD + k;
+				if (qi < NQ) s.quadOffset[qi] = before;
+				before += quadCount[k];
+			}
+			if (t == 0) s.quadOffset[NQ] = shades;
+			__syncthreads();
+			// Pass 2: one (quad, winning command) item per thread: the quad's weights and mip level
+			// once, then the lanes the command won.
+			for (int32_t it = t; it < shades; it += NT) {
+				const int32_t qi = segmentOf(s.quadOffset, NQ, it);
+				const int32_t jw = (int32_t)((s.quadWinners[qi] >> (8 * (it - s.quadOffset[qi]))) & 0xFFu);
+				const int32_t qx = (qi % NQX) * 2, qy = (qi / NQX) * 2;
 				QuadEval e;
 				if (!evalQuad(s.tail[jw].plane, s.rows[jw * TH + qy], s.rows[jw * TH + qy + 1], tileX0 + qx, tileY0 + qy, cv.height, e))
 					continue;
-				int l = -1;
-				for (int k = 0; k < 4; k++)
-					if ((!e.clipSides || e.clip[k]) && e.slot[k] == sl) l = k;
-				if (l < 0) continue;
+				uint32_t mask = 0;
+				for (int l = 0; l < 4; l++) {
+					if (e.clipSides && !e.clip[l]) continue;
+					const int32_t sl = e.slot[l];
+					if (keyWinner(s.keys[(qy + (sl >> 1)) * TW + qx + (sl & 1)], depthRead) == jw) mask |= 1u << l;
+				}
+				float r[4], g[4], b[4], a[4];
 				if (colorWrite) {
 					const bool affine = (s.tail[jw].flags & 1) != 0;
 					float wa[4], wb[4], wc[4];
-					for (int k = 0; k < 4; k++) laneWeights(e, k, affine, wa[k], wb[k], wc[k]);
-					float r, g, b, a;
-					shadeLane(s.si[jw], textures, wa, wb, wc, l, r, g, b, a);
-					s.color[p] = packSaturated(r, g, b, a);
+					for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
+					// All four lanes are shaded (every texel index is masked into its level), so the
+					// sixteen texel loads of the quad can be issued together; only won lanes are kept.
+					shadeQuad(s.si[jw], textures, wa, wb, wc, 0xFu, r, g, b, a);
 				}
-				s.depth[p] = pickLane(e.l0, l);
-				s.dirty[p] = 1;
+				for (int l = 0; l < 4; l++) {
+					if (!((mask >> l) & 1u)) continue;
+					const int32_t sl = e.slot[l];
+					const int32_t p = (qy + (sl >> 1)) * TW + qx + (sl & 1);
+					if (colorWrite) s.color[p] = packSaturated(r[l], g[l], b[l], a[l]);
+					s.depth[p] = e.l0[l];
+					s.dirty[p] = 1;
+				}
 			}
 			__syncthreads();
 		}

@@ -92,6 +92,61 @@ DH uint32_t mixColors(uint32_t a, uint32_t b, uint32_t w) {
 }
 
 // texture_sample_bilinear with MIP_INSIDE (textureAPI.h:342-438); highest = HIGHEST_RESOLUTION.
+// The four texel offsets (from t.texels) and 8-bit weights of one bilinear sample.
+struct BilinearSample {
+	uint32_t offset[4];  // upper left, upper right, lower left, lower right
+	uint32_t wx, wy;
+};
+
+DH BilinearSample bilinearSample(const TextureDesc &t, float u, float v, uint32_t mip, bool highest) {
+	uint32_t scaleU = 256u << t.log2w, scaleV = 256u << t.log2h;
+	uint32_t maskX = t.maxWidthAndMask, maskY = t.maxHeightAndMask;
+	uint32_t stride = t.log2w;
+	uint32_t layer = t.startOffset;
+	if (!highest) {
+		scaleU >>= mip; scaleV >>= mip;
+		maskX >>= mip; maskY >>= mip;
+		stride -= mip;
+		layer = layerOffset(t, mip);
+	}
+	uint32_t subX = truncU32((u + 256.0f) * fromU32(scaleU)) - 128u;
+	uint32_t subY = truncU32((v + 256.0f) * fromU32(scaleV)) - 128u;
+	BilinearSample b;
+	b.wx = subX & 255u;
+	b.wy = subY & 255u;
+	uint32_t left = subX >> 8, top = subY >> 8;
+	uint32_t right = (left + 1u) & maskX, bottom = (top + 1u) & maskY;
+	left &= maskX;
+	top &= maskY;
+	b.offset[0] = layer + ((top << stride) | left);
+	b.offset[1] = layer + ((top << stride) | right);
+	b.offset[2] = layer + ((bottom << stride) | left);
+	b.offset[3] = layer + ((bottom << stride) | right);
+	return b;
+}
+
+// Texels live in global memory; loading them through a global pointer keeps the loads off the
+// LDS counter, so a quad's sixteen loads can all be in flight at once.
+__device__ __forceinline__ uint32_t loadTexel(const uint32_t *texels, uint32_t offset) {
+	return ((const __attribute__((address_space(1))) uint32_t *)texels)[offset];
+}
+
+DH uint32_t mixSample(const BilinearSample &b, uint32_t ul, uint32_t ur, uint32_t bl, uint32_t br) {
+	return mixColors(mixColors(ul, ur, b.wx), mixColors(bl, br, b.wx), b.wy);
+}
+
+// The bilinear samples of the four lanes of a quad from one texture level: all sixteen texels
+// are loaded before any is mixed.
+__device__ __forceinline__ void sampleQuad(const TextureDesc &t, const float *u, const float *v, uint32_t mip, bool highest,
+                                           uint32_t *out) {
+	BilinearSample b[4];
+	for (int l = 0; l < 4; l++) b[l] = bilinearSample(t, u[l], v[l], mip, highest);
+	uint32_t px[16];
+	for (int l = 0; l < 4; l++)
+		for (int k = 0; k < 4; k++) px[l * 4 + k] = loadTexel(t.texels, b[l].offset[k]);
+	for (int l = 0; l < 4; l++) out[l] = mixSample(b[l], px[l * 4], px[l * 4 + 1], px[l * 4 + 2], px[l * 4 + 3]);
+}
+
 DH uint32_t sampleBilinear(const TextureDesc &t, float u, float v, uint32_t mip, bool highest) {
 	uint32_t scaleU = 256u << t.log2w, scaleV = 256u << t.log2h;
 	uint32_t maskX = t.maxWidthAndMask, maskY = t.maxHeightAndMask;
@@ -182,6 +237,12 @@ DH void shadeQuad(const ShaderInputs &s, const TextureDesc *textures, const floa
 		float u[4], v[4];
 		for (int l = 0; l < 4; l++) { u[l] = interp(s.u1, wa[l], wb[l], wc[l]); v[l] = interp(s.v1, wa[l], wb[l], wc[l]); }
 		uint32_t mip = mipLevelIndex(t, u[0], u[1], u[2], v[0], v[1], v[2]);
+		if (mask == 0xFu) {
+			uint32_t texel[4];
+			sampleQuad(t, u, v, mip, false, texel);
+			for (int l = 0; l < 4; l++) unpackColor(texel[l], r[l], g[l], b[l], a[l]);
+			return;
+		}
 		for (int l = 0; l < 4; l++)
 			if ((mask >> l) & 1u) unpackColor(sampleBilinear(t, u[l], v[l], mip, false), r[l], g[l], b[l], a[l]);
 		return;
@@ -211,10 +272,12 @@ DH void shadeQuad(const ShaderInputs &s, const TextureDesc *textures, const floa
 		float u[4], v[4];
 		for (int l = 0; l < 4; l++) { u[l] = interp(s.u1, wa[l], wb[l], wc[l]); v[l] = interp(s.v1, wa[l], wb[l], wc[l]); }
 		uint32_t mip = mipLevelIndex(t, u[0], u[1], u[2], v[0], v[1], v[2]);
+		uint32_t texel[4];
+		if (mask == 0xFu) sampleQuad(t, u, v, mip, false, texel);
 		for (int l = 0; l < 4; l++) {
 			if (!((mask >> l) & 1u)) continue;
 			float sr, sg, sb, sa;
-			unpackColor(sampleBilinear(t, u[l], v[l], mip, false), sr, sg, sb, sa);
+			unpackColor(mask == 0xFu ? texel[l] : sampleBilinear(t, u[l], v[l], mip, false), sr, sg, sb, sa);
 			r[l] = r[l] * sr; g[l] = g[l] * sg; b[l] = b[l] * sb; a[l] = a[l] * sa;
 		}
 	}
