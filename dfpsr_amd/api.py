@@ -21,10 +21,11 @@ class Texture:
 
     def __init__(self, handle):
         self._h = handle
+        self._destroy = lib().dfpsr_hip_texture_destroy
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().dfpsr_hip_texture_destroy(self._h)
+            self._destroy(self._h)  # held by the object: module globals may be gone at exit
             self._h = None
 
     @property
@@ -61,11 +62,12 @@ def texture_generatePyramid(texture: Texture) -> None:
 class Model:
     def __init__(self, handle):
         self._h = handle
+        self._destroy = lib().dfpsr_hip_model_destroy
         self._keep = []  # textures referenced by parts must outlive the model's frames
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().dfpsr_hip_model_destroy(self._h)
+            self._destroy(self._h)  # held by the object: module globals may be gone at exit
             self._h = None
 
     @property
@@ -116,11 +118,12 @@ def camera_create(perspective: bool, location, image_width: float, image_height:
 class Renderer:
     def __init__(self, handle):
         self._h = handle
+        self._destroy = lib().dfpsr_hip_renderer_destroy
         self._images = None
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().dfpsr_hip_renderer_destroy(self._h)
+            self._destroy(self._h)  # held by the object: module globals may be gone at exit
             self._h = None
 
     @property
@@ -279,10 +282,11 @@ class Batch:
         h = C.c_void_p()
         check(lib().dfpsr_hip_batch_create(C.byref(h)))
         self._h = h
+        self._destroy = lib().dfpsr_hip_batch_destroy
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().dfpsr_hip_batch_destroy(self._h)
+            self._destroy(self._h)  # held by the object: module globals may be gone at exit
             self._h = None
 
     def render(self, jobs, t: Tuning) -> None:
@@ -380,10 +384,11 @@ class DiffRenderer:
         h = C.c_void_p()
         check(lib().dfpsr_hip_diff_create(C.byref(h)))
         self._h = h
+        self._destroy = lib().dfpsr_hip_diff_destroy
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().dfpsr_hip_diff_destroy(self._h)
+            self._destroy(self._h)  # held by the object: module globals may be gone at exit
             self._h = None
 
     def forward(self, model: Model, views, texture, multipliers=None, t: Optional[Tuning] = None) -> None:

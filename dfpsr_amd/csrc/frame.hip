@@ -158,7 +158,7 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 		return 0;
 	}
 	const int32_t persistent = o.persistentPerCU > 0 ? 1 : 0;
-	const int32_t grid = persistent ? deviceCUs_ * o.persistentPerCU : XCDS * blocksOf(tileTotal, XCDS);
+	const int32_t grid = persistent ? deviceCUs_ * o.persistentPerCU : xcdGrid(tileTotal);
 	if (o.mode == 1) {
 		hipLaunchKernelGGL(k_raster_diff, dim3(grid), dim3(64), 0, stream, f, persistent);
 	} else if (deferredRaster(o)) {

@@ -750,31 +750,29 @@ __device__ __forceinline__ void reportCounters(const FrameDev &f) {
 	if (blockIdx.x == 0 && threadIdx.x < 4) f.hostCounters[threadIdx.x] = f.counters[threadIdx.x];
 }
 
-// Runs body(tile) over the ordered queue of non-empty tiles, XCD by XCD.  Without
-// persistence the grid has 8 * ceil(tiles / 8) workgroups and workgroup b takes slot
-// b / 8 of its XCD's range.  Persistent workgroups pull slots from their own XCD's head and
-// then from the other XCDs' heads; every head ends past its range, so all of them exit.
+// Runs body(tile) over the ordered queue of non-empty tiles (every tile when the raster
+// fills the canvases), XCD by XCD.  Without persistence the grid has xcdGrid(tiles)
+// workgroups and workgroup b takes the (b / 8)-th slot of XCD b % 8.  Persistent workgroups
+// pull slots from their own XCD's head and then from the other XCDs' heads; a head only
+// grows and every XCD's slots end below the queue length, so all of them exit.
 template <typename Body>
 __device__ void forEachQueuedTile(const FrameDev &f, int32_t persistent, int32_t *slotLds, Body body) {
 	const int32_t queued = f.clearInRaster ? f.tileTotal : f.counters[3];
 	const int32_t x = (int32_t)(blockIdx.x % XCDS);
 	if (!persistent) {
-		int32_t begin, end;
-		xcdRange(queued, x, begin, end);
-		const int32_t slot = begin + (int32_t)(blockIdx.x / XCDS);
-		if (slot < end) body(f.clearInRaster ? slot : f.tileQueue[slot]);
+		const int32_t slot = xcdSlot(x, (int32_t)(blockIdx.x / XCDS));
+		if (slot < queued) body(f.clearInRaster ? slot : f.tileQueue[slot]);
 		return;
 	}
+	// Persistent workgroups walk their own XCD's share first, then help the others.
 	for (int32_t r = 0; r < XCDS; r++) {
 		const int32_t xr = (x + r) % XCDS;
-		int32_t begin, end;
-		xcdRange(queued, xr, begin, end);
 		while (true) {
-			if (threadIdx.x == 0) *slotLds = begin + atomicAdd(&f.counters[16 + xr], 1);
+			if (threadIdx.x == 0) *slotLds = xcdSlot(xr, atomicAdd(&f.counters[16 + xr], 1));
 			__syncthreads();
 			const int32_t slot = *slotLds;
 			__syncthreads();
-			if (slot >= end) break;
+			if (slot >= queued) break;
 			body(f.clearInRaster ? slot : f.tileQueue[slot]);
 			__syncthreads();
 		}

@@ -134,13 +134,24 @@ constexpr int32_t SCAN_BLOCK = 256;
 constexpr int32_t COUNTERS = 32;
 
 // Workgroups are dispatched to gfx950's eight XCDs round-robin (workgroup b runs on XCD
-// b % 8, each XCD with its own L2).  The raster gives XCD x the x-th eighth of the ordered
-// tile queue, so neighbouring tiles -- and the texels they sample -- share one L2.
+// b % 8, each XCD with its own L2).  The ordered tile queue is cut into segments of
+// XCD_SEGMENT consecutive slots dealt to the XCDs in turn: XCD x takes segments x, x + 8,
+// x + 16, ...  Neighbouring tiles of a segment share commands and texels in one L2, and every
+// XCD gets a share of every part of the canvas, so a band of costly tiles (a horizon, say)
+// is spread over all eight instead of landing on one.
 constexpr int32_t XCDS = 8;
-DH void xcdRange(int32_t queued, int32_t x, int32_t &begin, int32_t &end) {
-	const int32_t per = (queued + XCDS - 1) / XCDS;
-	begin = x * per < queued ? x * per : queued;
-	end = begin + per < queued ? begin + per : queued;
+#ifndef DFPSR_XCD_SEGMENT
+#define DFPSR_XCD_SEGMENT 4
+#endif
+constexpr int32_t XCD_SEGMENT = DFPSR_XCD_SEGMENT;
+// Queue slot of the i-th tile of XCD x.  Increasing in i: the first slot >= queued ends x's share.
+DH int32_t xcdSlot(int32_t x, int32_t i) {
+	return ((i / XCD_SEGMENT) * XCDS + x) * XCD_SEGMENT + i % XCD_SEGMENT;
+}
+// Workgroups a one-tile-per-workgroup launch needs so that every slot below tileTotal has one.
+DH int32_t xcdGrid(int32_t tileTotal) {
+	const int32_t round = XCDS * XCD_SEGMENT;
+	return (tileTotal + round - 1) / round * round;
 }
 
 }  // namespace dfpsr
