@@ -38,7 +38,7 @@ class Camera(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("tile_config", C.c_int32), ("persistent_per_cu", C.c_int32), ("use_graph", C.c_int32),
                 ("clear", C.c_int32), ("clear_color", C.c_uint32), ("clear_depth", C.c_float),
-                ("raster_path", C.c_int32)]
+                ("raster_path", C.c_int32), ("binning", C.c_int32)]
 
 
 class Job(C.Structure):

@@ -265,11 +265,12 @@ def render_scene(scene, renderer: Optional[Renderer] = None, gpu_scene: Optional
 # ---------------------------------------------------------------------------------------
 
 def tuning(tile_config: int = 0, persistent_per_cu: int = 0, use_graph: bool = False, clear: bool = False,
-           clear_color: int = 0, clear_depth: float = 0.0, raster_path: int = 0) -> Tuning:
+           clear_color: int = 0, clear_depth: float = 0.0, raster_path: int = 0, binning: int = 0) -> Tuning:
     t = Tuning()
     t.tile_config, t.persistent_per_cu, t.use_graph = tile_config, persistent_per_cu, int(use_graph)
     t.clear, t.clear_color, t.clear_depth = int(clear), clear_color, clear_depth
     t.raster_path = raster_path
+    t.binning = binning
     return t
 
 

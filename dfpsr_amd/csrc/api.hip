@@ -255,6 +255,7 @@ static RenderOptions optionsFrom(const dfpsr_hip_tuning *t) {
 	o.clearColor = t->clear_color;
 	o.clearDepth = t->clear_depth;
 	o.rasterPath = t->raster_path;
+	o.binning = t->binning;
 	return o;
 }
 
@@ -263,6 +264,7 @@ static int checkTuning(const dfpsr_hip_tuning *t) {
 	if (t->tile_config < 0 || t->tile_config >= TILE_CONFIGS) return fail(DFPSR_HIP_ERROR_ARGUMENT, "tile_config must be 0, 1 or 2");
 	if (t->persistent_per_cu < 0 || t->persistent_per_cu > 64) return fail(DFPSR_HIP_ERROR_ARGUMENT, "persistent_per_cu must be 0..64");
 	if (t->raster_path < 0 || t->raster_path > 1) return fail(DFPSR_HIP_ERROR_ARGUMENT, "raster_path must be 0 or 1");
+	if (t->binning < 0 || t->binning > 1) return fail(DFPSR_HIP_ERROR_ARGUMENT, "binning must be 0 or 1");
 	return DFPSR_HIP_OK;
 }
 

@@ -67,6 +67,7 @@ struct RenderOptions {
 	float clearDepth = 0.0f;
 	int32_t mode = 0;                // 0: the reference's RGBA8 shaders; 1: the differentiable float shader
 	int32_t rasterPath = 0;          // 0: visibility raster where the frame allows it; 1: always in order
+	int32_t binning = 0;             // 0: direct per-tile bins while they fit; 1: counted, scanned and filled lists
 };
 
 class Frame {
@@ -118,6 +119,10 @@ private:
 	bool orthogonalAny_ = false;  // a task with an orthogonal camera
 	bool oddCanvas_ = false;      // a canvas of odd height
 	int32_t commandCapacity_ = 0, pairCapacity_ = 0;
+	// Entries per tile of the direct bins; grows to the longest list seen.  Bins are used while
+	// tiles x binCapacity_ stays within BIN_ENTRY_LIMIT, the scanned lists otherwise.
+	int32_t binCapacity_ = 64;
+	static constexpr int64_t BIN_ENTRY_LIMIT = (int64_t)1 << 25;
 	// All per-frame tables in one upload, skipped when identical to the previous frame's.
 	std::vector<uint8_t> tables_, uploadedTables_;
 	DevBuf dTables_;

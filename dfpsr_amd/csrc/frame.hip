@@ -8,6 +8,7 @@
 // hipGraph and replayed while the frame's tables and buffers stay the same.
 #include "frame.h"
 
+#include <algorithm>
 #include <cstring>
 
 namespace dfpsr {
@@ -146,11 +147,13 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, fanBlocks, f.counters + 0);
 	hipLaunchKernelGGL(k_setup_emit, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
 	if (events) HIP_TRY(hipEventRecord(events_[2], stream));
+	if (f.binCapacity == 0) {
 	hipLaunchKernelGGL(k_scan_tiles_local, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
 	hipLaunchKernelGGL(k_scan_block_sums64, dim3(1), dim3(SCAN_BLOCK), 0, stream, (long long *)f.blockSums, tileBlocks,
 	                   f.counters + 1, f.counters + 3);
 	hipLaunchKernelGGL(k_scan_tiles_add, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
 	hipLaunchKernelGGL(k_bin_fill, dim3(blocksOf(f.commandCapacity, 256)), dim3(256), 0, stream, f);
+	}
 	if (events) HIP_TRY(hipEventRecord(events_[3], stream));
 	}
 	if (!(parts & 2)) {
@@ -238,10 +241,12 @@ int Frame::run(const RenderOptions &o) {
 		return fail(-4, "device allocation failed");
 
 	for (int pass = 1; pass <= 3; pass++) {
+		const bool direct = o.binning == 0 && (int64_t)tileTotal * binCapacity_ <= BIN_ENTRY_LIMIT;
+		const size_t listEntries = direct ? (size_t)tileTotal * binCapacity_ : (size_t)pairCapacity_;
 		if (dCommands_.ensure((size_t)commandCapacity_ * sizeof(Command)) != hipSuccess ||
 		    dShaders_.ensure((size_t)commandCapacity_ * sizeof(ShaderInputs)) != hipSuccess ||
 		    dTriangleRows_.ensure((size_t)commandCapacity_ * sizeof(TriangleRows)) != hipSuccess ||
-		    dTileList_.ensure((size_t)pairCapacity_ * 4) != hipSuccess)
+		    dTileList_.ensure(listEntries * 4) != hipSuccess)
 			return fail(-4, "device allocation failed");
 		FrameDev f;
 		std::memset(&f, 0, sizeof(f));
@@ -274,6 +279,7 @@ int Frame::run(const RenderOptions &o) {
 		f.tileQueue = dTileQueue_.as<int32_t>();
 		f.canvasCommandBase = dCanvasBase_.as<int32_t>();
 		f.pairCapacity = pairCapacity_;
+		f.binCapacity = direct ? binCapacity_ : 0;
 		f.tileW = tileConfigW(tileConfig_);
 		f.tileH = tileConfigH(tileConfig_);
 		f.alphaAny = alphaAny_ ? 1 : 0;
@@ -357,7 +363,15 @@ int Frame::run(const RenderOptions &o) {
 			return 0;
 		}
 		if (overflow & 1) commandCapacity_ = hostCounters_[0] + 1024;
-		if (overflow & 2) pairCapacity_ = hostCounters_[1] + 1024;
+		if (overflow & 2) {
+			// Tile pairs no longer fit: the packed lists grow to the frame's pair count, the
+			// direct bins to the longest tile list with a quarter to spare.
+			pairCapacity_ = hostCounters_[1] + 1024;
+			if (direct) {
+				const int64_t longest = hostCounters_[6];
+				binCapacity_ = (int32_t)std::min<int64_t>(((longest + longest / 4 + 15) / 16) * 16, BIN_ENTRY_LIMIT);
+			}
+		}
 		if ((overflow & 1) && pairCapacity_ < 4 * commandCapacity_) pairCapacity_ = 4 * commandCapacity_;
 		dropGraph();
 	}

@@ -32,12 +32,17 @@ __device__ void projectPoint(const FrameDev &f, int32_t i) {
 // ---------------------------------------------------------------------------------------
 DH Rect canvasRect(const CanvasDev &cv) { return rectSides(0, 0, cv.width, cv.height); }
 
+// (command, tile) pairs and newly used tiles counted by one thread of the setup kernel.
+struct BinTally { int32_t pairs, tiles; };
+
 // Writes draw command `index` with everything the raster needs of it -- the command, its row
-// setup and (for the RGBA8 shaders) its finished shader inputs -- and counts it into every tile
+// setup and (for the RGBA8 shaders) its finished shader inputs -- and enters it into every tile
 // it can touch: the columns of cut(wholeBound, canvas), the rows of the aligned raster bound
-// (ITriangle2D.cpp:70-84).
+// (ITriangle2D.cpp:70-84).  With direct bins the index goes into the tile's list at once (in
+// any order: the raster sorts each list); otherwise only the tile's count grows here and the
+// list is filled after the counts are scanned.
 __device__ void emitCommand(const FrameDev &f, int32_t index, const Command &cmd, const CanvasDev &cv,
-                            const ShaderInputs &si) {
+                            const ShaderInputs &si, BinTally &tally) {
 	f.commands[index] = cmd;
 	const TriangleRows tr = triangleRows(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]}, canvasRect(cv));
 	f.triangleRows[index] = tr;
@@ -47,8 +52,24 @@ __device__ void emitCommand(const FrameDev &f, int32_t index, const Command &cmd
 	const int32_t ty0 = tr.alT / f.tileH;
 	int32_t ty1 = (tr.alB - 2) / f.tileH;
 	if (ty1 > cv.tilesY - 1) ty1 = cv.tilesY - 1;
-	for (int32_t ty = ty0; ty <= ty1; ty++)
-		for (int32_t tx = tx0; tx <= tx1; tx++) atomicAdd(&f.tileCount[cv.tileBase + ty * cv.tilesX + tx], 1);
+	for (int32_t ty = ty0; ty <= ty1; ty++) {
+		for (int32_t tx = tx0; tx <= tx1; tx++) {
+			const int32_t t = cv.tileBase + ty * cv.tilesX + tx;
+			if (f.binCapacity > 0) {
+				const int32_t slot = atomicAdd(&f.tileCount[t], 1);
+				if (slot < f.binCapacity) {
+					f.tileList[(size_t)t * f.binCapacity + slot] = index;
+				} else {
+					atomicOr(&f.counters[2], 2);
+					atomicMax(&f.counters[6], slot + 1);
+				}
+				tally.pairs++;
+				tally.tiles += slot == 0 ? 1 : 0;
+			} else {
+				atomicAdd(&f.tileCount[t], 1);
+			}
+		}
+	}
 }
 
 // The tiles a command can touch (as counted by emitCommand).
@@ -90,7 +111,7 @@ __device__ ShaderInputs triangleShader(const TaskDev &t, int32_t poly, int32_t k
 }
 
 template <bool EMIT>
-__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset) {
+__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset, BinTally &tally) {
 	const TaskDev &t = f.tasks[ti];
 	const CanvasDev &cv = f.canvases[t.canvas];
 	if (cv.color == nullptr && cv.depth == nullptr) return 0;
@@ -119,7 +140,7 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 				for (int q = 0; q < 3; q++) { c.fx[q] = P[q].flat.x; c.fy[q] = P[q].flat.y; }
 				c.start = pr.start; c.dx = pr.dx; c.dy = pr.dy;
 				c.task = ti; c.fan = local; c.canvas = t.canvas; c.flags = perspective ? 0 : 1;
-				emitCommand(f, offset, c, cv, si);
+				emitCommand(f, offset, c, cv, si, tally);
 			}
 			count = 1;
 		}
@@ -139,7 +160,7 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 					for (int q = 0; q < 3; q++) { c.fx[q] = S[q].flat.x; c.fy[q] = S[q].flat.y; }
 					c.start = pr.start; c.dx = pr.dx; c.dy = pr.dy;
 					c.task = ti; c.fan = local; c.canvas = t.canvas; c.flags = perspective ? 0 : 1;
-					emitCommand(f, offset + count, c, cv, si);
+					emitCommand(f, offset + count, c, cv, si, tally);
 				}
 				count++;
 			}
@@ -169,7 +190,8 @@ __device__ int32_t blockExclusiveScan(int32_t value, int32_t &exclusive) {
 
 __global__ void k_setup_count(FrameDev f) {
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
-	int32_t n = (g < f.fanTotal) ? setupTriangle<false>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), 0) : 0;
+	BinTally unused{0, 0};
+	int32_t n = (g < f.fanTotal) ? setupTriangle<false>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), 0, unused) : 0;
 	if (g < f.fanTotal) f.fanCount[g] = n;
 	int32_t ex;
 	int32_t total = blockExclusiveScan(n, ex);
@@ -238,22 +260,33 @@ __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_
 }
 
 __global__ void k_setup_emit(FrameDev f) {
+	__shared__ int32_t blockTally[2];
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	int32_t n = (g < f.fanTotal) ? f.fanCount[g] : 0;
 	int32_t ex;
 	blockExclusiveScan(n, ex);
-	if (g >= f.fanTotal) return;
-	const int32_t offset = f.blockSums[blockIdx.x] + ex;
-	const int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
-	// The commands of a canvas's tasks are consecutive, so the offset at the first fan
-	// triangle of each task bounds its canvas's first command: one atomic per task.
-	if (g == f.tasks[ti].fanBase) atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
-	if (n == 0) return;
-	if (offset + n > f.commandCapacity) {
-		atomicOr(&f.counters[2], 1);
-		return;
+	if (threadIdx.x < 2) blockTally[threadIdx.x] = 0;
+	BinTally tally{0, 0};
+	if (g < f.fanTotal) {
+		const int32_t offset = f.blockSums[blockIdx.x] + ex;
+		const int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
+		// The commands of a canvas's tasks are consecutive, so the offset at the first fan
+		// triangle of each task bounds its canvas's first command: one atomic per task.
+		if (g == f.tasks[ti].fanBase) atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
+		if (n > 0) {
+			if (offset + n > f.commandCapacity) atomicOr(&f.counters[2], 1);
+			else setupTriangle<true>(f, g, ti, offset, tally);
+		}
 	}
-	setupTriangle<true>(f, g, ti, offset);
+	if (f.binCapacity > 0) {
+		// Direct bins: the pair and used-tile totals, one global atomic per block.
+		__syncthreads();
+		if (tally.pairs) atomicAdd(&blockTally[0], tally.pairs);
+		if (tally.tiles) atomicAdd(&blockTally[1], tally.tiles);
+		__syncthreads();
+		if (threadIdx.x == 0 && blockTally[0]) atomicAdd(&f.counters[1], blockTally[0]);
+		if (threadIdx.x == 1 && blockTally[1]) atomicAdd(&f.counters[3], blockTally[1]);
+	}
 }
 
 // ---------------------------------------------------------------------------------------
@@ -684,7 +717,7 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 	const int32_t lane = threadIdx.x;
 	const int32_t px = tileX0 + 2 * (lane % (TW / 2));
 	const int32_t py = tileY0 + 2 * (lane / (TW / 2));
-	const int32_t *src = f.tileList + f.tileOffset[tile];
+	const int32_t *src = tileListOf(f, tile);
 
 	QuadPixels<MODE> q;
 	q.colorDirty = q.depthDirty = 0;
@@ -747,21 +780,27 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 // The raster kernels start after every counter is final: workgroup 0 copies them to the
 // frame's pinned host memory, which the host reads after the stream completes.
 __device__ __forceinline__ void reportCounters(const FrameDev &f) {
-	if (blockIdx.x == 0 && threadIdx.x < 4) f.hostCounters[threadIdx.x] = f.counters[threadIdx.x];
+	if (blockIdx.x == 0 && threadIdx.x < 8) f.hostCounters[threadIdx.x] = f.counters[threadIdx.x];
 }
 
-// Runs body(tile) over the ordered queue of non-empty tiles (every tile when the raster
-// fills the canvases), XCD by XCD.  Without persistence the grid has xcdGrid(tiles)
-// workgroups and workgroup b takes the (b / 8)-th slot of XCD b % 8.  Persistent workgroups
-// pull slots from their own XCD's head and then from the other XCDs' heads; a head only
-// grows and every XCD's slots end below the queue length, so all of them exit.
+// Runs body(tile) over the tiles with commands -- every tile when the raster fills the
+// canvases -- XCD by XCD.  The slots are the ordered queue of non-empty tiles after the scan,
+// or the tiles themselves with direct bins (empty ones skipped).  Without persistence the grid
+// has xcdGrid(tiles) workgroups and workgroup b takes the (b / 8)-th slot of XCD b % 8.
+// Persistent workgroups pull slots from their own XCD's head and then from the other XCDs'
+// heads; a head only grows and every XCD's slots end below the slot count, so all of them exit.
 template <typename Body>
 __device__ void forEachQueuedTile(const FrameDev &f, int32_t persistent, int32_t *slotLds, Body body) {
-	const int32_t queued = f.clearInRaster ? f.tileTotal : f.counters[3];
+	const bool allTiles = f.clearInRaster || f.binCapacity > 0;
+	const int32_t queued = allTiles ? f.tileTotal : f.counters[3];
 	const int32_t x = (int32_t)(blockIdx.x % XCDS);
+	auto visit = [&](int32_t slot) {
+		const int32_t tile = allTiles ? slot : f.tileQueue[slot];
+		if (f.clearInRaster || f.tileCount[tile] > 0) body(tile);
+	};
 	if (!persistent) {
 		const int32_t slot = xcdSlot(x, (int32_t)(blockIdx.x / XCDS));
-		if (slot < queued) body(f.clearInRaster ? slot : f.tileQueue[slot]);
+		if (slot < queued) visit(slot);
 		return;
 	}
 	// Persistent workgroups walk their own XCD's share first, then help the others.
@@ -773,7 +812,7 @@ __device__ void forEachQueuedTile(const FrameDev &f, int32_t persistent, int32_t
 			const int32_t slot = *slotLds;
 			__syncthreads();
 			if (slot >= queued) break;
-			body(f.clearInRaster ? slot : f.tileQueue[slot]);
+			visit(slot);
 			__syncthreads();
 		}
 	}
@@ -944,7 +983,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	const int32_t tileX0 = (local % cv.tilesX) * TW;
 	const int32_t tileY0 = (local / cv.tilesX) * TH;
 	const int32_t t = threadIdx.x;
-	const int32_t *src = f.tileList + f.tileOffset[tile];
+	const int32_t *src = tileListOf(f, tile);
 	const bool colorWrite = cv.color != nullptr;
 	const bool depthRead = cv.depth != nullptr;
 

@@ -76,17 +76,22 @@ struct FrameDev {
 	ShaderInputs *shaders;        // per command: the finished shader inputs of its source triangle
 	TriangleRows *triangleRows;   // per command: edges and bounds of its rows
 	int32_t commandCapacity;
-	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flags, [3] queued tiles,
-	                              // [4] raster head, [5] backward head, [16..23] per-XCD raster heads
+	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flags, [3] tiles with commands,
+	                              // [4] raster head, [5] backward head, [6] longest tile list when a
+	                              // bin overflowed, [16..23] per-XCD raster heads
 	int32_t *tileCount;           // tileTotal + 1
+	// Tile lists.  With binCapacity > 0 tile t owns tileList[t * binCapacity, ...) and the setup
+	// kernel appends to it directly; otherwise the lists are packed at tileOffset (an exclusive
+	// scan of tileCount) by the scan and fill kernels, pairCapacity entries in all.
+	int32_t binCapacity;
 	int32_t *tileOffset;          // tileTotal + 1 (exclusive scan of tileCount)
 	int32_t *tileFill;            // tileTotal
-	int32_t *tileList;            // pair capacity
+	int32_t *tileList;
 	int32_t pairCapacity;
 	int32_t *tileQueue;           // tiles with at least one command, in tile order (counters[3] of them)
 	int32_t *canvasCommandBase;   // per canvas: index of its first draw command (visibility ids count from it)
 	int32_t *queueHead;           // next queue slot for persistent raster workgroups
-	int32_t *hostCounters;        // pinned host copy of counters[0..3], written by the raster kernel
+	int32_t *hostCounters;        // pinned host copy of counters[0..7], written by the raster kernel
 	int32_t tileW, tileH;
 	int32_t alphaAny;             // any task uses the Alpha filter: tiles read colour first
 	// The visibility raster fills the canvases itself (image_fill) instead of a separate clear:
@@ -103,6 +108,11 @@ struct FrameDev {
 	float *gradTexture;
 	float *gradMultipliers;
 };
+
+// First entry of tile t's command list.
+DH const int32_t *tileListOf(const FrameDev &f, int32_t tile) {
+	return f.binCapacity > 0 ? f.tileList + (size_t)tile * f.binCapacity : f.tileList + f.tileOffset[tile];
+}
 
 DH int32_t findSegment(const int32_t *inclusiveEnd, int32_t count, int32_t index) {
 	int32_t lo = 0, hi = count - 1;

@@ -155,8 +155,10 @@ int dfpsr_hip_renderer_timing(const dfpsr_hip_renderer *renderer, float out[5]);
  * clear_depth first (the caller's image_fill).  raster_path = 0 draws frames whose commands are
  * all Solid and perspective, on canvases of even height, with the visibility raster (each
  * pixel's winning command is found with atomic max over (depth, command) keys and shaded
- * once); raster_path = 1 always walks every tile's commands in order.  Results never depend
- * on these settings. */
+ * once); raster_path = 1 always walks every tile's commands in order.  binning = 0 lets the
+ * setup kernel append each command to fixed-size per-tile lists (grown when a tile overflows
+ * them, while they stay within a memory bound); binning = 1 always counts, scans and then fills
+ * packed lists.  Results never depend on these settings. */
 typedef struct {
 	int32_t tile_config;
 	int32_t persistent_per_cu;
@@ -165,6 +167,7 @@ typedef struct {
 	uint32_t clear_color;
 	float clear_depth;
 	int32_t raster_path;
+	int32_t binning;
 } dfpsr_hip_tuning;
 
 /* Tuning used by renderer_end (default: 16x16 tiles, one workgroup per tile, no graph). */

@@ -180,15 +180,41 @@ PATH_SCENES = [even_height(s) for s in fixture_scenes.fixture_scenes()] + [S.cub
 @pytest.mark.parametrize("scene", PATH_SCENES, ids=lambda s: s.name)
 def test_visibility_raster_equals_in_order_raster(api, scene):
     """The visibility raster (atomic-max winner per pixel, shaded once) gives bit-identical
-    canvases to the in-order walk, for every tile shape, and both meet the oracle's bar."""
+    canvases to the in-order walk, for every tile shape and both ways of building the tile
+    lists, and all meet the oracle's bar."""
     print(f"[gpu] raster paths {scene.name} {scene.width}x{scene.height}", flush=True)
     results = []
     for tile_config in (0, 1, 2):
         for path in (0, 1):
-            r = api.renderer_create()
-            api.renderer_setTuning(r, api.tuning(tile_config=tile_config, raster_path=path))
-            results.append(api.render_scene(scene, r))
+            for binning in (0, 1):
+                r = api.renderer_create()
+                api.renderer_setTuning(r, api.tuning(tile_config=tile_config, raster_path=path, binning=binning))
+                results.append(api.render_scene(scene, r))
     for c, d in results[1:]:
         assert np.array_equal(c, results[0][0]) and np.array_equal(d.view(np.uint32), results[0][1].view(np.uint32))
     assert_parity(scene, results[0], H.oracle_render(scene), exact_color=scene.name.endswith("_id") or
                   scene.name.endswith("_id_even"))
+
+
+def test_direct_bins_grow_to_the_longest_tile_list(api):
+    """Tiles holding more commands than the direct bins' starting size (64) make the first frame
+    overflow them: the renderer grows the bins to the longest list and draws the frame again
+    (two passes); the next frame fits in one.  Canvases and pair counts equal those of the
+    scanned lists, and the oracle's bar holds."""
+    sc = S.cube_soup(width=48, height=32, triangles=900, seed=5, spread=0.3, cam_distance=2.0, name="dense_bins")
+    print(f"[gpu] dense bins {sc.width}x{sc.height}", flush=True)
+    scanned_renderer = api.renderer_create()
+    api.renderer_setTuning(scanned_renderer, api.tuning(binning=1))
+    scanned = api.render_scene(sc, scanned_renderer)
+    scanned_stats = scanned_renderer.stats()
+    r = api.renderer_create()
+    api.renderer_setTuning(r, api.tuning(binning=0))
+    first = api.render_scene(sc, r)
+    first_stats = r.stats()
+    second = api.render_scene(sc, r)
+    second_stats = r.stats()
+    assert first_stats[3] == 2 and second_stats[3] == 1
+    for got in (first, second):
+        assert np.array_equal(got[0], scanned[0]) and np.array_equal(got[1].view(np.uint32), scanned[1].view(np.uint32))
+    assert list(first_stats[:3]) == list(scanned_stats[:3]) == list(second_stats[:3])
+    assert_parity(sc, first, H.oracle_render(sc))
