@@ -52,22 +52,34 @@ __device__ void emitCommand(const FrameDev &f, int32_t index, const Command &cmd
 	const int32_t ty0 = tr.alT / f.tileH;
 	int32_t ty1 = (tr.alB - 2) / f.tileH;
 	if (ty1 > cv.tilesY - 1) ty1 = cv.tilesY - 1;
-	for (int32_t ty = ty0; ty <= ty1; ty++) {
-		for (int32_t tx = tx0; tx <= tx1; tx++) {
-			const int32_t t = cv.tileBase + ty * cv.tilesX + tx;
-			if (f.binCapacity > 0) {
-				const int32_t slot = atomicAdd(&f.tileCount[t], 1);
-				if (slot < f.binCapacity) {
-					f.tileList[(size_t)t * f.binCapacity + slot] = index;
-				} else {
-					atomicOr(&f.counters[2], 2);
-					atomicMax(&f.counters[6], slot + 1);
-				}
-				tally.pairs++;
-				tally.tiles += slot == 0 ? 1 : 0;
+	if (f.binCapacity == 0) {
+		for (int32_t ty = ty0; ty <= ty1; ty++)
+			for (int32_t tx = tx0; tx <= tx1; tx++) atomicAdd(&f.tileCount[cv.tileBase + ty * cv.tilesX + tx], 1);
+		return;
+	}
+	// Direct bins: the slot atomics of up to eight tiles are in flight together before their
+	// list entries are written, so a command over many tiles waits one round trip per eight.
+	constexpr int32_t BATCH = 8;
+	const int32_t w = tx1 - tx0 + 1, count = w * (ty1 - ty0 + 1);
+	for (int32_t base = 0; base < count; base += BATCH) {
+		int32_t tile[BATCH], slot[BATCH];
+#pragma unroll
+		for (int k = 0; k < BATCH; k++) {
+			const int32_t i = base + k;
+			tile[k] = i < count ? cv.tileBase + (ty0 + i / w) * cv.tilesX + tx0 + i % w : -1;
+			slot[k] = tile[k] >= 0 ? atomicAdd(&f.tileCount[tile[k]], 1) : 0;
+		}
+#pragma unroll
+		for (int k = 0; k < BATCH; k++) {
+			if (tile[k] < 0) continue;
+			if (slot[k] < f.binCapacity) {
+				f.tileList[(size_t)tile[k] * f.binCapacity + slot[k]] = index;
 			} else {
-				atomicAdd(&f.tileCount[t], 1);
+				atomicOr(&f.counters[2], 2);
+				atomicMax(&f.counters[6], slot[k] + 1);
 			}
+			tally.pairs++;
+			tally.tiles += slot[k] == 0 ? 1 : 0;
 		}
 	}
 }

@@ -189,7 +189,7 @@ def test_config3_adam_fit(api, fit_setup):
         loss = ((img - target) ** 2).sum()
         loss.backward()
         opt.step()
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     print(f"[gpu] fit: loss {losses[0]:.2f} -> {losses[-1]:.2f}", flush=True)
     assert losses[-1] < 0.1 * losses[0]
 
