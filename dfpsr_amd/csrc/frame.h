@@ -118,6 +118,7 @@ private:
 	bool alphaAny_ = false;
 	bool orthogonalAny_ = false;  // a task with an orthogonal camera
 	bool oddCanvas_ = false;      // a canvas of odd height
+	bool wideCanvas_ = false;     // a canvas wider than the visibility raster's 16-bit row columns
 	int32_t commandCapacity_ = 0, pairCapacity_ = 0;
 	// Entries per tile of the direct bins; grows to the longest list seen.  Bins are used while
 	// tiles x binCapacity_ stays within BIN_ENTRY_LIMIT, the scanned lists otherwise.
@@ -155,7 +156,7 @@ private:
 	// The visibility raster reproduces the in-order result when every command is Solid and
 	// perspective and no canvas has an odd height.
 	bool deferredRaster(const RenderOptions &o) const {
-		return o.mode == 0 && o.rasterPath == 0 && !alphaAny_ && !orthogonalAny_ && !oddCanvas_;
+		return o.mode == 0 && o.rasterPath == 0 && !alphaAny_ && !orthogonalAny_ && !oddCanvas_ && !wideCanvas_;
 	}
 };
 

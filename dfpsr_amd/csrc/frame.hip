@@ -63,6 +63,7 @@ void Frame::reset(int32_t tileConfig) {
 	alphaAny_ = false;
 	orthogonalAny_ = false;
 	oddCanvas_ = false;
+	wideCanvas_ = false;
 }
 
 int32_t Frame::addCanvas(const CanvasHost &c) {
@@ -82,6 +83,7 @@ int32_t Frame::addCanvas(const CanvasHost &c) {
 	d.gradColor = c.gradColor;
 	d.diffStride = c.diffStride;
 	if (c.height & 1) oddCanvas_ = true;
+	if (c.width > 32767) wideCanvas_ = true;
 	canvases_.push_back(d);
 	canvasTileEnd_.push_back(d.tileBase + d.tilesX * d.tilesY);
 	return (int32_t)canvases_.size() - 1;

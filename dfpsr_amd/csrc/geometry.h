@@ -441,40 +441,29 @@ DH int64_t divTrunc64(int64_t n, int64_t d) {
 	return q;
 }
 
-// Row r (0-based from clip.t) of rasterizeTriangle (ITriangle2D.cpp:150-169).
-DH RowInterval rowInterval(const EdgeSetup *edges, bool degenerate, const Rect &clip, int32_t r) {
-	const int32_t L = clip.l, R = clip.right();
-	RowInterval row;
-	if (degenerate) { row.left = R; row.right = L; return row; }
-	row.left = L;
-	row.right = R;
-	for (int k = 0; k < 3; k++) {
-		const EdgeSetup &e = edges[k];
-		if (e.kind == 1) {
-			int64_t limit = e.limitTop - e.offsetY * (int64_t)r;
-			int32_t side = tmin(tmax(L, (int32_t)(divTrunc64(limit + 1, e.offsetX) + 1)), R);
-			row.left = tmax(row.left, side);
-		} else if (e.kind == 2) {
-			int64_t limit = e.limitTop - e.offsetY * (int64_t)r;
-			int32_t side = tmin(tmax(L, (int32_t)(divTrunc64(limit, e.offsetX) + 1)), R);
-			row.right = tmin(row.right, side);
-		} else if (e.kind == 3) {
-			int64_t value = e.valueTop + e.offsetY * (int64_t)r;
-			if (value > e.threshold) { row.left = R; row.right = L; }
-		}
-	}
-	return row;
-}
-
 DH bool sameCorner(const l2 &a, const l2 &b) { return a.x == b.x && a.y == b.y; }
+
+// An edge as the rows read it: a left or right edge (kind 1, 2) keeps a = offsetX,
+// b = offsetY, c = limitTop; a horizontal edge (kind 3) keeps a = threshold, b = offsetY,
+// c = valueTop; an edge without effect (kind 0) keeps nothing.
+struct EdgeRows { int64_t a, b, c; };
+
+DH EdgeRows edgeRows(const EdgeSetup &st) {
+	EdgeRows e;
+	e.a = st.kind == 3 ? st.threshold : st.offsetX;
+	e.b = st.offsetY;
+	e.c = st.kind == 3 ? st.valueTop : st.limitTop;
+	return e;
+}
 
 // What every row of a triangle needs (ITriangle2D.cpp:70-176): the three edges of
 // rasterizeTriangle, the clipped columns and the aligned row bound.  Set up once per draw
-// command; rows are then independent of each other.
+// command; rows are then independent of each other.  flags: the edges' kinds in bits 0-1,
+// 2-3 and 4-5, bit 6 = two equal corners.
 struct __attribute__((aligned(16))) TriangleRows {
-	EdgeSetup edges[3];
+	EdgeRows edges[3];
 	int32_t clipL, clipR, alT, alB;
-	int32_t degenerate, pad;
+	int32_t flags;
 };
 
 DH TriangleRows triangleRows(const l2 &A, const l2 &B, const l2 &C, const Rect &canvas) {
@@ -482,25 +471,47 @@ DH TriangleRows triangleRows(const l2 &A, const l2 &B, const l2 &C, const Rect &
 	const Rect clipR = cutRect(wb, canvas);
 	const Rect al = alignedRasterBound(wb, canvas);
 	const Rect rowClip = rectSides(clipR.l, al.t, clipR.right(), al.bottom());
+	const EdgeSetup e0 = edgeSetup(A, B, rowClip), e1 = edgeSetup(B, C, rowClip), e2 = edgeSetup(C, A, rowClip);
 	TriangleRows tr;
-	tr.edges[0] = edgeSetup(A, B, rowClip);
-	tr.edges[1] = edgeSetup(B, C, rowClip);
-	tr.edges[2] = edgeSetup(C, A, rowClip);
+	tr.edges[0] = edgeRows(e0);
+	tr.edges[1] = edgeRows(e1);
+	tr.edges[2] = edgeRows(e2);
 	tr.clipL = clipR.l;
 	tr.clipR = clipR.right();
 	tr.alT = al.t;
 	tr.alB = al.bottom();
-	tr.degenerate = (sameCorner(A, B) || sameCorner(B, C) || sameCorner(C, A)) ? 1 : 0;
-	tr.pad = 0;
+	const bool degenerate = sameCorner(A, B) || sameCorner(B, C) || sameCorner(C, A);
+	tr.flags = e0.kind | (e1.kind << 2) | (e2.kind << 4) | (degenerate ? 64 : 0);
 	return tr;
 }
 
-// Row y of the triangle; empty outside the aligned bound.
+// Row y of the triangle (rasterizeTriangle, ITriangle2D.cpp:150-169, row y - alT of the
+// aligned bound); empty outside the aligned bound.
 DH RowInterval triangleRow(const TriangleRows &tr, int32_t y) {
-	RowInterval ri{0, 0};
-	if (y >= tr.alT && y < tr.alB)
-		ri = rowInterval(tr.edges, tr.degenerate != 0, rectSides(tr.clipL, tr.alT, tr.clipR, tr.alB), y - tr.alT);
-	return ri;
+	RowInterval row{0, 0};
+	if (y < tr.alT || y >= tr.alB) return row;
+	const int32_t L = tr.clipL, R = tr.clipR;
+	if (tr.flags & 64) { row.left = R; row.right = L; return row; }
+	const int64_t r = y - tr.alT;
+	row.left = L;
+	row.right = R;
+	for (int k = 0; k < 3; k++) {
+		const EdgeRows &e = tr.edges[k];
+		const int32_t kind = (tr.flags >> (2 * k)) & 3;
+		if (kind == 1) {
+			int64_t limit = e.c - e.b * r;
+			int32_t side = tmin(tmax(L, (int32_t)(divTrunc64(limit + 1, e.a) + 1)), R);
+			row.left = tmax(row.left, side);
+		} else if (kind == 2) {
+			int64_t limit = e.c - e.b * r;
+			int32_t side = tmin(tmax(L, (int32_t)(divTrunc64(limit, e.a) + 1)), R);
+			row.right = tmin(row.right, side);
+		} else if (kind == 3) {
+			int64_t value = e.c + e.b * r;
+			if (value > e.a) { row.left = R; row.right = L; }
+		}
+	}
+	return row;
 }
 
 }  // namespace dfpsr

@@ -864,7 +864,8 @@ template __global__ void k_raster<32, 32>(FrameDev, int32_t);
 // per (quad, winning command) for the shading.
 //
 // Frames with the Alpha filter (blending reads the colour held at that moment), orthogonal
-// cameras (a lower depth wins) or an odd-height canvas (the missing last row aliases lanes
+// cameras (a lower depth wins), canvases wider than 32767 pixels (rows are kept in 16 bits) or
+// an odd-height canvas (the missing last row aliases lanes
 // onto the row above within one command) take the in-order kernel above.
 // ---------------------------------------------------------------------------------------
 
@@ -874,6 +875,12 @@ __device__ __forceinline__ uint32_t depthOrder(float d) {
 	uint32_t u = __float_as_uint(d == 0.0f ? 0.0f : d);
 	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
+
+// A row interval in LDS: columns fit 16 bits (the visibility raster takes canvases up to
+// 32767 pixels wide).
+struct RowSpan { int16_t left, right; };
+__device__ __forceinline__ RowSpan rowSpan(RowInterval r) { return RowSpan{(int16_t)r.left, (int16_t)r.right}; }
+__device__ __forceinline__ RowInterval rowInterval(RowSpan r) { return RowInterval{r.left, r.right}; }
 
 // Threads of a visibility-raster workgroup: four pixels of the tile per thread.
 template <int TW, int TH> struct DeferShape {
@@ -907,14 +914,20 @@ template <int TW, int TH> struct DeferLds {
 	};
 	ShaderInputs si[S::CHUNK];                   // shader inputs,
 	CommandTail tail[S::CHUNK];                  // weight planes and flags
-	RowInterval rows[S::CHUNK * TH];             // chunk command j, tile row r
-	int32_t pairStart[PAIRS];                    // first quad column (pixel x) of each pair in the tile
-	int32_t pairOffset[PAIRS + 1];               // exclusive prefix of quads per pair
-	uint32_t quadWinners[(TW / 2) * (TH / 2)];   // distinct winning chunk commands of a quad, 8 bits each
-	int32_t quadOffset[(TW / 2) * (TH / 2) + 1]; // exclusive prefix of distinct winners per quad
+	RowSpan rows[S::CHUNK * TH];                 // chunk command j, tile row r
+	union {
+		struct {                                 // until pass 1 ends:
+			uint8_t pairStart[PAIRS];            // first quad column of each pair, from the tile's left
+			int16_t pairOffset[PAIRS + 1];       // exclusive prefix of quads per pair
+		};
+		struct {                                 // from the winners on:
+			uint32_t quadWinners[(TW / 2) * (TH / 2)];   // distinct winning chunk commands of a quad, 8 bits each
+			int16_t quadOffset[(TW / 2) * (TH / 2) + 1]; // exclusive prefix of distinct winners per quad
+		};
+	};
 	uint32_t color[TW * TH];                     // the tile's canvas pixels as they stand
 	float depth[TW * TH];
-	uint8_t dirty[TW * TH];
+	uint32_t dirty[TW * TH / 32];                // pixels written, one bit each
 	int32_t scan[S::NT];
 	TextureDesc textures[LDS_TEXTURES];
 	int32_t roundInfo[4];
@@ -970,7 +983,8 @@ __device__ __forceinline__ int32_t keyWinner(unsigned long long key, bool depthR
 }
 
 // Index of the segment holding item it in an exclusive prefix of count + 1 entries.
-__device__ __forceinline__ int32_t segmentOf(const int32_t *offset, int32_t count, int32_t it) {
+template <typename T>
+__device__ __forceinline__ int32_t segmentOf(const T *offset, int32_t count, int32_t it) {
 	int32_t lo = 0, hi = count - 1;
 	while (lo < hi) {
 		const int32_t mid = (lo + hi + 1) >> 1;
@@ -1003,13 +1017,12 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 		if (f.clearInRaster) {
 			s.depth[p] = f.clearDepth;
 			s.color[p] = f.clearColor;
-			s.dirty[p] = 1;
 		} else {
 			const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
 			s.depth[p] = (depthRead && x < cv.width && y < cv.height) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
-			s.dirty[p] = 0;
 		}
 	}
+	for (int32_t w = t; w < NP / 32; w += NT) s.dirty[w] = f.clearInRaster ? ~0u : 0u;
 
 	int32_t last = -1;
 	int32_t done = 0;
@@ -1025,7 +1038,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 			__syncthreads();
 			// Rows of every chunk command in the tile, one (command, row) per thread: the exact
 			// intervals of rasterizeTriangle inside the aligned bound, empty outside it.
-			for (int32_t idx = t; idx < cn * TH; idx += NT) s.rows[idx] = triangleRow(s.tr[idx / TH], tileY0 + idx % TH);
+			for (int32_t idx = t; idx < cn * TH; idx += NT) s.rows[idx] = rowSpan(triangleRow(s.tr[idx / TH], tileY0 + idx % TH));
 			__syncthreads();
 			// Keys start at the held depth with the lowest command part above every command's,
 			// so only a strictly greater depth replaces it; a NaN depth is never replaced.
@@ -1042,7 +1055,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				const int32_t j = pr / RP, rp = pr % RP;
 				int32_t count = 0, x0 = 0;
 				if (pr < PAIRS && j < cn) {
-					const RowInterval upper = s.rows[j * TH + 2 * rp], lower = s.rows[j * TH + 2 * rp + 1];
+					const RowInterval upper = rowInterval(s.rows[j * TH + 2 * rp]), lower = rowInterval(s.rows[j * TH + 2 * rp + 1]);
 					if (upper.right > upper.left || lower.right > lower.left) {
 						const int32_t outerBlockStart = (int32_t)((uint32_t)tmin(upper.left, lower.left) & ~1u);
 						const int32_t outerBlockEnd = (int32_t)(((uint32_t)tmax(upper.right, lower.right) + 1u) & ~1u);
@@ -1051,7 +1064,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 						count = x1 > x0 ? (x1 - x0) >> 1 : 0;
 					}
 				}
-				if (pr < PAIRS) s.pairStart[pr] = x0;
+				if (pr < PAIRS) s.pairStart[pr] = (uint8_t)(x0 - tileX0);
 				pairCount[k] = count;
 				threadSum += count;
 			}
@@ -1059,19 +1072,20 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 			const int32_t items = groupPrefix<NT>(threadSum, before, s.scan);
 			for (int32_t k = 0; k < PAIRS_PER_THREAD; k++) {
 				const int32_t pr = t * PAIRS_PER_THREAD + k;
-				if (pr < PAIRS) s.pairOffset[pr] = before;
+				if (pr < PAIRS) s.pairOffset[pr] = (int16_t)before;
 				before += pairCount[k];
 			}
-			if (t == 0) s.pairOffset[PAIRS] = items;
+			if (t == 0) s.pairOffset[PAIRS] = (int16_t)items;
 			__syncthreads();
 			// Pass 1: one (command, quad) item per thread offers the covered lanes' keys.
 			for (int32_t it = t; it < items; it += NT) {
 				const int32_t pr = segmentOf(s.pairOffset, PAIRS, it);
 				const int32_t j = pr / RP, rp = pr % RP;
-				const int32_t px = s.pairStart[pr] + 2 * (it - s.pairOffset[pr]);
+				const int32_t px = tileX0 + s.pairStart[pr] + 2 * (it - s.pairOffset[pr]);
 				const int32_t py = tileY0 + 2 * rp;
 				QuadEval e;
-				if (!evalQuad(s.tail[j].plane, s.rows[j * TH + 2 * rp], s.rows[j * TH + 2 * rp + 1], px, py, cv.height, e)) continue;
+				if (!evalQuad(s.tail[j].plane, rowInterval(s.rows[j * TH + 2 * rp]), rowInterval(s.rows[j * TH + 2 * rp + 1]), px, py, cv.height, e))
+					continue;
 				for (int l = 0; l < 4; l++) {
 					if (e.clipSides && !e.clip[l]) continue;
 					const int32_t sl = e.slot[l];
@@ -1109,10 +1123,10 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 			const int32_t shades = groupPrefix<NT>(threadSum, before, s.scan);
 			for (int32_t k = 0; k < QUADS_PER_THREAD; k++) {
 				const int32_t qi = t * QUADS_PER_THREAD + k;
-				if (qi < NQ) s.quadOffset[qi] = before;
+				if (qi < NQ) s.quadOffset[qi] = (int16_t)before;
 				before += quadCount[k];
 			}
-			if (t == 0) s.quadOffset[NQ] = shades;
+			if (t == 0) s.quadOffset[NQ] = (int16_t)shades;
 			__syncthreads();
 			// Pass 2: one (quad, winning command) item per thread: the quad's weights and mip level
 			// once, then the lanes the command won.
@@ -1121,7 +1135,8 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				const int32_t jw = (int32_t)((s.quadWinners[qi] >> (8 * (it - s.quadOffset[qi]))) & 0xFFu);
 				const int32_t qx = (qi % NQX) * 2, qy = (qi / NQX) * 2;
 				QuadEval e;
-				if (!evalQuad(s.tail[jw].plane, s.rows[jw * TH + qy], s.rows[jw * TH + qy + 1], tileX0 + qx, tileY0 + qy, cv.height, e))
+				if (!evalQuad(s.tail[jw].plane, rowInterval(s.rows[jw * TH + qy]), rowInterval(s.rows[jw * TH + qy + 1]), tileX0 + qx,
+				              tileY0 + qy, cv.height, e))
 					continue;
 				uint32_t mask = 0;
 				for (int l = 0; l < 4; l++) {
@@ -1129,22 +1144,30 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 					const int32_t sl = e.slot[l];
 					if (keyWinner(s.keys[(qy + (sl >> 1)) * TW + qx + (sl & 1)], depthRead) == jw) mask |= 1u << l;
 				}
-				float r[4], g[4], b[4], a[4];
-				if (colorWrite) {
+				// Depths first, so that only the weights and the won lanes' pixels outlive the quad
+				// evaluation while the colours are made.
+				uint32_t slots = 0;
+				for (int l = 0; l < 4; l++) {
+					if (!((mask >> l) & 1u)) continue;
+					const int32_t sl = e.slot[l];
+					const int32_t p = (qy + (sl >> 1)) * TW + qx + (sl & 1);
+					s.depth[p] = e.l0[l];
+					if (!f.clearInRaster) atomicOr(&s.dirty[p >> 5], 1u << (p & 31));
+					slots |= (uint32_t)sl << (2 * l);
+				}
+				if (colorWrite && mask) {
 					const bool affine = (s.tail[jw].flags & 1) != 0;
 					float wa[4], wb[4], wc[4];
 					for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
 					// All four lanes are shaded (every texel index is masked into its level), so the
 					// sixteen texel loads of the quad can be issued together; only won lanes are kept.
+					float r[4], g[4], b[4], a[4];
 					shadeQuad(s.si[jw], textures, wa, wb, wc, 0xFu, r, g, b, a);
-				}
-				for (int l = 0; l < 4; l++) {
-					if (!((mask >> l) & 1u)) continue;
-					const int32_t sl = e.slot[l];
-					const int32_t p = (qy + (sl >> 1)) * TW + qx + (sl & 1);
-					if (colorWrite) s.color[p] = packSaturated(r[l], g[l], b[l], a[l]);
-					s.depth[p] = e.l0[l];
-					s.dirty[p] = 1;
+					for (int l = 0; l < 4; l++) {
+						if (!((mask >> l) & 1u)) continue;
+						const int32_t sl = (int32_t)((slots >> (2 * l)) & 3u);
+						s.color[(qy + (sl >> 1)) * TW + qx + (sl & 1)] = packSaturated(r[l], g[l], b[l], a[l]);
+					}
 				}
 			}
 			__syncthreads();
@@ -1154,7 +1177,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	}
 
 	for (int32_t p = t; p < NP; p += NT) {
-		if (!s.dirty[p]) continue;
+		if (!((s.dirty[p >> 5] >> (p & 31)) & 1u)) continue;
 		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
 		if (x >= cv.width || y >= cv.height) continue;
 		if (colorWrite) cv.color[(size_t)y * cv.colorStride + x] = s.color[p];
@@ -1162,8 +1185,10 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	}
 }
 
+// Four waves per SIMD: the raster is bound by latency, and the fourth wave hides more of it than
+// the few registers the compiler then keeps in scratch cost.
 template <int TW, int TH>
-__global__ void __launch_bounds__(TW * TH / 4) k_raster_deferred(FrameDev f, int32_t persistent) {
+__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred(FrameDev f, int32_t persistent) {
 	__shared__ DeferLds<TW, TH> s;
 	reportCounters(f);
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
