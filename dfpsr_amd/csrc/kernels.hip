@@ -35,6 +35,22 @@ DH Rect canvasRect(const CanvasDev &cv) { return rectSides(0, 0, cv.width, cv.he
 // (command, tile) pairs and newly used tiles counted by one thread of the setup kernel.
 struct BinTally { int32_t pairs, tiles; };
 
+// A command's tiles as the setup block bins them: the first tile, the width of the tile
+// rectangle, the canvas's tiles per row, and the command's first pair in the block's pairs.
+struct BinJob { int32_t index, firstTile, width, tilesX, pairStart; };
+constexpr int32_t BIN_JOBS = 512;  // commands a setup block bins together (more: each thread bins its own)
+
+// Enters pair slot[k] of tile[k] into the tile's list (direct bins).
+__device__ __forceinline__ void binPair(const FrameDev &f, int32_t tile, int32_t slot, int32_t index, BinTally &tally) {
+	if (slot < f.binCapacity) {
+		f.tileList[(size_t)tile * f.binCapacity + slot] = index;
+	} else {
+		atomicOr(&f.counters[2], 2);
+		atomicMax(&f.counters[6], slot + 1);
+	}
+	tally.tiles += slot == 0 ? 1 : 0;
+}
+
 // Writes draw command `index` with everything the raster needs of it -- the command, its row
 // setup and (for the RGBA8 shaders) its finished shader inputs -- and enters it into every tile
 // it can touch: the columns of cut(wholeBound, canvas), the rows of the aligned raster bound
@@ -42,25 +58,34 @@ struct BinTally { int32_t pairs, tiles; };
 // any order: the raster sorts each list); otherwise only the tile's count grows here and the
 // list is filled after the counts are scanned.
 __device__ void emitCommand(const FrameDev &f, int32_t index, const Command &cmd, const CanvasDev &cv,
-                            const ShaderInputs &si, BinTally &tally) {
+                            const ShaderInputs &si, BinTally &tally, BinJob *jobs, int32_t jobBase) {
 	f.commands[index] = cmd;
 	const TriangleRows tr = triangleRows(l2{cmd.fx[0], cmd.fy[0]}, l2{cmd.fx[1], cmd.fy[1]}, l2{cmd.fx[2], cmd.fy[2]}, canvasRect(cv));
 	f.triangleRows[index] = tr;
 	if (f.mode == 0) f.shaders[index] = si;
-	if (tr.clipR <= tr.clipL) return;  // the bound misses the canvas
-	const int32_t tx0 = tr.clipL / f.tileW, tx1 = (tr.clipR - 1) / f.tileW;
-	const int32_t ty0 = tr.alT / f.tileH;
-	int32_t ty1 = (tr.alB - 2) / f.tileH;
-	if (ty1 > cv.tilesY - 1) ty1 = cv.tilesY - 1;
+	int32_t tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;  // no tiles when the bound misses the canvas
+	if (tr.clipR > tr.clipL) {
+		tx0 = tr.clipL / f.tileW;
+		tx1 = (tr.clipR - 1) / f.tileW;
+		ty0 = tr.alT / f.tileH;
+		ty1 = tmin((tr.alB - 2) / f.tileH, cv.tilesY - 1);
+	}
 	if (f.binCapacity == 0) {
 		for (int32_t ty = ty0; ty <= ty1; ty++)
 			for (int32_t tx = tx0; tx <= tx1; tx++) atomicAdd(&f.tileCount[cv.tileBase + ty * cv.tilesX + tx], 1);
 		return;
 	}
-	// Direct bins: the slot atomics of up to eight tiles are in flight together before their
-	// list entries are written, so a command over many tiles waits one round trip per eight.
+	const int32_t w = tx1 - tx0 + 1, count = tmax(w, 0) * tmax(ty1 - ty0 + 1, 0);
+	if (jobs) {
+		// Binned later by the whole block, pairs spread over its threads.  Every command writes
+		// its job, with no pairs when it touches no tile.
+		jobs[index - jobBase] = BinJob{index, cv.tileBase + ty0 * cv.tilesX + tx0, tmax(w, 1), cv.tilesX, tally.pairs};
+		tally.pairs += count;
+		return;
+	}
+	// Direct bins of this thread alone: the slot atomics of up to eight tiles are in flight
+	// together before their list entries are written.
 	constexpr int32_t BATCH = 8;
-	const int32_t w = tx1 - tx0 + 1, count = w * (ty1 - ty0 + 1);
 	for (int32_t base = 0; base < count; base += BATCH) {
 		int32_t tile[BATCH], slot[BATCH];
 #pragma unroll
@@ -72,14 +97,8 @@ __device__ void emitCommand(const FrameDev &f, int32_t index, const Command &cmd
 #pragma unroll
 		for (int k = 0; k < BATCH; k++) {
 			if (tile[k] < 0) continue;
-			if (slot[k] < f.binCapacity) {
-				f.tileList[(size_t)tile[k] * f.binCapacity + slot[k]] = index;
-			} else {
-				atomicOr(&f.counters[2], 2);
-				atomicMax(&f.counters[6], slot[k] + 1);
-			}
+			binPair(f, tile[k], slot[k], index, tally);
 			tally.pairs++;
-			tally.tiles += slot[k] == 0 ? 1 : 0;
 		}
 	}
 }
@@ -123,7 +142,8 @@ __device__ ShaderInputs triangleShader(const TaskDev &t, int32_t poly, int32_t k
 }
 
 template <bool EMIT>
-__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset, BinTally &tally) {
+__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset, BinTally &tally,
+                                 BinJob *jobs = nullptr, int32_t jobBase = 0) {
 	const TaskDev &t = f.tasks[ti];
 	const CanvasDev &cv = f.canvases[t.canvas];
 	if (cv.color == nullptr && cv.depth == nullptr) return 0;
@@ -152,7 +172,7 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 				for (int q = 0; q < 3; q++) { c.fx[q] = P[q].flat.x; c.fy[q] = P[q].flat.y; }
 				c.start = pr.start; c.dx = pr.dx; c.dy = pr.dy;
 				c.task = ti; c.fan = local; c.canvas = t.canvas; c.flags = perspective ? 0 : 1;
-				emitCommand(f, offset, c, cv, si, tally);
+				emitCommand(f, offset, c, cv, si, tally, jobs, jobBase);
 			}
 			count = 1;
 		}
@@ -172,7 +192,7 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 					for (int q = 0; q < 3; q++) { c.fx[q] = S[q].flat.x; c.fy[q] = S[q].flat.y; }
 					c.start = pr.start; c.dx = pr.dx; c.dy = pr.dy;
 					c.task = ti; c.fan = local; c.canvas = t.canvas; c.flags = perspective ? 0 : 1;
-					emitCommand(f, offset + count, c, cv, si, tally);
+					emitCommand(f, offset + count, c, cv, si, tally, jobs, jobBase);
 				}
 				count++;
 			}
@@ -273,32 +293,73 @@ __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_
 
 __global__ void k_setup_emit(FrameDev f) {
 	__shared__ int32_t blockTally[2];
+	__shared__ BinJob jobs[BIN_JOBS];
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	int32_t n = (g < f.fanTotal) ? f.fanCount[g] : 0;
 	int32_t ex;
-	blockExclusiveScan(n, ex);
+	const int32_t blockCommands = blockExclusiveScan(n, ex);
 	if (threadIdx.x < 2) blockTally[threadIdx.x] = 0;
+	// With direct bins the block's commands are binned together (below) when they fit the table
+	// and the command arrays (a block past the capacity leaves its jobs unwritten).
+	const int32_t jobBase = f.blockSums[blockIdx.x];
+	const bool together = f.binCapacity > 0 && blockCommands <= BIN_JOBS && jobBase + blockCommands <= f.commandCapacity;
 	BinTally tally{0, 0};
 	if (g < f.fanTotal) {
-		const int32_t offset = f.blockSums[blockIdx.x] + ex;
+		const int32_t offset = jobBase + ex;
 		const int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
 		// The commands of a canvas's tasks are consecutive, so the offset at the first fan
 		// triangle of each task bounds its canvas's first command: one atomic per task.
 		if (g == f.tasks[ti].fanBase) atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
 		if (n > 0) {
 			if (offset + n > f.commandCapacity) atomicOr(&f.counters[2], 1);
-			else setupTriangle<true>(f, g, ti, offset, tally);
+			else setupTriangle<true>(f, g, ti, offset, tally, together ? jobs : nullptr, jobBase);
 		}
 	}
-	if (f.binCapacity > 0) {
-		// Direct bins: the pair and used-tile totals, one global atomic per block.
+	if (f.binCapacity == 0) return;
+	if (together) {
+		// Pairs of the block's commands, one per thread and round: job j owns pairs
+		// [pairStart, pairStart + width * height) in command order.  A command whose setup ran
+		// out of room (capacity) wrote no job; then no command of the frame is drawn anyway.
+		int32_t pairBase;
+		const int32_t pairs = blockExclusiveScan(tally.pairs, pairBase);
+		for (int32_t k = 0; k < n; k++) jobs[ex + k].pairStart += pairBase;
 		__syncthreads();
-		if (tally.pairs) atomicAdd(&blockTally[0], tally.pairs);
-		if (tally.tiles) atomicAdd(&blockTally[1], tally.tiles);
-		__syncthreads();
-		if (threadIdx.x == 0 && blockTally[0]) atomicAdd(&f.counters[1], blockTally[0]);
-		if (threadIdx.x == 1 && blockTally[1]) atomicAdd(&f.counters[3], blockTally[1]);
+		const int32_t jobCount = blockCommands;
+		BinTally binned{0, 0};
+		constexpr int32_t UNROLL = 4;
+		for (int32_t p0 = threadIdx.x; p0 < pairs; p0 += UNROLL * SCAN_BLOCK) {
+			int32_t tile[UNROLL], slot[UNROLL], index[UNROLL];
+#pragma unroll
+			for (int u = 0; u < UNROLL; u++) {
+				const int32_t p = p0 + u * SCAN_BLOCK;
+				tile[u] = -1;
+				if (p < pairs) {
+					int32_t lo = 0, hi = jobCount - 1;
+					while (lo < hi) {
+						const int32_t mid = (lo + hi + 1) >> 1;
+						if (jobs[mid].pairStart <= p) lo = mid; else hi = mid - 1;
+					}
+					const BinJob &j = jobs[lo];
+					const int32_t i = p - j.pairStart;
+					tile[u] = j.firstTile + (i / j.width) * j.tilesX + i % j.width;
+					index[u] = j.index;
+					slot[u] = atomicAdd(&f.tileCount[tile[u]], 1);
+				}
+			}
+#pragma unroll
+			for (int u = 0; u < UNROLL; u++)
+				if (tile[u] >= 0) binPair(f, tile[u], slot[u], index[u], binned);
+		}
+		tally.tiles = binned.tiles;
+		tally.pairs = threadIdx.x == 0 ? pairs : 0;
 	}
+	// The pair and used-tile totals, one global atomic per block.
+	__syncthreads();
+	if (tally.pairs) atomicAdd(&blockTally[0], tally.pairs);
+	if (tally.tiles) atomicAdd(&blockTally[1], tally.tiles);
+	__syncthreads();
+	if (threadIdx.x == 0 && blockTally[0]) atomicAdd(&f.counters[1], blockTally[0]);
+	if (threadIdx.x == 1 && blockTally[1]) atomicAdd(&f.counters[3], blockTally[1]);
 }
 
 // ---------------------------------------------------------------------------------------
