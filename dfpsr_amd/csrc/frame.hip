@@ -146,8 +146,9 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	hipLaunchKernelGGL(k_frame_begin, dim3(blocksOf(beginThreads, 256)), dim3(256), 0, stream, f);
 	if (events) HIP_TRY(hipEventRecord(events_[1], stream));
 	hipLaunchKernelGGL(k_setup_count, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
-	hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, fanBlocks, f.counters + 0);
-	hipLaunchKernelGGL(k_setup_emit, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
+	const int32_t sumBlocks = fanBlocks <= SETUP_SUM_BLOCKS ? 1 : 0;
+	if (!sumBlocks) hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, fanBlocks, f.counters + 0);
+	hipLaunchKernelGGL(k_setup_emit, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f, sumBlocks);
 	if (events) HIP_TRY(hipEventRecord(events_[2], stream));
 	if (f.binCapacity == 0) {
 	hipLaunchKernelGGL(k_scan_tiles_local, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);

@@ -291,7 +291,9 @@ __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_
 	}
 }
 
-__global__ void k_setup_emit(FrameDev f) {
+// sumBlocks = 1: each block adds up the totals of the blocks before it (and the last block writes
+// the command total) instead of reading offsets scanned by k_scan_block_sums.
+__global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 	__shared__ int32_t blockTally[2];
 	__shared__ BinJob jobs[BIN_JOBS];
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
@@ -299,9 +301,17 @@ __global__ void k_setup_emit(FrameDev f) {
 	int32_t ex;
 	const int32_t blockCommands = blockExclusiveScan(n, ex);
 	if (threadIdx.x < 2) blockTally[threadIdx.x] = 0;
+	int32_t jobBase;
+	if (sumBlocks) {
+		int32_t partial = 0, unusedPrefix;
+		for (int32_t i = threadIdx.x; i < (int32_t)blockIdx.x; i += SCAN_BLOCK) partial += f.blockSums[i];
+		jobBase = blockExclusiveScan(partial, unusedPrefix);
+		if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) f.counters[0] = jobBase + blockCommands;
+	} else {
+		jobBase = f.blockSums[blockIdx.x];
+	}
 	// With direct bins the block's commands are binned together (below) when they fit the table
 	// and the command arrays (a block past the capacity leaves its jobs unwritten).
-	const int32_t jobBase = f.blockSums[blockIdx.x];
 	const bool together = f.binCapacity > 0 && blockCommands <= BIN_JOBS && jobBase + blockCommands <= f.commandCapacity;
 	BinTally tally{0, 0};
 	if (g < f.fanTotal) {

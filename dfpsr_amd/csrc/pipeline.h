@@ -128,7 +128,8 @@ __global__ void k_frame_begin(FrameDev f);
 __global__ void k_setup_count(FrameDev f);
 __global__ void k_scan_block_sums(int32_t *blockSums, int32_t blocks, int32_t *total);
 __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_t *totalLow, int32_t *totalHigh);
-__global__ void k_setup_emit(FrameDev f);
+__global__ void k_setup_emit(FrameDev f, int32_t sumBlocks);
+constexpr int32_t SETUP_SUM_BLOCKS = 2048;  // up to this many setup blocks, each sums its offset itself
 __global__ void k_scan_tiles_local(FrameDev f);
 __global__ void k_scan_tiles_add(FrameDev f);
 __global__ void k_bin_fill(FrameDev f);

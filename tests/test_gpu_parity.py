@@ -218,3 +218,13 @@ def test_direct_bins_grow_to_the_longest_tile_list(api):
         assert np.array_equal(got[0], scanned[0]) and np.array_equal(got[1].view(np.uint32), scanned[1].view(np.uint32))
     assert list(first_stats[:3]) == list(scanned_stats[:3]) == list(second_stats[:3])
     assert_parity(sc, first, H.oracle_render(sc))
+
+
+def test_setup_offsets_scanned_for_many_fan_triangles(api):
+    """Above 2048 setup blocks (524288 fan triangles) the command offsets come from the block
+    scan kernel instead of each block summing the totals before it; both give the oracle's
+    canvases."""
+    sc = S.cube_soup(width=64, height=48, triangles=540000, seed=3, spread=0.02, cam_distance=2.5, name="many_triangles")
+    print(f"[gpu] many fan triangles {sc.indices.shape[0]}", flush=True)
+    got = api.render_scene(sc)
+    assert_parity(sc, got, H.oracle_render(sc))
