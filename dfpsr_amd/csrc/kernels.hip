@@ -426,9 +426,21 @@ __global__ void k_bin_fill(FrameDev f) {
 // Tile raster / shade.
 // ---------------------------------------------------------------------------------------
 
-// Bitonic sort of n ints in LDS by the workgroup's NL lanes (n <= TILE_SORT_CAP).
+// Sort of n distinct ints in LDS by the workgroup's NL lanes (n <= TILE_SORT_CAP).  Up to NL
+// values are ranked (each lane counts the smaller values and writes its own to that place);
+// longer lists take a bitonic sort.
 template <int NL>
 __device__ void sortLds(int32_t *a, int32_t n) {
+	if (n <= NL) {
+		const int32_t i = (int32_t)threadIdx.x;
+		const int32_t v = i < n ? a[i] : 0;
+		int32_t rank = 0;
+		for (int32_t j = 0; j < n; j++) rank += a[j] < v ? 1 : 0;
+		__syncthreads();
+		if (i < n) a[rank] = v;
+		__syncthreads();
+		return;
+	}
 	int32_t size = 1;
 	while (size < n) size <<= 1;
 	for (int32_t i = n + (int32_t)threadIdx.x; i < size; i += NL) a[i] = 0x7FFFFFFF;
