@@ -201,23 +201,42 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 	return count;
 }
 
-// Block-wide exclusive scan of one int32 per thread in LDS; returns the block total.
-__device__ int32_t blockExclusiveScan(int32_t value, int32_t &exclusive) {
-	__shared__ int32_t s[SCAN_BLOCK];
-	int32_t tid = threadIdx.x;
-	s[tid] = value;
-	__syncthreads();
-	for (int32_t offset = 1; offset < SCAN_BLOCK; offset <<= 1) {
-		int32_t v = (tid >= offset) ? s[tid - offset] : 0;
-		__syncthreads();
-		s[tid] += v;
-		__syncthreads();
+// Exclusive prefix over one value per thread of an NT-thread workgroup (NT a multiple of 64);
+// returns the total.  Each wavefront scans its 64 values with lane shuffles, then the
+// wavefront totals meet in waveTotals[NT / 64]: two barriers in all.
+template <int NT>
+__device__ __forceinline__ int32_t workgroupPrefix(int32_t value, int32_t &before, int32_t *waveTotals) {
+	const int32_t lane = (int32_t)(threadIdx.x & 63), wave = (int32_t)(threadIdx.x >> 6);
+	int32_t x = value;
+#pragma unroll
+	for (int32_t offset = 1; offset < 64; offset <<= 1) {
+		const int32_t y = __shfl_up(x, offset, 64);
+		if (lane >= offset) x += y;
 	}
-	int32_t inclusive = s[tid];
-	int32_t total = s[SCAN_BLOCK - 1];
+	if (NT == 64) {
+		__syncthreads();  // a barrier like the multi-wavefront form
+		before = x - value;
+		return __shfl(x, 63, 64);
+	}
+	if (lane == 63) waveTotals[wave] = x;
 	__syncthreads();
-	exclusive = inclusive - value;
+	int32_t waveBase = 0, total = 0;
+#pragma unroll
+	for (int32_t w = 0; w < NT / 64; w++) {
+		const int32_t v = waveTotals[w];
+		waveBase += w < wave ? v : 0;
+		total += v;
+	}
+	__syncthreads();  // waveTotals may be written again right after
+	before = waveBase + x - value;
 	return total;
+}
+
+// Block-wide exclusive scan of one int32 per thread of a SCAN_BLOCK-thread block; returns the
+// block total.
+__device__ int32_t blockExclusiveScan(int32_t value, int32_t &exclusive) {
+	__shared__ int32_t waveTotals[SCAN_BLOCK / 64];
+	return workgroupPrefix<SCAN_BLOCK>(value, exclusive, waveTotals);
 }
 
 __global__ void k_setup_count(FrameDev f) {
@@ -1011,7 +1030,7 @@ template <int TW, int TH> struct DeferLds {
 	uint32_t color[TW * TH];                     // the tile's canvas pixels as they stand
 	float depth[TW * TH];
 	uint32_t dirty[TW * TH / 32];                // pixels written, one bit each
-	int32_t scan[S::NT];
+	int32_t scan[S::NT / 64];                    // wavefront totals of the prefix sums
 	TextureDesc textures[LDS_TEXTURES];
 	int32_t roundInfo[4];
 	int32_t slot;
@@ -1040,23 +1059,6 @@ __device__ __forceinline__ void stageToLds(T *dst, const uint8_t *src, int32_t s
 // Waits for this thread's LDS-direct loads; a barrier then publishes them to the workgroup.
 __device__ __forceinline__ void stageWait() { __builtin_amdgcn_s_waitcnt(0); }
 
-// Exclusive prefix over one value per thread of an NT-thread workgroup; returns the total.
-template <int NT>
-__device__ __forceinline__ int32_t groupPrefix(int32_t value, int32_t &before, int32_t *scan) {
-	const int32_t t = threadIdx.x;
-	scan[t] = value;
-	__syncthreads();
-	for (int32_t offset = 1; offset < NT; offset <<= 1) {
-		const int32_t v = t >= offset ? scan[t - offset] : 0;
-		__syncthreads();
-		scan[t] += v;
-		__syncthreads();
-	}
-	const int32_t inclusive = scan[t], total = scan[NT - 1];
-	__syncthreads();
-	before = inclusive - value;
-	return total;
-}
 
 // The chunk command that won a pixel's key, or -1.
 __device__ __forceinline__ int32_t keyWinner(unsigned long long key, bool depthRead) {
@@ -1152,7 +1154,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				threadSum += count;
 			}
 			int32_t before;
-			const int32_t items = groupPrefix<NT>(threadSum, before, s.scan);
+			const int32_t items = workgroupPrefix<NT>(threadSum, before, s.scan);
 			for (int32_t k = 0; k < PAIRS_PER_THREAD; k++) {
 				const int32_t pr = t * PAIRS_PER_THREAD + k;
 				if (pr < PAIRS) s.pairOffset[pr] = (int16_t)before;
@@ -1203,7 +1205,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				quadCount[k] = count;
 				threadSum += count;
 			}
-			const int32_t shades = groupPrefix<NT>(threadSum, before, s.scan);
+			const int32_t shades = workgroupPrefix<NT>(threadSum, before, s.scan);
 			for (int32_t k = 0; k < QUADS_PER_THREAD; k++) {
 				const int32_t qi = t * QUADS_PER_THREAD + k;
 				if (qi < NQ) s.quadOffset[qi] = (int16_t)before;
