@@ -296,6 +296,9 @@ int Frame::run(const RenderOptions &o) {
 		f.multipliers = multipliers_;
 		lastRun_ = f;
 		lastRunValid_ = false;
+		// The raster kernel copies counters[0..7] into pinned host memory as it starts; the
+		// sentinel, written before anything of the frame is launched, shows whether it did.
+		*(volatile int32_t *)&hostCounters_[2] = -1;
 
 		if (o.graph) {
 			// The graphs stay valid while the launch arguments, the options and the timing switch
@@ -340,11 +343,8 @@ int Frame::run(const RenderOptions &o) {
 			int rc = enqueue(f, o, fanBlocks, tileBlocks, timing, 3);
 			if (rc) return rc;
 		}
-		// The raster kernel copies counters[0..3] into pinned host memory as it starts; the
-		// sentinel shows whether it did.
-		hostCounters_[2] = -1;
 		HIP_TRY(hipStreamSynchronize(stream));
-		if (hostCounters_[2] == -1) return fail(-3, "renderer_end: the frame did not report its counters");
+		if (*(volatile int32_t *)&hostCounters_[2] == -1) return fail(-3, "renderer_end: the frame did not report its counters");
 		stats[0] = hostCounters_[0];
 		stats[1] = hostCounters_[1];
 		stats[2] = hostCounters_[3];
