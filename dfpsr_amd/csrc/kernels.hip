@@ -7,6 +7,8 @@
 // shader/RgbaMultiply.h:74-175.
 #include "pipeline.h"
 
+#include <type_traits>
+
 namespace dfpsr {
 
 // ---------------------------------------------------------------------------------------
@@ -1009,6 +1011,9 @@ static_assert(sizeof(TriangleRows) % 16 == 0 && sizeof(ShaderInputs) % 16 == 0 &
 template <int TW, int TH> struct DeferLds {
 	using S = DeferShape<TW, TH>;
 	static constexpr int32_t PAIRS = S::CHUNK * (TH / 2);  // (command, row pair) slots of a chunk
+	// Pass-1 items are found by table up to one per pixel per chunk (by binary search beyond).
+	static constexpr int32_t ITEM_TABLE = TW * TH;
+	using PairIndex = typename std::conditional<(PAIRS <= 256), uint8_t, uint16_t>::type;
 	int32_t list[S::LIST];                       // the round's commands, sorted
 	union {
 		TriangleRows tr[S::CHUNK];               // the chunk's commands: row setup (until the rows are made),
@@ -1021,10 +1026,12 @@ template <int TW, int TH> struct DeferLds {
 		struct {                                 // until pass 1 ends:
 			uint8_t pairStart[PAIRS];            // first quad column of each pair, from the tile's left
 			int16_t pairOffset[PAIRS + 1];       // exclusive prefix of quads per pair
+			PairIndex pairOfItem[ITEM_TABLE];    // the pair of each of the first ITEM_TABLE pass-1 items
 		};
 		struct {                                 // from the winners on:
 			uint32_t quadWinners[(TW / 2) * (TH / 2)];   // distinct winning chunk commands of a quad, 8 bits each
 			int16_t quadOffset[(TW / 2) * (TH / 2) + 1]; // exclusive prefix of distinct winners per quad
+			uint8_t quadOfShade[TW * TH];                // the quad of each pass-2 item (at most four per quad)
 		};
 	};
 	uint32_t color[TW * TH];                     // the tile's canvas pixels as they stand
@@ -1084,6 +1091,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	constexpr int32_t NT = S::NT, CHUNK = S::CHUNK;
 	constexpr int32_t NP = TW * TH, RP = TH / 2;
 	constexpr int32_t PAIRS = DeferLds<TW, TH>::PAIRS;
+	constexpr int32_t ITEM_TABLE = DeferLds<TW, TH>::ITEM_TABLE;
 	constexpr int32_t PAIRS_PER_THREAD = PAIRS / NT > 0 ? PAIRS / NT : 1;
 	constexpr int32_t NQX = TW / 2, NQ = (TW / 2) * (TH / 2);
 	constexpr int32_t QUADS_PER_THREAD = (NQ + NT - 1) / NT;
@@ -1158,13 +1166,15 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 			for (int32_t k = 0; k < PAIRS_PER_THREAD; k++) {
 				const int32_t pr = t * PAIRS_PER_THREAD + k;
 				if (pr < PAIRS) s.pairOffset[pr] = (int16_t)before;
+				for (int32_t i = before; i < before + pairCount[k] && i < ITEM_TABLE; i++)
+					s.pairOfItem[i] = (typename DeferLds<TW, TH>::PairIndex)pr;
 				before += pairCount[k];
 			}
 			if (t == 0) s.pairOffset[PAIRS] = (int16_t)items;
 			__syncthreads();
 			// Pass 1: one (command, quad) item per thread offers the covered lanes' keys.
 			for (int32_t it = t; it < items; it += NT) {
-				const int32_t pr = segmentOf(s.pairOffset, PAIRS, it);
+				const int32_t pr = it < ITEM_TABLE ? (int32_t)s.pairOfItem[it] : segmentOf(s.pairOffset, PAIRS, it);
 				const int32_t j = pr / RP, rp = pr % RP;
 				const int32_t px = tileX0 + s.pairStart[pr] + 2 * (it - s.pairOffset[pr]);
 				const int32_t py = tileY0 + 2 * rp;
@@ -1209,6 +1219,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 			for (int32_t k = 0; k < QUADS_PER_THREAD; k++) {
 				const int32_t qi = t * QUADS_PER_THREAD + k;
 				if (qi < NQ) s.quadOffset[qi] = (int16_t)before;
+				for (int32_t i = 0; i < quadCount[k]; i++) s.quadOfShade[before + i] = (uint8_t)qi;
 				before += quadCount[k];
 			}
 			if (t == 0) s.quadOffset[NQ] = (int16_t)shades;
@@ -1216,7 +1227,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 			// Pass 2: one (quad, winning command) item per thread: the quad's weights and mip level
 			// once, then the lanes the command won.
 			for (int32_t it = t; it < shades; it += NT) {
-				const int32_t qi = segmentOf(s.quadOffset, NQ, it);
+				const int32_t qi = s.quadOfShade[it];
 				const int32_t jw = (int32_t)((s.quadWinners[qi] >> (8 * (it - s.quadOffset[qi]))) & 0xFFu);
 				const int32_t qx = (qi % NQX) * 2, qy = (qi / NQX) * 2;
 				QuadEval e;
