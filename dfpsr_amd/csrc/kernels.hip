@@ -516,10 +516,19 @@ struct QuadEval {
 };
 
 // n steps of both rows' weights by d, one rounded addition at a time as the reference steps
-// them (n <= 0: none).
+// them (n <= 0: none).  Four steps per trip keep the loop control of lanes with different n
+// off most steps; the additions and their order are the same.
 __device__ __forceinline__ void stepPair(f3 &wU, f3 &wL, const f3 &d, int32_t n) {
+	int32_t i = 0;
 #pragma clang loop unroll(disable)
-	for (int32_t i = 0; i < n; i++) { wU = add3(wU, d); wL = add3(wL, d); }
+	for (; i + 4 <= n; i += 4) {
+		wU = add3(wU, d); wL = add3(wL, d);
+		wU = add3(wU, d); wL = add3(wL, d);
+		wU = add3(wU, d); wL = add3(wL, d);
+		wU = add3(wU, d); wL = add3(wL, d);
+	}
+#pragma clang loop unroll(disable)
+	for (; i < n; i++) { wU = add3(wU, d); wL = add3(wL, d); }
 }
 
 // upper/lower: row intervals of rows py and py + 1 (py even).  Returns false when the command
@@ -553,8 +562,14 @@ __device__ __forceinline__ bool evalQuad(const Projection &cmd, RowInterval uppe
 		e.l0[0] = wU.x; e.l0[1] = wU.x + dx.x; e.l0[2] = wL.x; e.l0[3] = wL.x + dx.x;
 		e.l1[0] = wU.y; e.l1[1] = wU.y + dx.y; e.l1[2] = wL.y; e.l1[3] = wL.y + dx.y;
 		e.l2[0] = wU.z; e.l2[1] = wU.z + dx.z; e.l2[2] = wL.z; e.l2[3] = wL.z + dx.z;
+		int32_t x = innerBlockStart;
 #pragma clang loop unroll(disable)
-		for (int32_t x = innerBlockStart; x < px; x += 2) {
+		for (; x + 4 <= px; x += 4) {
+			for (int l = 0; l < 4; l++) { e.l0[l] = e.l0[l] + dx2.x; e.l1[l] = e.l1[l] + dx2.y; e.l2[l] = e.l2[l] + dx2.z; }
+			for (int l = 0; l < 4; l++) { e.l0[l] = e.l0[l] + dx2.x; e.l1[l] = e.l1[l] + dx2.y; e.l2[l] = e.l2[l] + dx2.z; }
+		}
+#pragma clang loop unroll(disable)
+		for (; x < px; x += 2) {
 			for (int l = 0; l < 4; l++) { e.l0[l] = e.l0[l] + dx2.x; e.l1[l] = e.l1[l] + dx2.y; e.l2[l] = e.l2[l] + dx2.z; }
 		}
 	} else {
