@@ -156,7 +156,7 @@ private:
 	// The visibility raster reproduces the in-order result when every command is Solid and
 	// perspective and no canvas has an odd height.
 	bool deferredRaster(const RenderOptions &o) const {
-		return o.mode == 0 && o.rasterPath == 0 && !alphaAny_ && !orthogonalAny_ && !oddCanvas_ && !wideCanvas_;
+		return o.rasterPath == 0 && !alphaAny_ && !orthogonalAny_ && !oddCanvas_ && !wideCanvas_;
 	}
 };
 

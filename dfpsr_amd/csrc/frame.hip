@@ -166,7 +166,8 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	const int32_t persistent = o.persistentPerCU > 0 ? 1 : 0;
 	const int32_t grid = persistent ? deviceCUs_ * o.persistentPerCU : xcdGrid(tileTotal);
 	if (o.mode == 1) {
-		hipLaunchKernelGGL(k_raster_diff, dim3(grid), dim3(64), 0, stream, f, persistent);
+		if (deferredRaster(o)) hipLaunchKernelGGL(k_raster_deferred_diff, dim3(grid), dim3(64), 0, stream, f, persistent);
+		else hipLaunchKernelGGL(k_raster_diff, dim3(grid), dim3(64), 0, stream, f, persistent);
 	} else if (deferredRaster(o)) {
 		switch (tileConfig_) {
 		case 1: hipLaunchKernelGGL((k_raster_deferred<32, 16>), dim3(grid), dim3(32 * 16 / 4), 0, stream, f, persistent); break;
@@ -286,7 +287,7 @@ int Frame::run(const RenderOptions &o) {
 		f.tileW = tileConfigW(tileConfig_);
 		f.tileH = tileConfigH(tileConfig_);
 		f.alphaAny = alphaAny_ ? 1 : 0;
-		f.clearInRaster = (o.clear && deferredRaster(o)) ? 1 : 0;
+		f.clearInRaster = (o.clear && o.mode == 0 && deferredRaster(o)) ? 1 : 0;
 		f.clearColor = o.clearColor;
 		f.clearDepth = o.clearDepth;
 		f.mode = o.mode;

@@ -733,8 +733,8 @@ struct DiffInputs {
 	int32_t point[3];      // model point of each corner (per-point multipliers)
 };
 
-__device__ __forceinline__ DiffInputs diffInputs(const TaskDev &task, const Command &cmd) {
-	int32_t fp = task.fanPolygon[cmd.fan];
+__device__ __forceinline__ DiffInputs diffInputs(const TaskDev &task, int32_t fan) {
+	int32_t fp = task.fanPolygon[fan];
 	int32_t poly = fp >> 1, k = fp & 1;
 	const float *tc = task.texcoords + poly * 16;
 	const int32_t ca = 0, cb = (1 + k) * 4, cc = (2 + k) * 4;
@@ -756,7 +756,7 @@ __device__ void drawCommandDiff(const FrameDev &f, const Command &cmd, int32_t c
 	const bool affine = (cmd.flags & 1) != 0;
 	bool vis[4];
 	if (!quadVisibility<1>(e, q, depthRead, affine, vis)) return;
-	const DiffInputs d = diffInputs(task, cmd);
+	const DiffInputs d = diffInputs(task, cmd.fan);
 	float m[3][3];
 	for (int c = 0; c < 3; c++)
 		for (int ch = 0; ch < 3; ch++) m[c][ch] = multiplier(f, d.point[c], ch);
@@ -1022,6 +1022,41 @@ static_assert(offsetof(Command, start) == COMMAND_TAIL_OFFSET && offsetof(Comman
 static_assert(sizeof(TriangleRows) % 16 == 0 && sizeof(ShaderInputs) % 16 == 0 && sizeof(CommandTail) % 16 == 0,
               "staged records are whole 16-byte units");
 
+// The differentiable shader (drawCommandDiff) for the lanes in mask of one command's quad at
+// (px, py), written straight to the float colour canvas with the visibility id base + lane.
+__device__ void shadeDiffLanes(const FrameDev &f, const CommandTail &ct, const QuadEval &e, uint32_t mask, uint32_t slots,
+                               int32_t idBase, const CanvasDev &cv, int32_t px, int32_t py) {
+	const bool affine = (ct.flags & 1) != 0;
+	const TaskDev &task = f.tasks[ct.task];
+	const DiffInputs d = diffInputs(task, ct.fan);
+	float m[3][3];
+	for (int c = 0; c < 3; c++)
+		for (int ch = 0; ch < 3; ch++) m[c][ch] = multiplier(f, d.point[c], ch);
+	for (int l = 0; l < 4; l++) {
+		if (!((mask >> l) & 1u)) continue;
+		const int32_t sl = (int32_t)((slots >> (2 * l)) & 3u);
+		const int32_t x = px + (sl & 1), y = py + (sl >> 1);
+		if (x >= cv.width || y >= cv.height) continue;
+		float wa, wb, wc;
+		laneWeights(e, l, affine, wa, wb, wc);
+		const float u = interp(d.u, wa, wb, wc), v = interp(d.v, wa, wb, wc);
+		const BilinearTaps taps = bilinearTaps(f.ftexLog2w, f.ftexLog2h, u, v);
+		float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+		for (int k = 0; k < 4; k++) {
+			const float4 texel = *(const float4 *)(f.ftexTexels + (size_t)taps.index[k] * 4);
+			t[0] += taps.weight[k] * texel.x;
+			t[1] += taps.weight[k] * texel.y;
+			t[2] += taps.weight[k] * texel.z;
+			t[3] += taps.weight[k] * texel.w;
+		}
+		const size_t p = (size_t)y * cv.diffStride + x;
+		*(float4 *)(cv.diffColor + p * 4) = make_float4((wa * m[0][0] + wb * m[1][0] + wc * m[2][0]) * t[0],
+		                                                (wa * m[0][1] + wb * m[1][1] + wc * m[2][1]) * t[1],
+		                                                (wa * m[0][2] + wb * m[1][2] + wc * m[2][2]) * t[2], t[3]);
+		cv.visibility[p] = idBase + l;
+	}
+}
+
 // LDS of one visibility-raster workgroup.
 template <int TW, int TH> struct DeferLds {
 	using S = DeferShape<TW, TH>;
@@ -1100,7 +1135,9 @@ __device__ __forceinline__ int32_t segmentOf(const T *offset, int32_t count, int
 	return lo;
 }
 
-template <int TW, int TH>
+// MODE 0: the RGBA8 shaders into the colour canvas.  MODE 1: the differentiable float shader
+// into the float colour canvas plus the visibility id of each pixel's winning lane.
+template <int TW, int TH, int MODE = 0>
 __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW, TH> &s, const TextureDesc *textures) {
 	using S = DeferShape<TW, TH>;
 	constexpr int32_t NT = S::NT, CHUNK = S::CHUNK;
@@ -1118,7 +1155,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	const int32_t tileY0 = (local / cv.tilesX) * TH;
 	const int32_t t = threadIdx.x;
 	const int32_t *src = tileListOf(f, tile);
-	const bool colorWrite = cv.color != nullptr;
+	const bool colorWrite = MODE == 0 && cv.color != nullptr;
 	const bool depthRead = cv.depth != nullptr;
 
 	for (int32_t p = t; p < NP; p += NT) {
@@ -1266,6 +1303,10 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 					if (!f.clearInRaster) atomicOr(&s.dirty[p >> 5], 1u << (p & 31));
 					slots |= (uint32_t)sl << (2 * l);
 				}
+				if (MODE == 1 && mask) {
+					const int32_t id = (s.list[c0 + jw] - f.canvasCommandBase[ci]) * 4;
+					shadeDiffLanes(f, s.tail[jw], e, mask, slots, id, cv, tileX0 + qx, tileY0 + qy);
+				}
 				if (colorWrite && mask) {
 					const bool affine = (s.tail[jw].flags & 1) != 0;
 					float wa[4], wb[4], wc[4];
@@ -1316,6 +1357,14 @@ __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_e
 }
 
 template __global__ void k_raster_deferred<16, 16>(FrameDev, int32_t);
+
+// The differentiable forward by the visibility raster, 16x16 tiles (the adjoint's tiles).
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred_diff(FrameDev f, int32_t persistent) {
+	__shared__ DeferLds<16, 16> s;
+	reportCounters(f);
+	if (f.counters[2]) return;
+	forEachQueuedTile(f, persistent, &s.slot, [&](int32_t tile) { rasterTileDeferred<16, 16, 1>(f, tile, s, nullptr); });
+}
 template __global__ void k_raster_deferred<32, 16>(FrameDev, int32_t);
 template __global__ void k_raster_deferred<32, 32>(FrameDev, int32_t);
 
@@ -1364,7 +1413,7 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 		const bool affine = (cmd.flags & 1) != 0;
 		float wa, wbw, wc;
 		laneWeights(e, lane, affine, wa, wbw, wc);
-		const DiffInputs d = diffInputs(task, cmd);
+		const DiffInputs d = diffInputs(task, cmd.fan);
 		float u = interp(d.u, wa, wbw, wc), v = interp(d.v, wa, wbw, wc);
 		BilinearTaps taps = bilinearTaps(f.ftexLog2w, f.ftexLog2h, u, v);
 		float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};

@@ -82,6 +82,28 @@ def test_forward_matches_oracle(api):
         assert (vis >= 0).mean() > 0.3
 
 
+def test_forward_visibility_raster_equals_in_order_raster(api):
+    """The differentiable forward through the visibility raster (winner per pixel by atomic max,
+    shaded once) writes the same float colours, visibility ids and depths, bit for bit, as the
+    in-order tile walk."""
+    views = small_views()
+    tex = torch.from_numpy(S.float_texture(64, 5)).cuda()
+    mul = torch.from_numpy((1.0 + 0.3 * S.Rng(10).uniform(views[0].points.shape[0] * 3, -1, 1)).reshape(-1, 3)
+                           .astype(np.float32)).cuda()
+    out = []
+    for path in (0, 1):
+        gv = api.GpuViews(views)
+        r = api.DiffRenderer()
+        r.forward(gv.model, gv.views, tex, mul, api.tuning(clear=True, raster_path=path))
+        torch.cuda.synchronize()
+        out.append([(gv.color[i].cpu().numpy(), gv.visibility[i].cpu().numpy(), gv.depth[i].cpu().numpy())
+                    for i in range(len(views))])
+    for i in range(len(views)):
+        for a, b in zip(out[0][i], out[1][i]):
+            assert np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                                  b.view(np.uint32) if b.dtype == np.float32 else b), f"view {i}"
+
+
 def test_adjoint_matches_oracle_and_dot_product(api):
     views = small_views()
     gv = api.GpuViews(views)
