@@ -161,26 +161,46 @@ def differentiable_form(steps: int = 10) -> dict:
         for g in gv.grad:
             g.fill_(1e-3)
         r = api.DiffRenderer()
-        t = api.tuning(clear=True, persistent_per_cu=4, use_graph=True)
+
+        def timed(fn, count):
+            torch.cuda.synchronize()
+            start = time.perf_counter()
+            for _ in range(count):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - start) / count
+
+        # Untimed choice of persistent workgroups per CU (0 = one per tile) for each pass; the
+        # forward's choice is kept while the adjoint's is tried, since the adjoint reads its output.
+        per_cu = {}
+        for stage in ("forward", "backward"):
+            best = None
+            for k in (0, 4):
+                t = api.tuning(clear=True, persistent_per_cu=per_cu.get("forward", k), use_graph=True)
+                tb = api.tuning(clear=True, persistent_per_cu=k, use_graph=True)
+                r.forward(gv.model, gv.views, tex, mul, t)
+                if stage == "forward":
+                    ms = timed(lambda: r.forward(gv.model, gv.views, tex, mul, t), 3)
+                else:
+                    r.backward(gt, gm, tb)
+                    ms = timed(lambda: r.backward(gt, gm, tb), 3)
+                if best is None or ms < best[0]:
+                    best = (ms, k)
+            per_cu[stage] = best[1]
+        t_fwd = api.tuning(clear=True, persistent_per_cu=per_cu["forward"], use_graph=True)
+        t_bwd = api.tuning(clear=True, persistent_per_cu=per_cu["backward"], use_graph=True)
         for _ in range(2):
-            r.forward(gv.model, gv.views, tex, mul, t)
-            r.backward(gt, gm, t)
-        torch.cuda.synchronize()
-        start = time.perf_counter()
-        for _ in range(steps):
-            r.forward(gv.model, gv.views, tex, mul, t)
-        torch.cuda.synchronize()
-        fwd = (time.perf_counter() - start) / steps
-        start = time.perf_counter()
-        for _ in range(steps):
-            r.backward(gt, gm, t)
-        torch.cuda.synchronize()
-        bwd = (time.perf_counter() - start) / steps
+            r.forward(gv.model, gv.views, tex, mul, t_fwd)
+            r.backward(gt, gm, t_bwd)
+        fwd = timed(lambda: r.forward(gv.model, gv.views, tex, mul, t_fwd), steps)
+        bwd = timed(lambda: r.backward(gt, gm, t_bwd), steps)
         pixels = sum(v.width * v.height for v in views)
         progress(f"differentiable {key}: forward {fwd * 1e3:.3f} ms, backward {bwd * 1e3:.3f} ms")
         out[key] = {"views": len(views), "triangles": int(views[0].triangle_count), "texture": tex_size,
                     "forward_ms": round(fwd * 1e3, 3), "backward_ms": round(bwd * 1e3, 3),
-                    "step_mpix_per_s": round(pixels / (fwd + bwd) / 1e6, 1)}
+                    "step_ms": round((fwd + bwd) * 1e3, 3),
+                    "step_mpix_per_s": round(pixels / (fwd + bwd) / 1e6, 1),
+                    "persistent_per_cu": per_cu}
     return out
 
 

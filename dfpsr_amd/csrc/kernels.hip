@@ -1383,65 +1383,100 @@ __global__ void __launch_bounds__(64) k_raster_diff(FrameDev f, int32_t persiste
 // interpolants of the (command, lane) that drew it -- the same rows and the same weight
 // stepping as the forward -- and scatters dL/dcolour into the four bilinear texels and the
 // three corners' RGB multipliers with float atomics.  Workgroups pull tiles from a queue
-// (all tiles of all views) until it is empty.
+// (all tiles of all views) until it is empty.  The multiplier gradients of the pixels of one
+// command are first added up in LDS (a small open-addressed table of the tile's commands), so
+// a tile makes one global atomic per (command, corner, channel) instead of one per pixel;
+// commands beyond the table's room add theirs directly.
+constexpr int32_t BACKWARD_SLOTS = 64;
+
 __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 	__shared__ int32_t tileLds;
+	__shared__ int32_t slotCommand[BACKWARD_SLOTS];
+	__shared__ int32_t slotPoint[BACKWARD_SLOTS][3];
+	__shared__ float slotGrad[BACKWARD_SLOTS][9];
+	const int32_t t = (int32_t)threadIdx.x;
 	while (true) {
-		if (threadIdx.x == 0) tileLds = atomicAdd(head, 1);
+		if (t == 0) tileLds = atomicAdd(head, 1);
+		if (t < BACKWARD_SLOTS) slotCommand[t] = -1;
+		for (int32_t i = t; i < BACKWARD_SLOTS * 9; i += 256) (&slotGrad[0][0])[i] = 0.0f;
 		__syncthreads();
 		const int32_t tile = tileLds;
-		__syncthreads();
 		if (tile >= f.tileTotal) break;
-		const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
-		const CanvasDev cv = f.canvases[ci];
-		const int32_t local = tile - cv.tileBase;
-		const int32_t x = (local % cv.tilesX) * f.tileW + (int32_t)(threadIdx.x % 16);
-		const int32_t y = (local / cv.tilesX) * f.tileH + (int32_t)(threadIdx.x / 16);
-		if (x >= cv.width || y >= cv.height) continue;
-		const size_t p = (size_t)y * cv.diffStride + x;
-		const int32_t id = cv.visibility[p];
-		if (id < 0) continue;
-		const int32_t c = (id >> 2) + f.canvasCommandBase[ci], lane = id & 3;
-		const Command cmd = f.commands[c];
-		const TaskDev &task = f.tasks[cmd.task];
-		const TriangleRows &tr = f.triangleRows[c];
-		const int32_t py = y & ~1, px = x & ~1;
-		RowInterval upper = triangleRow(tr, py);
-		RowInterval lower = triangleRow(tr, py + 1);
-		QuadEval e;
-		if (!evalQuad(cmd, upper, lower, px, py, cv.height, e)) continue;
-		const bool affine = (cmd.flags & 1) != 0;
-		float wa, wbw, wc;
-		laneWeights(e, lane, affine, wa, wbw, wc);
-		const DiffInputs d = diffInputs(task, cmd.fan);
-		float u = interp(d.u, wa, wbw, wc), v = interp(d.v, wa, wbw, wc);
-		BilinearTaps taps = bilinearTaps(f.ftexLog2w, f.ftexLog2h, u, v);
-		float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-		for (int k = 0; k < 4; k++) {
-			const float4 texel = *(const float4 *)(f.ftexTexels + (size_t)taps.index[k] * 4);
-			t[0] += taps.weight[k] * texel.x;
-			t[1] += taps.weight[k] * texel.y;
-			t[2] += taps.weight[k] * texel.z;
-			t[3] += taps.weight[k] * texel.w;
-		}
-		const float4 g = *(const float4 *)(cv.gradColor + p * 4);
-		const float w3[3] = {wa, wbw, wc};
-		float mixed[3];
-		for (int ch = 0; ch < 3; ch++)
-			mixed[ch] = wa * multiplier(f, d.point[0], ch) + wbw * multiplier(f, d.point[1], ch) + wc * multiplier(f, d.point[2], ch);
-		if (f.gradTexture) {
-			const float gt[4] = {g.x * mixed[0], g.y * mixed[1], g.z * mixed[2], g.w};
+		do {
+			const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
+			const CanvasDev cv = f.canvases[ci];
+			const int32_t local = tile - cv.tileBase;
+			const int32_t x = (local % cv.tilesX) * f.tileW + t % 16;
+			const int32_t y = (local / cv.tilesX) * f.tileH + t / 16;
+			if (x >= cv.width || y >= cv.height) break;
+			const size_t p = (size_t)y * cv.diffStride + x;
+			const int32_t id = cv.visibility[p];
+			if (id < 0) break;
+			const int32_t c = (id >> 2) + f.canvasCommandBase[ci], lane = id & 3;
+			const Command cmd = f.commands[c];
+			const TaskDev &task = f.tasks[cmd.task];
+			const TriangleRows &tr = f.triangleRows[c];
+			const int32_t py = y & ~1, px = x & ~1;
+			RowInterval upper = triangleRow(tr, py);
+			RowInterval lower = triangleRow(tr, py + 1);
+			QuadEval e;
+			if (!evalQuad(cmd, upper, lower, px, py, cv.height, e)) break;
+			const bool affine = (cmd.flags & 1) != 0;
+			float wa, wbw, wc;
+			laneWeights(e, lane, affine, wa, wbw, wc);
+			const DiffInputs d = diffInputs(task, cmd.fan);
+			float u = interp(d.u, wa, wbw, wc), v = interp(d.v, wa, wbw, wc);
+			BilinearTaps taps = bilinearTaps(f.ftexLog2w, f.ftexLog2h, u, v);
+			float tx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 			for (int k = 0; k < 4; k++) {
-				if (taps.weight[k] == 0.0f) continue;
-				float *dst = f.gradTexture + (size_t)taps.index[k] * 4;
-				for (int ch = 0; ch < 4; ch++) unsafeAtomicAdd(dst + ch, gt[ch] * taps.weight[k]);
+				const float4 texel = *(const float4 *)(f.ftexTexels + (size_t)taps.index[k] * 4);
+				tx[0] += taps.weight[k] * texel.x;
+				tx[1] += taps.weight[k] * texel.y;
+				tx[2] += taps.weight[k] * texel.z;
+				tx[3] += taps.weight[k] * texel.w;
 			}
-		}
-		if (f.gradMultipliers) {
-			const float gm[3] = {g.x * t[0], g.y * t[1], g.z * t[2]};
+			const float4 g = *(const float4 *)(cv.gradColor + p * 4);
+			const float w3[3] = {wa, wbw, wc};
+			float mixed[3];
+			for (int ch = 0; ch < 3; ch++)
+				mixed[ch] = wa * multiplier(f, d.point[0], ch) + wbw * multiplier(f, d.point[1], ch) + wc * multiplier(f, d.point[2], ch);
+			if (f.gradTexture) {
+				const float gt[4] = {g.x * mixed[0], g.y * mixed[1], g.z * mixed[2], g.w};
+				for (int k = 0; k < 4; k++) {
+					if (taps.weight[k] == 0.0f) continue;
+					float *dst = f.gradTexture + (size_t)taps.index[k] * 4;
+					for (int ch = 0; ch < 4; ch++) unsafeAtomicAdd(dst + ch, gt[ch] * taps.weight[k]);
+				}
+			}
+			if (f.gradMultipliers) {
+				const float gm[3] = {g.x * tx[0], g.y * tx[1], g.z * tx[2]};
+				int32_t slot = -1;
+				const int32_t h = (int32_t)(((uint32_t)c * 2654435761u) >> 26);
+				for (int32_t k = 0; k < BACKWARD_SLOTS; k++) {
+					const int32_t i = (h + k) & (BACKWARD_SLOTS - 1);
+					const int32_t old = atomicCAS(&slotCommand[i], -1, c);
+					if (old == -1) {
+						for (int c3 = 0; c3 < 3; c3++) slotPoint[i][c3] = d.point[c3];
+						slot = i;
+						break;
+					}
+					if (old == c) {
+						slot = i;
+						break;
+					}
+				}
+				for (int c3 = 0; c3 < 3; c3++)
+					for (int ch = 0; ch < 3; ch++) {
+						if (slot >= 0) atomicAdd(&slotGrad[slot][c3 * 3 + ch], gm[ch] * w3[c3]);
+						else unsafeAtomicAdd(f.gradMultipliers + d.point[c3] * 3 + ch, gm[ch] * w3[c3]);
+					}
+			}
+		} while (false);
+		__syncthreads();
+		if (f.gradMultipliers && t < BACKWARD_SLOTS && slotCommand[t] >= 0)
 			for (int c3 = 0; c3 < 3; c3++)
-				for (int ch = 0; ch < 3; ch++) unsafeAtomicAdd(f.gradMultipliers + d.point[c3] * 3 + ch, gm[ch] * w3[c3]);
-		}
+				for (int ch = 0; ch < 3; ch++) unsafeAtomicAdd(f.gradMultipliers + slotPoint[t][c3] * 3 + ch, slotGrad[t][c3 * 3 + ch]);
+		__syncthreads();
 	}
 }
 
