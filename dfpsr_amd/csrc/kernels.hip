@@ -1386,19 +1386,28 @@ __global__ void __launch_bounds__(64) k_raster_diff(FrameDev f, int32_t persiste
 // (all tiles of all views) until it is empty.  The multiplier gradients of the pixels of one
 // command are first added up in LDS (a small open-addressed table of the tile's commands), so
 // a tile makes one global atomic per (command, corner, channel) instead of one per pixel;
-// commands beyond the table's room add theirs directly.
+// commands beyond the table's room add theirs directly.  Texture gradients are gathered the
+// same way per texel (neighbouring pixels of a magnified texture share their taps).
 constexpr int32_t BACKWARD_SLOTS = 64;
+constexpr int32_t TEXEL_SLOTS = 256;
+constexpr int32_t TEXEL_PROBES = 8;
 
 __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 	__shared__ int32_t tileLds;
 	__shared__ int32_t slotCommand[BACKWARD_SLOTS];
 	__shared__ int32_t slotPoint[BACKWARD_SLOTS][3];
 	__shared__ float slotGrad[BACKWARD_SLOTS][9];
+	__shared__ int32_t texelKey[TEXEL_SLOTS];
+	__shared__ float4 texelGrad[TEXEL_SLOTS];
 	const int32_t t = (int32_t)threadIdx.x;
 	while (true) {
 		if (t == 0) tileLds = atomicAdd(head, 1);
 		if (t < BACKWARD_SLOTS) slotCommand[t] = -1;
 		for (int32_t i = t; i < BACKWARD_SLOTS * 9; i += 256) (&slotGrad[0][0])[i] = 0.0f;
+		for (int32_t i = t; i < TEXEL_SLOTS; i += 256) {
+			texelKey[i] = -1;
+			texelGrad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		}
 		__syncthreads();
 		const int32_t tile = tileLds;
 		if (tile >= f.tileTotal) break;
@@ -1444,8 +1453,24 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 				const float gt[4] = {g.x * mixed[0], g.y * mixed[1], g.z * mixed[2], g.w};
 				for (int k = 0; k < 4; k++) {
 					if (taps.weight[k] == 0.0f) continue;
-					float *dst = f.gradTexture + (size_t)taps.index[k] * 4;
-					for (int ch = 0; ch < 4; ch++) unsafeAtomicAdd(dst + ch, gt[ch] * taps.weight[k]);
+					const int32_t texel = (int32_t)taps.index[k];
+					int32_t slot = -1;
+					const int32_t h = (int32_t)(((uint32_t)texel * 2654435761u) >> 24);
+					for (int32_t q = 0; q < TEXEL_PROBES; q++) {
+						const int32_t i = (h + q) & (TEXEL_SLOTS - 1);
+						const int32_t old = atomicCAS(&texelKey[i], -1, texel);
+						if (old == -1 || old == texel) {
+							slot = i;
+							break;
+						}
+					}
+					if (slot >= 0) {
+						float *acc = (float *)&texelGrad[slot];
+						for (int ch = 0; ch < 4; ch++) atomicAdd(acc + ch, gt[ch] * taps.weight[k]);
+					} else {
+						float *dst = f.gradTexture + (size_t)texel * 4;
+						for (int ch = 0; ch < 4; ch++) unsafeAtomicAdd(dst + ch, gt[ch] * taps.weight[k]);
+					}
 				}
 			}
 			if (f.gradMultipliers) {
@@ -1473,6 +1498,17 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 			}
 		} while (false);
 		__syncthreads();
+		if (f.gradTexture)
+			for (int32_t i = t; i < TEXEL_SLOTS; i += 256) {
+				const int32_t texel = texelKey[i];
+				if (texel < 0) continue;
+				const float4 gsum = texelGrad[i];
+				float *dst = f.gradTexture + (size_t)texel * 4;
+				unsafeAtomicAdd(dst + 0, gsum.x);
+				unsafeAtomicAdd(dst + 1, gsum.y);
+				unsafeAtomicAdd(dst + 2, gsum.z);
+				unsafeAtomicAdd(dst + 3, gsum.w);
+			}
 		if (f.gradMultipliers && t < BACKWARD_SLOTS && slotCommand[t] >= 0)
 			for (int c3 = 0; c3 < 3; c3++)
 				for (int ch = 0; ch < 3; ch++) unsafeAtomicAdd(f.gradMultipliers + slotPoint[t][c3] * 3 + ch, slotGrad[t][c3 * 3 + ch]);
