@@ -170,25 +170,35 @@ def differentiable_form(steps: int = 10) -> dict:
             torch.cuda.synchronize()
             return (time.perf_counter() - start) / count
 
-        # Untimed choice of persistent workgroups per CU (0 = one per tile) for each pass; the
-        # forward's choice is kept while the adjoint's is tried, since the adjoint reads its output.
-        per_cu = {}
-        for stage in ("forward", "backward"):
-            best = None
-            for k in (0, 4):
-                t = api.tuning(clear=True, persistent_per_cu=per_cu.get("forward", k), use_graph=True)
-                tb = api.tuning(clear=True, persistent_per_cu=k, use_graph=True)
-                r.forward(gv.model, gv.views, tex, mul, t)
-                if stage == "forward":
-                    ms = timed(lambda: r.forward(gv.model, gv.views, tex, mul, t), 3)
-                else:
-                    r.backward(gt, gm, tb)
-                    ms = timed(lambda: r.backward(gt, gm, tb), 3)
-                if best is None or ms < best[0]:
-                    best = (ms, k)
-            per_cu[stage] = best[1]
-        t_fwd = api.tuning(clear=True, persistent_per_cu=per_cu["forward"], use_graph=True)
-        t_bwd = api.tuning(clear=True, persistent_per_cu=per_cu["backward"], use_graph=True)
+        # Untimed choice of the tile shape, and for each pass of the persistent workgroups per CU
+        # (0 = one per tile): per tile shape the forward's best setting, then the adjoint's on that
+        # forward's output; the shape with the shortest step wins.
+        best = None
+        for tile_config in (0, 1, 2):
+            choice = {}
+            total = 0.0
+            for stage in ("forward", "backward"):
+                stage_best = None
+                for k in (0, 4):
+                    tk = api.tuning(clear=True, tile_config=tile_config, persistent_per_cu=k, use_graph=True)
+                    if stage == "forward":
+                        r.forward(gv.model, gv.views, tex, mul, tk)
+                        ms = timed(lambda: r.forward(gv.model, gv.views, tex, mul, tk), 3)
+                    else:
+                        r.backward(gt, gm, tk)
+                        ms = timed(lambda: r.backward(gt, gm, tk), 3)
+                    if stage_best is None or ms < stage_best[0]:
+                        stage_best = (ms, k)
+                choice[stage] = stage_best[1]
+                total += stage_best[0]
+                if stage == "forward":  # the adjoint reads this forward's output
+                    r.forward(gv.model, gv.views, tex, mul,
+                              api.tuning(clear=True, tile_config=tile_config, persistent_per_cu=choice["forward"], use_graph=True))
+            if best is None or total < best[0]:
+                best = (total, tile_config, choice)
+        tile_config, per_cu = best[1], best[2]
+        t_fwd = api.tuning(clear=True, tile_config=tile_config, persistent_per_cu=per_cu["forward"], use_graph=True)
+        t_bwd = api.tuning(clear=True, tile_config=tile_config, persistent_per_cu=per_cu["backward"], use_graph=True)
         for _ in range(2):
             r.forward(gv.model, gv.views, tex, mul, t_fwd)
             r.backward(gt, gm, t_bwd)
@@ -200,7 +210,7 @@ def differentiable_form(steps: int = 10) -> dict:
                     "forward_ms": round(fwd * 1e3, 3), "backward_ms": round(bwd * 1e3, 3),
                     "step_ms": round((fwd + bwd) * 1e3, 3),
                     "step_mpix_per_s": round(pixels / (fwd + bwd) / 1e6, 1),
-                    "persistent_per_cu": per_cu}
+                    "tile": ["16x16", "32x16", "32x32"][tile_config], "persistent_per_cu": per_cu}
     return out
 
 

@@ -564,6 +564,7 @@ struct dfpsr_hip_diff {
 	Frame frame;
 	std::vector<dfpsr_hip_view> lastViews;
 	const dfpsr_hip_model *lastModel = nullptr;
+	int32_t lastTileConfig = -1;
 };
 
 extern "C" int dfpsr_hip_diff_create(dfpsr_hip_diff **out) {
@@ -587,13 +588,12 @@ extern "C" int dfpsr_hip_diff_forward(dfpsr_hip_diff *d, const dfpsr_hip_model *
 	int rc = checkTuning(tuning);
 	if (rc) return rc;
 	RenderOptions o = optionsFrom(tuning);
-	o.tileConfig = 0;
 	o.mode = 1;
-	bool same = d->lastModel == model && !model->dirty && (int32_t)d->lastViews.size() == count &&
+	bool same = d->lastModel == model && !model->dirty && d->lastTileConfig == o.tileConfig && (int32_t)d->lastViews.size() == count &&
 	            (count == 0 || std::memcmp(d->lastViews.data(), views, (size_t)count * sizeof(dfpsr_hip_view)) == 0);
 	d->frame.setDifferentiable(texture, log2w, log2h, multipliers);
 	if (!same) {
-		d->frame.reset(0);
+		d->frame.reset(o.tileConfig);
 		rc = const_cast<dfpsr_hip_model *>(model)->upload();
 		if (rc) return rc;
 		for (int32_t i = 0; i < count; i++) {
@@ -623,6 +623,7 @@ extern "C" int dfpsr_hip_diff_forward(dfpsr_hip_diff *d, const dfpsr_hip_model *
 		if (rc) return rc;
 		d->lastViews.assign(views, views + count);
 		d->lastModel = model;
+		d->lastTileConfig = o.tileConfig;
 	}
 	rc = d->frame.run(o);
 	if (rc) d->lastViews.clear();

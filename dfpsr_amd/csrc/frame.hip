@@ -166,8 +166,21 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	const int32_t persistent = o.persistentPerCU > 0 ? 1 : 0;
 	const int32_t grid = persistent ? deviceCUs_ * o.persistentPerCU : xcdGrid(tileTotal);
 	if (o.mode == 1) {
-		if (deferredRaster(o)) hipLaunchKernelGGL(k_raster_deferred_diff, dim3(grid), dim3(64), 0, stream, f, persistent);
-		else hipLaunchKernelGGL(k_raster_diff, dim3(grid), dim3(64), 0, stream, f, persistent);
+		const bool deferred = deferredRaster(o);
+		switch (tileConfig_) {
+		case 1:
+			if (deferred) hipLaunchKernelGGL((k_raster_deferred_diff<32, 16>), dim3(grid), dim3(128), 0, stream, f, persistent);
+			else hipLaunchKernelGGL((k_raster_diff<32, 16>), dim3(grid), dim3(128), 0, stream, f, persistent);
+			break;
+		case 2:
+			if (deferred) hipLaunchKernelGGL((k_raster_deferred_diff<32, 32>), dim3(grid), dim3(256), 0, stream, f, persistent);
+			else hipLaunchKernelGGL((k_raster_diff<32, 32>), dim3(grid), dim3(256), 0, stream, f, persistent);
+			break;
+		default:
+			if (deferred) hipLaunchKernelGGL((k_raster_deferred_diff<16, 16>), dim3(grid), dim3(64), 0, stream, f, persistent);
+			else hipLaunchKernelGGL((k_raster_diff<16, 16>), dim3(grid), dim3(64), 0, stream, f, persistent);
+			break;
+		}
 	} else if (deferredRaster(o)) {
 		switch (tileConfig_) {
 		case 1: hipLaunchKernelGGL((k_raster_deferred<32, 16>), dim3(grid), dim3(32 * 16 / 4), 0, stream, f, persistent); break;

@@ -1358,26 +1358,36 @@ __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_e
 
 template __global__ void k_raster_deferred<16, 16>(FrameDev, int32_t);
 
-// The differentiable forward by the visibility raster, 16x16 tiles (the adjoint's tiles).
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred_diff(FrameDev f, int32_t persistent) {
-	__shared__ DeferLds<16, 16> s;
+// The differentiable forward by the visibility raster.
+template <int TW, int TH>
+__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred_diff(FrameDev f, int32_t persistent) {
+	__shared__ DeferLds<TW, TH> s;
 	reportCounters(f);
 	if (f.counters[2]) return;
-	forEachQueuedTile(f, persistent, &s.slot, [&](int32_t tile) { rasterTileDeferred<16, 16, 1>(f, tile, s, nullptr); });
+	forEachQueuedTile(f, persistent, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH, 1>(f, tile, s, nullptr); });
 }
+
+template __global__ void k_raster_deferred_diff<16, 16>(FrameDev, int32_t);
+template __global__ void k_raster_deferred_diff<32, 16>(FrameDev, int32_t);
+template __global__ void k_raster_deferred_diff<32, 32>(FrameDev, int32_t);
 template __global__ void k_raster_deferred<32, 16>(FrameDev, int32_t);
 template __global__ void k_raster_deferred<32, 32>(FrameDev, int32_t);
 
 // The differentiable forward: the same tile walk with the float shader, persistent or not.
-__global__ void __launch_bounds__(64) k_raster_diff(FrameDev f, int32_t persistent) {
+template <int TW, int TH>
+__global__ void __launch_bounds__((TW / 2) * (TH / 2)) k_raster_diff(FrameDev f, int32_t persistent) {
 	__shared__ int32_t list[TILE_SORT_CAP];
-	__shared__ RowInterval rows[16];
+	__shared__ RowInterval rows[TH];
 	__shared__ int32_t roundInfo[3];
 	__shared__ int32_t slotLds;
 	reportCounters(f);
 	if (f.counters[2]) return;
-	forEachQueuedTile(f, persistent, &slotLds, [&](int32_t tile) { rasterTile<16, 16, 1>(f, tile, list, rows, roundInfo); });
+	forEachQueuedTile(f, persistent, &slotLds, [&](int32_t tile) { rasterTile<TW, TH, 1>(f, tile, list, rows, roundInfo); });
 }
+
+template __global__ void k_raster_diff<16, 16>(FrameDev, int32_t);
+template __global__ void k_raster_diff<32, 16>(FrameDev, int32_t);
+template __global__ void k_raster_diff<32, 32>(FrameDev, int32_t);
 
 // The adjoint of the differentiable forward with visibility fixed: each pixel re-derives the
 // interpolants of the (command, lane) that drew it -- the same rows and the same weight
@@ -1411,12 +1421,12 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 		__syncthreads();
 		const int32_t tile = tileLds;
 		if (tile >= f.tileTotal) break;
-		do {
-			const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
-			const CanvasDev cv = f.canvases[ci];
-			const int32_t local = tile - cv.tileBase;
-			const int32_t x = (local % cv.tilesX) * f.tileW + t % 16;
-			const int32_t y = (local / cv.tilesX) * f.tileH + t / 16;
+		const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
+		const CanvasDev cv = f.canvases[ci];
+		const int32_t local = tile - cv.tileBase;
+		for (int32_t pix = t; pix < f.tileW * f.tileH; pix += 256) do {
+			const int32_t x = (local % cv.tilesX) * f.tileW + pix % f.tileW;
+			const int32_t y = (local / cv.tilesX) * f.tileH + pix / f.tileW;
 			if (x >= cv.width || y >= cv.height) break;
 			const size_t p = (size_t)y * cv.diffStride + x;
 			const int32_t id = cv.visibility[p];

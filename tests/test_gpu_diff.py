@@ -85,23 +85,50 @@ def test_forward_matches_oracle(api):
 def test_forward_visibility_raster_equals_in_order_raster(api):
     """The differentiable forward through the visibility raster (winner per pixel by atomic max,
     shaded once) writes the same float colours, visibility ids and depths, bit for bit, as the
-    in-order tile walk."""
+    in-order tile walk, for every tile shape."""
     views = small_views()
     tex = torch.from_numpy(S.float_texture(64, 5)).cuda()
     mul = torch.from_numpy((1.0 + 0.3 * S.Rng(10).uniform(views[0].points.shape[0] * 3, -1, 1)).reshape(-1, 3)
                            .astype(np.float32)).cuda()
     out = []
-    for path in (0, 1):
+    for tile_config in (0, 1, 2):
+        for path in (0, 1):
+            gv = api.GpuViews(views)
+            r = api.DiffRenderer()
+            r.forward(gv.model, gv.views, tex, mul, api.tuning(clear=True, raster_path=path, tile_config=tile_config))
+            torch.cuda.synchronize()
+            out.append([(gv.color[i].cpu().numpy(), gv.visibility[i].cpu().numpy(), gv.depth[i].cpu().numpy())
+                        for i in range(len(views))])
+    for k in range(1, len(out)):
+        for i in range(len(views)):
+            for a, b in zip(out[0][i], out[k][i]):
+                assert np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                                      b.view(np.uint32) if b.dtype == np.float32 else b), f"run {k} view {i}"
+
+
+def test_adjoint_same_for_every_tile_shape(api):
+    """The adjoint walks whatever tiles the forward used: texture and multiplier gradients agree
+    across tile shapes up to the order of the float atomics."""
+    views = small_views()
+    tex = torch.from_numpy(S.float_texture(64, 6)).cuda()
+    mul = torch.from_numpy((1.0 + 0.3 * S.Rng(11).uniform(views[0].points.shape[0] * 3, -1, 1)).reshape(-1, 3)
+                           .astype(np.float32)).cuda()
+    rng = np.random.default_rng(5)
+    grads = [torch.from_numpy(rng.standard_normal((v.height, v.width, 4)).astype(np.float32)) for v in views]
+    results = []
+    for tile_config in (0, 1, 2):
         gv = api.GpuViews(views)
+        for g, buf in zip(grads, gv.grad):
+            buf.copy_(g)
         r = api.DiffRenderer()
-        r.forward(gv.model, gv.views, tex, mul, api.tuning(clear=True, raster_path=path))
+        r.forward(gv.model, gv.views, tex, mul, api.tuning(clear=True, tile_config=tile_config))
+        gt, gm = torch.zeros_like(tex), torch.zeros_like(mul)
+        r.backward(gt, gm, api.tuning())
         torch.cuda.synchronize()
-        out.append([(gv.color[i].cpu().numpy(), gv.visibility[i].cpu().numpy(), gv.depth[i].cpu().numpy())
-                    for i in range(len(views))])
-    for i in range(len(views)):
-        for a, b in zip(out[0][i], out[1][i]):
-            assert np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
-                                  b.view(np.uint32) if b.dtype == np.float32 else b), f"view {i}"
+        results.append((gt.cpu().numpy().astype(np.float64), gm.cpu().numpy().astype(np.float64)))
+    for gt, gm in results[1:]:
+        assert np.linalg.norm(gt - results[0][0]) <= 1e-5 * np.linalg.norm(results[0][0])
+        assert np.linalg.norm(gm - results[0][1]) <= 1e-5 * np.linalg.norm(results[0][1])
 
 
 def test_adjoint_matches_oracle_and_dot_product(api):
