@@ -1073,6 +1073,10 @@ template <int TW, int TH> struct DeferLds {
 	CommandTail tail[S::CHUNK];                  // weight planes and flags
 	RowSpan rows[S::CHUNK * TH];                 // chunk command j, tile row r
 	union {
+		struct {                                 // during the rows:
+			int32_t rowFirst[S::CHUNK];          // each command's first tile row inside its aligned bound
+			int32_t rowStart[S::CHUNK + 1];      // exclusive prefix of those rows per command
+		};
 		struct {                                 // until pass 1 ends:
 			uint8_t pairStart[PAIRS];            // first quad column of each pair, from the tile's left
 			int16_t pairOffset[PAIRS + 1];       // exclusive prefix of quads per pair
@@ -1181,9 +1185,34 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 			stageToLds<NT>(s.tail, (const uint8_t *)f.commands, (int32_t)sizeof(Command), COMMAND_TAIL_OFFSET, s.list, c0, cn);
 			stageWait();
 			__syncthreads();
-			// Rows of every chunk command in the tile, one (command, row) per thread: the exact
-			// intervals of rasterizeTriangle inside the aligned bound, empty outside it.
-			for (int32_t idx = t; idx < cn * TH; idx += NT) s.rows[idx] = rowSpan(triangleRow(s.tr[idx / TH], tileY0 + idx % TH));
+			// Rows of every chunk command in the tile: empty outside the command's aligned bound,
+			// and inside it the exact interval of rasterizeTriangle, one (command, row) per thread.
+			for (int32_t idx = t; idx < cn * TH; idx += NT) s.rows[idx] = RowSpan{0, 0};
+			if (t < 64) {
+				int32_t first = 0, count = 0;
+				if (t < cn) {
+					first = tmin(tmax(s.tr[t].alT - tileY0, 0), TH);
+					count = tmax(tmin(tmax(s.tr[t].alB - tileY0, 0), TH) - first, 0);
+				}
+				int32_t x = count;
+#pragma unroll
+				for (int32_t offset = 1; offset < 64; offset <<= 1) {
+					const int32_t y = __shfl_up(x, offset, 64);
+					if (t >= offset) x += y;
+				}
+				if (t < cn) {
+					s.rowFirst[t] = first;
+					s.rowStart[t] = x - count;
+				}
+				if (t == cn - 1) s.rowStart[cn] = x;
+			}
+			__syncthreads();
+			const int32_t rowItems = s.rowStart[cn];
+			for (int32_t it = t; it < rowItems; it += NT) {
+				const int32_t j = segmentOf(s.rowStart, cn, it);
+				const int32_t r = s.rowFirst[j] + (it - s.rowStart[j]);
+				s.rows[j * TH + r] = rowSpan(triangleRow(s.tr[j], tileY0 + r));
+			}
 			__syncthreads();
 			// Keys start at the held depth with the lowest command part above every command's,
 			// so only a strictly greater depth replaces it; a NaN depth is never replaced.
