@@ -1077,15 +1077,10 @@ template <int TW, int TH> struct DeferLds {
 			int32_t rowFirst[S::CHUNK];          // each command's first tile row inside its aligned bound
 			int32_t rowStart[S::CHUNK + 1];      // exclusive prefix of those rows per command
 		};
-		struct {                                 // until pass 1 ends:
+		struct {                                 // from the pairs to pass 2:
 			uint8_t pairStart[PAIRS];            // first quad column of each pair, from the tile's left
 			int16_t pairOffset[PAIRS + 1];       // exclusive prefix of quads per pair
 			PairIndex pairOfItem[ITEM_TABLE];    // the pair of each of the first ITEM_TABLE pass-1 items
-		};
-		struct {                                 // from the winners on:
-			uint32_t quadWinners[(TW / 2) * (TH / 2)];   // distinct winning chunk commands of a quad, 8 bits each
-			int16_t quadOffset[(TW / 2) * (TH / 2) + 1]; // exclusive prefix of distinct winners per quad
-			uint8_t quadOfShade[TW * TH];                // the quad of each pass-2 item (at most four per quad)
 		};
 	};
 	uint32_t color[TW * TH];                     // the tile's canvas pixels as they stand
@@ -1149,8 +1144,6 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	constexpr int32_t PAIRS = DeferLds<TW, TH>::PAIRS;
 	constexpr int32_t ITEM_TABLE = DeferLds<TW, TH>::ITEM_TABLE;
 	constexpr int32_t PAIRS_PER_THREAD = PAIRS / NT > 0 ? PAIRS / NT : 1;
-	constexpr int32_t NQX = TW / 2, NQ = (TW / 2) * (TH / 2);
-	constexpr int32_t QUADS_PER_THREAD = (NQ + NT - 1) / NT;
 	const int32_t n = f.tileCount[tile];
 	const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
 	const CanvasDev cv = f.canvases[ci];
@@ -1260,8 +1253,9 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				const int32_t px = tileX0 + s.pairStart[pr] + 2 * (it - s.pairOffset[pr]);
 				const int32_t py = tileY0 + 2 * rp;
 				QuadEval e;
-				if (!evalQuad(s.tail[j].plane, rowInterval(s.rows[j * TH + 2 * rp]), rowInterval(s.rows[j * TH + 2 * rp + 1]), px, py, cv.height, e))
-					continue;
+				const bool ok = evalQuad(s.tail[j].plane, rowInterval(s.rows[j * TH + 2 * rp]), rowInterval(s.rows[j * TH + 2 * rp + 1]),
+				                         px, py, cv.height, e);
+				if (!ok) continue;
 				for (int l = 0; l < 4; l++) {
 					if (e.clipSides && !e.clip[l]) continue;
 					const int32_t sl = e.slot[l];
@@ -1275,52 +1269,25 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				}
 			}
 			__syncthreads();
-			// Winners: the distinct winning commands of each quad, in pixel order.
-			int32_t quadCount[QUADS_PER_THREAD];
-			threadSum = 0;
-			for (int32_t k = 0; k < QUADS_PER_THREAD; k++) {
-				const int32_t qi = t * QUADS_PER_THREAD + k;
-				int32_t count = 0;
-				uint32_t packed = 0;
-				if (qi < NQ) {
-					const int32_t p0 = (qi / NQX) * 2 * TW + (qi % NQX) * 2;
-					for (int sl = 0; sl < 4; sl++) {
-						const int32_t w = keyWinner(s.keys[p0 + (sl >> 1) * TW + (sl & 1)], depthRead);
-						if (w < 0) continue;
-						bool seen = false;
-						for (int r = 0; r < count; r++) seen |= ((packed >> (8 * r)) & 0xFFu) == (uint32_t)w;
-						if (!seen) { packed |= (uint32_t)w << (8 * count); count++; }
-					}
-					s.quadWinners[qi] = packed;
-				}
-				quadCount[k] = count;
-				threadSum += count;
-			}
-			const int32_t shades = workgroupPrefix<NT>(threadSum, before, s.scan);
-			for (int32_t k = 0; k < QUADS_PER_THREAD; k++) {
-				const int32_t qi = t * QUADS_PER_THREAD + k;
-				if (qi < NQ) s.quadOffset[qi] = (int16_t)before;
-				for (int32_t i = 0; i < quadCount[k]; i++) s.quadOfShade[before + i] = (uint8_t)qi;
-				before += quadCount[k];
-			}
-			if (t == 0) s.quadOffset[NQ] = (int16_t)shades;
-			__syncthreads();
-			// Pass 2: one (quad, winning command) item per thread: the quad's weights and mip level
-			// once, then the lanes the command won.
-			for (int32_t it = t; it < shades; it += NT) {
-				const int32_t qi = s.quadOfShade[it];
-				const int32_t jw = (int32_t)((s.quadWinners[qi] >> (8 * (it - s.quadOffset[qi]))) & 0xFFu);
-				const int32_t qx = (qi % NQX) * 2, qy = (qi / NQX) * 2;
+			// Pass 2: the same items, evaluated again (keeping pass 1's evaluations would cost more
+			// registers than the evaluation); each writes the lanes whose pixels its command won,
+			// with the quad's weights and mip level as in the in-order walk.  A pixel has one
+			// winner, so exactly one item writes it.
+			for (int32_t it = t; it < items; it += NT) {
+				const int32_t pr = it < ITEM_TABLE ? (int32_t)s.pairOfItem[it] : segmentOf(s.pairOffset, PAIRS, it);
+				const int32_t j = pr / RP, rp = pr % RP;
+				const int32_t qx = s.pairStart[pr] + 2 * (it - s.pairOffset[pr]), qy = 2 * rp;
 				QuadEval e;
-				if (!evalQuad(s.tail[jw].plane, rowInterval(s.rows[jw * TH + qy]), rowInterval(s.rows[jw * TH + qy + 1]), tileX0 + qx,
+				if (!evalQuad(s.tail[j].plane, rowInterval(s.rows[j * TH + qy]), rowInterval(s.rows[j * TH + qy + 1]), tileX0 + qx,
 				              tileY0 + qy, cv.height, e))
 					continue;
 				uint32_t mask = 0;
 				for (int l = 0; l < 4; l++) {
 					if (e.clipSides && !e.clip[l]) continue;
 					const int32_t sl = e.slot[l];
-					if (keyWinner(s.keys[(qy + (sl >> 1)) * TW + qx + (sl & 1)], depthRead) == jw) mask |= 1u << l;
+					if (keyWinner(s.keys[(qy + (sl >> 1)) * TW + qx + (sl & 1)], depthRead) == j) mask |= 1u << l;
 				}
+				if (!mask) continue;
 				// Depths first, so that only the weights and the won lanes' pixels outlive the quad
 				// evaluation while the colours are made.
 				uint32_t slots = 0;
@@ -1332,18 +1299,18 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 					if (!f.clearInRaster) atomicOr(&s.dirty[p >> 5], 1u << (p & 31));
 					slots |= (uint32_t)sl << (2 * l);
 				}
-				if (MODE == 1 && mask) {
-					const int32_t id = (s.list[c0 + jw] - f.canvasCommandBase[ci]) * 4;
-					shadeDiffLanes(f, s.tail[jw], e, mask, slots, id, cv, tileX0 + qx, tileY0 + qy);
+				if (MODE == 1) {
+					const int32_t id = (s.list[c0 + j] - f.canvasCommandBase[ci]) * 4;
+					shadeDiffLanes(f, s.tail[j], e, mask, slots, id, cv, tileX0 + qx, tileY0 + qy);
 				}
-				if (colorWrite && mask) {
-					const bool affine = (s.tail[jw].flags & 1) != 0;
+				if (colorWrite) {
+					const bool affine = (s.tail[j].flags & 1) != 0;
 					float wa[4], wb[4], wc[4];
 					for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
 					// All four lanes are shaded (every texel index is masked into its level), so the
 					// sixteen texel loads of the quad can be issued together; only won lanes are kept.
 					float r[4], g[4], b[4], a[4];
-					shadeQuad(s.si[jw], textures, wa, wb, wc, 0xFu, r, g, b, a);
+					shadeQuad(s.si[j], textures, wa, wb, wc, 0xFu, r, g, b, a);
 					for (int l = 0; l < 4; l++) {
 						if (!((mask >> l) & 1u)) continue;
 						const int32_t sl = (int32_t)((slots >> (2 * l)) & 3u);
