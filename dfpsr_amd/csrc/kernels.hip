@@ -980,7 +980,7 @@ template __global__ void k_raster<32, 32>(FrameDev, int32_t);
 // is the command alone and the last covering command wins.  Commands run in chunks per tile;
 // each chunk starts from the depths the previous one left.  Work is spread densely: one
 // thread per (command, row) for the exact rows, per (command, covered quad) for the keys and
-// per (quad, winning command) for the shading.
+// per (command, covered quad) again for the shading of the lanes each command won.
 //
 // Frames with the Alpha filter (blending reads the colour held at that moment), orthogonal
 // cameras (a lower depth wins), canvases wider than 32767 pixels (rows are kept in 16 bits) or
