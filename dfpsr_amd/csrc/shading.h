@@ -173,11 +173,17 @@ DH uint32_t sampleBilinear(const TextureDesc &t, float u, float v, uint32_t mip,
 	return mixColors(mixColors(ul, ur, wx), mixColors(bl, br, wx), wy);
 }
 
+// clampUpper + truncateToU32 of one channel: the clamp maps NaN to 255.1 (as minps does with the
+// limit as second operand), so only values below -2^31 remain for truncU32's 0x80000000, and
+// raising them to -2^31 gives exactly that.
+DH uint32_t saturatedChannel(float x) {
+	const float clamped = x < 255.1f ? x : 255.1f;
+	return (uint32_t)(int32_t)fmaxf(clamped, -2147483648.0f);
+}
+
 // clampUpper + truncateToU32 + byte packing for RGBA (PackOrder.h:191-212).
 DH uint32_t packSaturated(float r, float g, float b, float a) {
-	const float lim = 255.1f;
-	return truncU32(r < lim ? r : lim) | (truncU32(g < lim ? g : lim) << 8) | (truncU32(b < lim ? b : lim) << 16) |
-	       (truncU32(a < lim ? a : lim) << 24);
+	return saturatedChannel(r) | (saturatedChannel(g) << 8) | (saturatedChannel(b) << 16) | (saturatedChannel(a) << 24);
 }
 
 DH void unpackColor(uint32_t c, float &r, float &g, float &b, float &a) {
