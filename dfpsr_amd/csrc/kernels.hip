@@ -143,18 +143,35 @@ __device__ ShaderInputs triangleShader(const TaskDev &t, int32_t poly, int32_t k
 	return si;
 }
 
+// The projected corners of fan triangle g (of task ti) and its polygon.
+struct TriangleCorners {
+	Projected P[3];
+	int32_t poly, k;
+};
+
+__device__ __forceinline__ TriangleCorners loadCorners(const FrameDev &f, int32_t g, int32_t ti) {
+	const TaskDev &t = f.tasks[ti];
+	const int32_t fp = t.fanPolygon[g - t.fanBase];
+	TriangleCorners tc;
+	tc.poly = fp >> 1;
+	tc.k = fp & 1;
+	const int32_t corner[3] = {0, 1 + tc.k, 2 + tc.k};
+	for (int c = 0; c < 3; c++) tc.P[c] = f.projected[t.pointBase + t.indices[tc.poly * 4 + corner[c]]];
+	return tc;
+}
+
+// corners: the triangle's corners when the caller has loaded them already.
 template <bool EMIT>
 __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset, BinTally &tally,
-                                 BinJob *jobs = nullptr, int32_t jobBase = 0) {
+                                 BinJob *jobs = nullptr, int32_t jobBase = 0, const TriangleCorners *corners = nullptr) {
 	const TaskDev &t = f.tasks[ti];
 	const CanvasDev &cv = f.canvases[t.canvas];
 	if (cv.color == nullptr && cv.depth == nullptr) return 0;
 	int32_t local = g - t.fanBase;
-	int32_t fp = t.fanPolygon[local];
-	int32_t poly = fp >> 1, k = fp & 1;
+	const TriangleCorners tc = corners ? *corners : loadCorners(f, g, ti);
+	const int32_t poly = tc.poly, k = tc.k;
 	const int32_t corner[3] = {0, 1 + k, 2 + k};
-	Projected P[3];
-	for (int c = 0; c < 3; c++) P[c] = f.projected[t.pointBase + t.indices[poly * 4 + corner[c]]];
+	const Projected *P = tc.P;
 	const CameraParams &cam = t.camera;
 	if (triangleVisibility(P[0].cs, P[1].cs, P[2].cs, cam.cull) == VIS_HIDDEN) return 0;
 	if (t.filter != 0) {
@@ -319,6 +336,10 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 	__shared__ BinJob jobs[BIN_JOBS];
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	int32_t n = (g < f.fanTotal) ? f.fanCount[g] : 0;
+	// The corners are loaded before the block's prefix sums, so their latency overlaps them.
+	const int32_t ti = g < f.fanTotal ? findSegment(f.taskFanEnd, f.taskCount, g) : 0;
+	TriangleCorners corners;
+	if (n > 0) corners = loadCorners(f, g, ti);
 	int32_t ex;
 	const int32_t blockCommands = blockExclusiveScan(n, ex);
 	if (threadIdx.x < 2) blockTally[threadIdx.x] = 0;
@@ -337,13 +358,12 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 	BinTally tally{0, 0};
 	if (g < f.fanTotal) {
 		const int32_t offset = jobBase + ex;
-		const int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
 		// The commands of a canvas's tasks are consecutive, so the offset at the first fan
 		// triangle of each task bounds its canvas's first command: one atomic per task.
 		if (g == f.tasks[ti].fanBase) atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
 		if (n > 0) {
 			if (offset + n > f.commandCapacity) atomicOr(&f.counters[2], 1);
-			else setupTriangle<true>(f, g, ti, offset, tally, together ? jobs : nullptr, jobBase);
+			else setupTriangle<true>(f, g, ti, offset, tally, together ? jobs : nullptr, jobBase, &corners);
 		}
 	}
 	if (f.binCapacity == 0) return;
