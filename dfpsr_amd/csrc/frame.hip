@@ -169,23 +169,35 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 		const bool deferred = deferredRaster(o);
 		switch (tileConfig_) {
 		case 1:
-			if (deferred) hipLaunchKernelGGL((k_raster_deferred_diff<32, 16>), dim3(grid), dim3(128), 0, stream, f, persistent);
-			else hipLaunchKernelGGL((k_raster_diff<32, 16>), dim3(grid), dim3(128), 0, stream, f, persistent);
+			if (!deferred) hipLaunchKernelGGL((k_raster_diff<32, 16>), dim3(grid), dim3(128), 0, stream, f, persistent);
+			else if (persistent) hipLaunchKernelGGL((k_raster_deferred_diff<32, 16, 1>), dim3(grid), dim3(128), 0, stream, f);
+			else hipLaunchKernelGGL((k_raster_deferred_diff<32, 16, 0>), dim3(grid), dim3(128), 0, stream, f);
 			break;
 		case 2:
-			if (deferred) hipLaunchKernelGGL((k_raster_deferred_diff<32, 32>), dim3(grid), dim3(256), 0, stream, f, persistent);
-			else hipLaunchKernelGGL((k_raster_diff<32, 32>), dim3(grid), dim3(256), 0, stream, f, persistent);
+			if (!deferred) hipLaunchKernelGGL((k_raster_diff<32, 32>), dim3(grid), dim3(256), 0, stream, f, persistent);
+			else if (persistent) hipLaunchKernelGGL((k_raster_deferred_diff<32, 32, 1>), dim3(grid), dim3(256), 0, stream, f);
+			else hipLaunchKernelGGL((k_raster_deferred_diff<32, 32, 0>), dim3(grid), dim3(256), 0, stream, f);
 			break;
 		default:
-			if (deferred) hipLaunchKernelGGL((k_raster_deferred_diff<16, 16>), dim3(grid), dim3(64), 0, stream, f, persistent);
-			else hipLaunchKernelGGL((k_raster_diff<16, 16>), dim3(grid), dim3(64), 0, stream, f, persistent);
+			if (!deferred) hipLaunchKernelGGL((k_raster_diff<16, 16>), dim3(grid), dim3(64), 0, stream, f, persistent);
+			else if (persistent) hipLaunchKernelGGL((k_raster_deferred_diff<16, 16, 1>), dim3(grid), dim3(64), 0, stream, f);
+			else hipLaunchKernelGGL((k_raster_deferred_diff<16, 16, 0>), dim3(grid), dim3(64), 0, stream, f);
 			break;
 		}
 	} else if (deferredRaster(o)) {
 		switch (tileConfig_) {
-		case 1: hipLaunchKernelGGL((k_raster_deferred<32, 16>), dim3(grid), dim3(32 * 16 / 4), 0, stream, f, persistent); break;
-		case 2: hipLaunchKernelGGL((k_raster_deferred<32, 32>), dim3(grid), dim3(32 * 32 / 4), 0, stream, f, persistent); break;
-		default: hipLaunchKernelGGL((k_raster_deferred<16, 16>), dim3(grid), dim3(16 * 16 / 4), 0, stream, f, persistent); break;
+		case 1:
+			if (persistent) hipLaunchKernelGGL((k_raster_deferred<32, 16, 1>), dim3(grid), dim3(32 * 16 / 4), 0, stream, f);
+			else hipLaunchKernelGGL((k_raster_deferred<32, 16, 0>), dim3(grid), dim3(32 * 16 / 4), 0, stream, f);
+			break;
+		case 2:
+			if (persistent) hipLaunchKernelGGL((k_raster_deferred<32, 32, 1>), dim3(grid), dim3(32 * 32 / 4), 0, stream, f);
+			else hipLaunchKernelGGL((k_raster_deferred<32, 32, 0>), dim3(grid), dim3(32 * 32 / 4), 0, stream, f);
+			break;
+		default:
+			if (persistent) hipLaunchKernelGGL((k_raster_deferred<16, 16, 1>), dim3(grid), dim3(16 * 16 / 4), 0, stream, f);
+			else hipLaunchKernelGGL((k_raster_deferred<16, 16, 0>), dim3(grid), dim3(16 * 16 / 4), 0, stream, f);
+			break;
 		}
 	} else switch (tileConfig_) {
 	case 1: hipLaunchKernelGGL((k_raster<32, 16>), dim3(grid), dim3(128), 0, stream, f, persistent); break;

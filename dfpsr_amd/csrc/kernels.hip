@@ -1354,9 +1354,11 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 }
 
 // Four waves per SIMD: the raster is bound by latency, and the fourth wave hides more of it than
-// the few registers the compiler then keeps in scratch cost.
-template <int TW, int TH>
-__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred(FrameDev f, int32_t persistent) {
+// the few registers the compiler then keeps in scratch cost.  P = 1: persistent workgroups.  The
+// two forms are separate kernels so that each gets its own register allocation (the persistent
+// loop keeps more values live).
+template <int TW, int TH, int P>
+__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred(FrameDev f) {
 	__shared__ DeferLds<TW, TH> s;
 	reportCounters(f);
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
@@ -1369,25 +1371,30 @@ __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_e
 		__syncthreads();
 	}
 	const TextureDesc *textures = ldsTextures ? s.textures : f.textures;
-	forEachQueuedTile(f, persistent, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH>(f, tile, s, textures); });
+	forEachQueuedTile(f, P, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH>(f, tile, s, textures); });
 }
 
-template __global__ void k_raster_deferred<16, 16>(FrameDev, int32_t);
-
 // The differentiable forward by the visibility raster.
-template <int TW, int TH>
-__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred_diff(FrameDev f, int32_t persistent) {
+template <int TW, int TH, int P>
+__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred_diff(FrameDev f) {
 	__shared__ DeferLds<TW, TH> s;
 	reportCounters(f);
 	if (f.counters[2]) return;
-	forEachQueuedTile(f, persistent, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH, 1>(f, tile, s, nullptr); });
+	forEachQueuedTile(f, P, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH, 1>(f, tile, s, nullptr); });
 }
 
-template __global__ void k_raster_deferred_diff<16, 16>(FrameDev, int32_t);
-template __global__ void k_raster_deferred_diff<32, 16>(FrameDev, int32_t);
-template __global__ void k_raster_deferred_diff<32, 32>(FrameDev, int32_t);
-template __global__ void k_raster_deferred<32, 16>(FrameDev, int32_t);
-template __global__ void k_raster_deferred<32, 32>(FrameDev, int32_t);
+template __global__ void k_raster_deferred<16, 16, 0>(FrameDev);
+template __global__ void k_raster_deferred<16, 16, 1>(FrameDev);
+template __global__ void k_raster_deferred<32, 16, 0>(FrameDev);
+template __global__ void k_raster_deferred<32, 16, 1>(FrameDev);
+template __global__ void k_raster_deferred<32, 32, 0>(FrameDev);
+template __global__ void k_raster_deferred<32, 32, 1>(FrameDev);
+template __global__ void k_raster_deferred_diff<16, 16, 0>(FrameDev);
+template __global__ void k_raster_deferred_diff<16, 16, 1>(FrameDev);
+template __global__ void k_raster_deferred_diff<32, 16, 0>(FrameDev);
+template __global__ void k_raster_deferred_diff<32, 16, 1>(FrameDev);
+template __global__ void k_raster_deferred_diff<32, 32, 0>(FrameDev);
+template __global__ void k_raster_deferred_diff<32, 32, 1>(FrameDev);
 
 // The differentiable forward: the same tile walk with the float shader, persistent or not.
 template <int TW, int TH>
