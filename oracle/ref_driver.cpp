@@ -4,7 +4,9 @@
 // makes (Source/SDK/terrain/main.cpp:120-176 for model building,
 // :381-421 for the frame), and exposes a few of the reference's internal
 // functions so the oracle's restatement can be pinned piece by piece.
-// Nothing here is shipped; the GPU box never runs it for parity.
+// Nothing here is shipped: the GPU parity tests check against the oracle's
+// restatement, and the benchmark uses this build only to time the reference's
+// CPU renderer.
 
 #include "DFPSR/includeFramework.h"
 #include "DFPSR/implementation/render/ITriangle2D.h"
