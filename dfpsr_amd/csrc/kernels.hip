@@ -467,9 +467,9 @@ __global__ void k_bin_fill(FrameDev f) {
 // Tile raster / shade.
 // ---------------------------------------------------------------------------------------
 
-// Sort of n distinct ints in LDS by the workgroup's NL lanes (n <= TILE_SORT_CAP).  Up to NL
-// values are ranked (each lane counts the smaller values and writes its own to that place);
-// longer lists take a bitonic sort.
+// Sort of n distinct ints in LDS by the workgroup's NL lanes (n <= TILE_SORT_CAP).  Up to 4 NL
+// values are ranked (each lane counts the values smaller than each of its own and writes them
+// to those places: two barriers); longer lists take a bitonic sort.
 template <int NL>
 __device__ void sortLds(int32_t *a, int32_t n) {
 	if (n <= NL) {
@@ -479,6 +479,20 @@ __device__ void sortLds(int32_t *a, int32_t n) {
 		for (int32_t j = 0; j < n; j++) rank += a[j] < v ? 1 : 0;
 		__syncthreads();
 		if (i < n) a[rank] = v;
+		__syncthreads();
+		return;
+	}
+	if (n <= 4 * NL) {
+		const int32_t i = (int32_t)threadIdx.x;
+		int32_t v[4], rank[4] = {0, 0, 0, 0};
+		for (int k = 0; k < 4; k++) v[k] = i + k * NL < n ? a[i + k * NL] : 0x7FFFFFFF;
+		for (int32_t j = 0; j < n; j++) {
+			const int32_t aj = a[j];
+			for (int k = 0; k < 4; k++) rank[k] += aj < v[k] ? 1 : 0;
+		}
+		__syncthreads();
+		for (int k = 0; k < 4; k++)
+			if (i + k * NL < n) a[rank[k]] = v[k];
 		__syncthreads();
 		return;
 	}
