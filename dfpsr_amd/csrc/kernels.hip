@@ -1426,6 +1426,13 @@ template __global__ void k_raster_diff<16, 16>(FrameDev, int32_t);
 template __global__ void k_raster_diff<32, 16>(FrameDev, int32_t);
 template __global__ void k_raster_diff<32, 32>(FrameDev, int32_t);
 
+// Orders one wave's LDS writes before its later reads of what other lanes of the wave wrote.
+__device__ __forceinline__ void waveSync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // The adjoint of the differentiable forward with visibility fixed: each pixel re-derives the
 // interpolants of the (command, lane) that drew it -- the same rows and the same weight
 // stepping as the forward -- and scatters dL/dcolour into the four bilinear texels and the
@@ -1434,7 +1441,10 @@ template __global__ void k_raster_diff<32, 32>(FrameDev, int32_t);
 // command are first added up in LDS (a small open-addressed table of the tile's commands), so
 // a tile makes one global atomic per (command, corner, channel) instead of one per pixel;
 // commands beyond the table's room add theirs directly.  Texture gradients are gathered the
-// same way per texel (neighbouring pixels of a magnified texture share their taps).
+// same way per texel (neighbouring pixels of a magnified texture share their taps); taps that
+// find no room are staged per wave in LDS and added with one lane per channel, so that one
+// atomic instruction covers the four channels of 16 texels (L2 serves float atomics per cache
+// line and request: four channels in one request instead of four requests).
 constexpr int32_t BACKWARD_SLOTS = 64;
 constexpr int32_t TEXEL_SLOTS = 256;
 constexpr int32_t TEXEL_PROBES = 8;
@@ -1446,6 +1456,8 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 	__shared__ float slotGrad[BACKWARD_SLOTS][9];
 	__shared__ int32_t texelKey[TEXEL_SLOTS];
 	__shared__ float4 texelGrad[TEXEL_SLOTS];
+	__shared__ int32_t stageTexel[256 * 4];
+	__shared__ float stageGrad[256 * 4 * 4];
 	const int32_t t = (int32_t)threadIdx.x;
 	while (true) {
 		if (t == 0) tileLds = atomicAdd(head, 1);
@@ -1461,7 +1473,11 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 		const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
 		const CanvasDev cv = f.canvases[ci];
 		const int32_t local = tile - cv.tileBase;
-		for (int32_t pix = t; pix < f.tileW * f.tileH; pix += 256) do {
+		// Tiles hold a multiple of 256 pixels, so every wave takes each round of this loop.
+		for (int32_t pix = t; pix < f.tileW * f.tileH; pix += 256) {
+		for (int k = 0; k < 4; k++) stageTexel[t * 4 + k] = -1;
+		bool staged = false;
+		do {
 			const int32_t x = (local % cv.tilesX) * f.tileW + pix % f.tileW;
 			const int32_t y = (local / cv.tilesX) * f.tileH + pix / f.tileW;
 			if (x >= cv.width || y >= cv.height) break;
@@ -1515,8 +1531,9 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 						float *acc = (float *)&texelGrad[slot];
 						for (int ch = 0; ch < 4; ch++) atomicAdd(acc + ch, gt[ch] * taps.weight[k]);
 					} else {
-						float *dst = f.gradTexture + (size_t)texel * 4;
-						for (int ch = 0; ch < 4; ch++) unsafeAtomicAdd(dst + ch, gt[ch] * taps.weight[k]);
+						stageTexel[t * 4 + k] = texel;
+						staged = true;
+						for (int ch = 0; ch < 4; ch++) stageGrad[(t * 4 + k) * 4 + ch] = gt[ch] * taps.weight[k];
 					}
 				}
 			}
@@ -1544,21 +1561,30 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 					}
 			}
 		} while (false);
+		if (__any(staged)) {
+			// The wave's 256 staged taps in 16 rounds of 16 taps x 4 channels; the stage is
+			// the wave's own, so a wave-scope barrier orders its writes and reads.
+			waveSync();
+			const int32_t lane = t & 63, ch = lane & 3, waveBase = (t & ~63) * 4;
+			for (int32_t r = 0; r < 16; r++) {
+				const int32_t e = waveBase + r * 16 + (lane >> 2);
+				const int32_t texel = stageTexel[e];
+				if (texel >= 0) unsafeAtomicAdd(f.gradTexture + (size_t)texel * 4 + ch, stageGrad[e * 4 + ch]);
+			}
+		}
+		waveSync();
+		}
 		__syncthreads();
 		if (f.gradTexture)
-			for (int32_t i = t; i < TEXEL_SLOTS; i += 256) {
-				const int32_t texel = texelKey[i];
-				if (texel < 0) continue;
-				const float4 gsum = texelGrad[i];
-				float *dst = f.gradTexture + (size_t)texel * 4;
-				unsafeAtomicAdd(dst + 0, gsum.x);
-				unsafeAtomicAdd(dst + 1, gsum.y);
-				unsafeAtomicAdd(dst + 2, gsum.z);
-				unsafeAtomicAdd(dst + 3, gsum.w);
+			for (int32_t i = t; i < TEXEL_SLOTS * 4; i += 256) {
+				const int32_t texel = texelKey[i >> 2];
+				if (texel >= 0) unsafeAtomicAdd(f.gradTexture + (size_t)texel * 4 + (i & 3), (&texelGrad[i >> 2].x)[i & 3]);
 			}
-		if (f.gradMultipliers && t < BACKWARD_SLOTS && slotCommand[t] >= 0)
-			for (int c3 = 0; c3 < 3; c3++)
-				for (int ch = 0; ch < 3; ch++) unsafeAtomicAdd(f.gradMultipliers + slotPoint[t][c3] * 3 + ch, slotGrad[t][c3 * 3 + ch]);
+		if (f.gradMultipliers)
+			for (int32_t i = t; i < BACKWARD_SLOTS * 9; i += 256) {
+				const int32_t slot = i / 9, m = i % 9;
+				if (slotCommand[slot] >= 0) unsafeAtomicAdd(f.gradMultipliers + slotPoint[slot][m / 3] * 3 + m % 3, slotGrad[slot][m]);
+			}
 		__syncthreads();
 	}
 }
