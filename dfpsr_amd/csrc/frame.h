@@ -133,7 +133,7 @@ private:
 	size_t offTasks_ = 0, offFanEnd_ = 0, offPointEnd_ = 0, offCanvases_ = 0, offTileEnd_ = 0, offTextures_ = 0;
 	bool prepared_ = false;
 	DevBuf dShaders_, dTriangleRows_;
-	DevBuf dCanvasBase_, dProjected_, dFanCount_, dBlockSums_, dCommands_, dCounters_, dTileCount_, dTileOffset_, dTileFill_, dTileList_, dTileQueue_;
+	DevBuf dCanvasBase_, dProjected_, dFanCount_, dBlockSums_, dCommands_, dCounters_, dTileCount_, dTileOffset_, dTileFill_, dTileList_, dTileQueue_, dPairRecords_;
 	int32_t *hostCounters_ = nullptr;
 	int32_t *hostCountersDevice_ = nullptr;
 	hipEvent_t events_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};

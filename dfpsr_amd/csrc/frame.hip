@@ -155,7 +155,7 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	hipLaunchKernelGGL(k_scan_block_sums64, dim3(1), dim3(SCAN_BLOCK), 0, stream, (long long *)f.blockSums, tileBlocks,
 	                   f.counters + 1, f.counters + 3);
 	hipLaunchKernelGGL(k_scan_tiles_add, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
-	hipLaunchKernelGGL(k_bin_fill, dim3(blocksOf(f.commandCapacity, 256)), dim3(256), 0, stream, f);
+	hipLaunchKernelGGL(k_bin_fill, dim3(blocksOf(f.pairCapacity, 256)), dim3(256), 0, stream, f);
 	}
 	if (events) HIP_TRY(hipEventRecord(events_[3], stream));
 	}
@@ -275,7 +275,8 @@ int Frame::run(const RenderOptions &o) {
 		if (dCommands_.ensure((size_t)commandCapacity_ * sizeof(Command)) != hipSuccess ||
 		    dShaders_.ensure((size_t)commandCapacity_ * sizeof(ShaderInputs)) != hipSuccess ||
 		    dTriangleRows_.ensure((size_t)commandCapacity_ * sizeof(TriangleRows)) != hipSuccess ||
-		    dTileList_.ensure(listEntries * 4) != hipSuccess)
+		    dTileList_.ensure(listEntries * 4) != hipSuccess ||
+		    (!direct && dPairRecords_.ensure((size_t)pairCapacity_ * sizeof(PairRecord)) != hipSuccess))
 			return fail(-4, "device allocation failed");
 		FrameDev f;
 		std::memset(&f, 0, sizeof(f));
@@ -308,6 +309,7 @@ int Frame::run(const RenderOptions &o) {
 		f.tileQueue = dTileQueue_.as<int32_t>();
 		f.canvasCommandBase = dCanvasBase_.as<int32_t>();
 		f.pairCapacity = pairCapacity_;
+		f.pairRecords = direct ? nullptr : dPairRecords_.as<PairRecord>();
 		f.binCapacity = direct ? binCapacity_ : 0;
 		f.tileW = tileConfigW(tileConfig_);
 		f.tileH = tileConfigH(tileConfig_);

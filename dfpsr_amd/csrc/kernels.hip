@@ -41,6 +41,7 @@ struct BinTally { int32_t pairs, tiles; };
 // rectangle, the canvas's tiles per row, and the command's first pair in the block's pairs.
 struct BinJob { int32_t index, firstTile, width, tilesX, pairStart; };
 constexpr int32_t BIN_JOBS = 512;  // commands a setup block bins together (more: each thread bins its own)
+constexpr int32_t TILE_TABLE_LOG2 = 10, TILE_TABLE = 1 << TILE_TABLE_LOG2, TILE_TABLE_PROBES = 8;
 
 // Enters pair slot[k] of tile[k] into the tile's list (direct bins).
 __device__ __forceinline__ void binPair(const FrameDev &f, int32_t tile, int32_t slot, int32_t index, BinTally &tally) {
@@ -53,12 +54,19 @@ __device__ __forceinline__ void binPair(const FrameDev &f, int32_t tile, int32_t
 	tally.tiles += slot == 0 ? 1 : 0;
 }
 
+// Keeps pair record r (packed lists): the command's slot in the tile's list.
+__device__ __forceinline__ void recordPair(const FrameDev &f, int32_t r, int32_t tile, int32_t slot, int32_t index) {
+	if (r < f.pairCapacity) f.pairRecords[r] = PairRecord{tile, slot, index};
+	else atomicOr(&f.counters[2], 2);
+}
+
 // Writes draw command `index` with everything the raster needs of it -- the command, its row
 // setup and (for the RGBA8 shaders) its finished shader inputs -- and enters it into every tile
 // it can touch: the columns of cut(wholeBound, canvas), the rows of the aligned raster bound
 // (ITriangle2D.cpp:70-84).  With direct bins the index goes into the tile's list at once (in
-// any order: the raster sorts each list); otherwise only the tile's count grows here and the
-// list is filled after the counts are scanned.
+// any order: the raster sorts each list); otherwise the tile's count grows here and a pair
+// record (tile, slot, command) keeps the slot until the counts are scanned and the list is
+// filled from the records.
 __device__ void emitCommand(const FrameDev &f, int32_t index, const Command &cmd, const CanvasDev &cv,
                             const ShaderInputs &si, BinTally &tally, BinJob *jobs, int32_t jobBase) {
 	f.commands[index] = cmd;
@@ -72,17 +80,21 @@ __device__ void emitCommand(const FrameDev &f, int32_t index, const Command &cmd
 		ty0 = tr.alT / f.tileH;
 		ty1 = tmin((tr.alB - 2) / f.tileH, cv.tilesY - 1);
 	}
-	if (f.binCapacity == 0) {
-		for (int32_t ty = ty0; ty <= ty1; ty++)
-			for (int32_t tx = tx0; tx <= tx1; tx++) atomicAdd(&f.tileCount[cv.tileBase + ty * cv.tilesX + tx], 1);
-		return;
-	}
 	const int32_t w = tx1 - tx0 + 1, count = tmax(w, 0) * tmax(ty1 - ty0 + 1, 0);
 	if (jobs) {
 		// Binned later by the whole block, pairs spread over its threads.  Every command writes
 		// its job, with no pairs when it touches no tile.
 		jobs[index - jobBase] = BinJob{index, cv.tileBase + ty0 * cv.tilesX + tx0, tmax(w, 1), cv.tilesX, tally.pairs};
 		tally.pairs += count;
+		return;
+	}
+	if (f.binCapacity == 0) {
+		// Packed lists, this thread alone: records reserved for the command's pairs at once.
+		const int32_t recordBase = count > 0 ? atomicAdd(&f.counters[7], count) : 0;
+		for (int32_t i = 0; i < count; i++) {
+			const int32_t tile = cv.tileBase + (ty0 + i / w) * cv.tilesX + tx0 + i % w;
+			recordPair(f, recordBase + i, tile, atomicAdd(&f.tileCount[tile], 1), index);
+		}
 		return;
 	}
 	// Direct bins of this thread alone: the slot atomics of up to eight tiles are in flight
@@ -103,21 +115,6 @@ __device__ void emitCommand(const FrameDev &f, int32_t index, const Command &cmd
 			tally.pairs++;
 		}
 	}
-}
-
-// The tiles a command can touch (as counted by emitCommand).
-struct TileRange { int32_t tx0, tx1, ty0, ty1; bool any; };
-
-DH TileRange commandTiles(const TriangleRows &tr, const CanvasDev &cv, int32_t tw, int32_t th) {
-	TileRange r;
-	r.any = tr.clipR > tr.clipL;
-	if (!r.any) { r.tx0 = r.tx1 = r.ty0 = r.ty1 = 0; return r; }
-	r.tx0 = tr.clipL / tw;
-	r.tx1 = (tr.clipR - 1) / tw;
-	r.ty0 = tr.alT / th;
-	r.ty1 = (tr.alB - 2) / th;
-	if (r.ty1 > cv.tilesY - 1) r.ty1 = cv.tilesY - 1;
-	return r;
 }
 
 // The finished shader inputs of fan triangle (poly, k) of a task (Shader.h:37-57,
@@ -333,9 +330,16 @@ __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_
 // the command total) instead of reading offsets scanned by k_scan_block_sums.
 __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 	__shared__ int32_t blockTally[2];
+	__shared__ int32_t recordBaseLds;
 	__shared__ BinJob jobs[BIN_JOBS];
+	__shared__ int32_t tableTile[TILE_TABLE];
+	__shared__ int32_t tableCount[TILE_TABLE];
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	int32_t n = (g < f.fanTotal) ? f.fanCount[g] : 0;
+	for (int32_t e = threadIdx.x; e < TILE_TABLE; e += SCAN_BLOCK) {  // empty before the prefix sums' barriers
+		tableTile[e] = -1;
+		tableCount[e] = 0;
+	}
 	// The corners are loaded before the block's prefix sums, so their latency overlaps them.
 	const int32_t ti = g < f.fanTotal ? findSegment(f.taskFanEnd, f.taskCount, g) : 0;
 	TriangleCorners corners;
@@ -352,9 +356,9 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 	} else {
 		jobBase = f.blockSums[blockIdx.x];
 	}
-	// With direct bins the block's commands are binned together (below) when they fit the table
-	// and the command arrays (a block past the capacity leaves its jobs unwritten).
-	const bool together = f.binCapacity > 0 && blockCommands <= BIN_JOBS && jobBase + blockCommands <= f.commandCapacity;
+	// The block's commands are binned together (below) when they fit the table and the command
+	// arrays (a block past the capacity leaves its jobs unwritten).
+	const bool together = blockCommands <= BIN_JOBS && jobBase + blockCommands <= f.commandCapacity;
 	BinTally tally{0, 0};
 	if (g < f.fanTotal) {
 		const int32_t offset = jobBase + ex;
@@ -366,7 +370,6 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 			else setupTriangle<true>(f, g, ti, offset, tally, together ? jobs : nullptr, jobBase, &corners);
 		}
 	}
-	if (f.binCapacity == 0) return;
 	if (together) {
 		// Pairs of the block's commands, one per thread and round: job j owns pairs
 		// [pairStart, pairStart + width * height) in command order.  A command whose setup ran
@@ -374,16 +377,24 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 		int32_t pairBase;
 		const int32_t pairs = blockExclusiveScan(tally.pairs, pairBase);
 		for (int32_t k = 0; k < n; k++) jobs[ex + k].pairStart += pairBase;
+		if (threadIdx.x == 0 && f.binCapacity == 0) recordBaseLds = pairs > 0 ? atomicAdd(&f.counters[7], pairs) : 0;
 		__syncthreads();
+		const int32_t recordBase = recordBaseLds;
 		const int32_t jobCount = blockCommands;
 		BinTally binned{0, 0};
+		// Rounds of up to UNROLL pairs per thread.  The pairs of a round that go to one tile take
+		// their slots with one global atomic: each pair draws a rank from the tile's entry in an
+		// LDS table (TILE_TABLE entries, open addressing), one thread per entry then adds the
+		// tile's count to its global count, and the pair's slot is the returned base plus its
+		// rank.  A pair that finds no entry takes its slot with a global atomic of its own.
 		constexpr int32_t UNROLL = 4;
-		for (int32_t p0 = threadIdx.x; p0 < pairs; p0 += UNROLL * SCAN_BLOCK) {
-			int32_t tile[UNROLL], slot[UNROLL], index[UNROLL];
+		for (int32_t round = 0; round < pairs; round += UNROLL * SCAN_BLOCK) {
+			int32_t tile[UNROLL], slot[UNROLL], index[UNROLL], entry[UNROLL];
 #pragma unroll
 			for (int u = 0; u < UNROLL; u++) {
-				const int32_t p = p0 + u * SCAN_BLOCK;
+				const int32_t p = round + (int32_t)threadIdx.x + u * SCAN_BLOCK;
 				tile[u] = -1;
+				entry[u] = -1;
 				if (p < pairs) {
 					int32_t lo = 0, hi = jobCount - 1;
 					while (lo < hi) {
@@ -394,12 +405,35 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 					const int32_t i = p - j.pairStart;
 					tile[u] = j.firstTile + (i / j.width) * j.tilesX + i % j.width;
 					index[u] = j.index;
-					slot[u] = atomicAdd(&f.tileCount[tile[u]], 1);
+					const int32_t h = (int32_t)(((uint32_t)tile[u] * 2654435761u) >> (32 - TILE_TABLE_LOG2));
+					for (int32_t q = 0; q < TILE_TABLE_PROBES; q++) {
+						const int32_t e = (h + q) & (TILE_TABLE - 1);
+						const int32_t old = atomicCAS(&tableTile[e], -1, tile[u]);
+						if (old == -1 || old == tile[u]) {
+							entry[u] = e;
+							break;
+						}
+					}
+					slot[u] = entry[u] >= 0 ? atomicAdd(&tableCount[entry[u]], 1) : atomicAdd(&f.tileCount[tile[u]], 1);
 				}
 			}
+			__syncthreads();
+			for (int32_t e = threadIdx.x; e < TILE_TABLE; e += SCAN_BLOCK)
+				if (tableTile[e] >= 0) tableCount[e] = atomicAdd(&f.tileCount[tableTile[e]], tableCount[e]);
+			__syncthreads();
 #pragma unroll
-			for (int u = 0; u < UNROLL; u++)
-				if (tile[u] >= 0) binPair(f, tile[u], slot[u], index[u], binned);
+			for (int u = 0; u < UNROLL; u++) {
+				if (tile[u] < 0) continue;
+				const int32_t s = entry[u] >= 0 ? tableCount[entry[u]] + slot[u] : slot[u];
+				if (f.binCapacity > 0) binPair(f, tile[u], s, index[u], binned);
+				else recordPair(f, recordBase + round + (int32_t)threadIdx.x + u * SCAN_BLOCK, tile[u], s, index[u]);
+			}
+			__syncthreads();
+			for (int32_t e = threadIdx.x; e < TILE_TABLE; e += SCAN_BLOCK) {
+				tableTile[e] = -1;
+				tableCount[e] = 0;
+			}
+			__syncthreads();
 		}
 		tally.tiles = binned.tiles;
 		tally.pairs = threadIdx.x == 0 ? pairs : 0;
@@ -419,8 +453,7 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 // ---------------------------------------------------------------------------------------
 
 // Tile offsets and the ordered queue of non-empty tiles come from one scan of
-// (count | nonEmpty << 32); the block-local queue position waits in tileFill, which the
-// list fill uses as its cursors afterwards.
+// (count | nonEmpty << 32); the block-local queue position waits in tileFill in between.
 __global__ void k_scan_tiles_local(FrameDev f) {
 	int32_t i = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	int32_t v = (i < f.tileTotal) ? f.tileCount[i] : 0;
@@ -439,7 +472,6 @@ __global__ void k_scan_tiles_add(FrameDev f) {
 		const long long base = ((const long long *)f.blockSums)[blockIdx.x];
 		f.tileOffset[i] += (int32_t)(base & 0xFFFFFFFFll);
 		if (f.tileCount[i] > 0) f.tileQueue[(int32_t)(base >> 32) + f.tileFill[i]] = i;
-		f.tileFill[i] = 0;
 	}
 	if (i == f.tileTotal - 1) {
 		int32_t end = f.tileOffset[i] + f.tileCount[i];
@@ -448,19 +480,12 @@ __global__ void k_scan_tiles_add(FrameDev f) {
 	}
 }
 
+// The packed lists from the pair records: each command goes to its slot after its tile's offset.
 __global__ void k_bin_fill(FrameDev f) {
-	int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= f.counters[0] || f.counters[2]) return;
-	const CanvasDev &cv = f.canvases[f.commands[c].canvas];
-	TileRange r = commandTiles(f.triangleRows[c], cv, f.tileW, f.tileH);
-	if (!r.any) return;
-	for (int32_t ty = r.ty0; ty <= r.ty1; ty++) {
-		for (int32_t tx = r.tx0; tx <= r.tx1; tx++) {
-			int32_t t = cv.tileBase + ty * cv.tilesX + tx;
-			int32_t slot = atomicAdd(&f.tileFill[t], 1);
-			f.tileList[f.tileOffset[t] + slot] = c;
-		}
-	}
+	const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (f.counters[2] || r >= f.counters[7] || r >= f.pairCapacity) return;
+	const PairRecord rec = f.pairRecords[r];
+	f.tileList[f.tileOffset[rec.tile] + rec.slot] = rec.command;
 }
 
 // ---------------------------------------------------------------------------------------

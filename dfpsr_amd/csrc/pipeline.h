@@ -57,6 +57,10 @@ struct __attribute__((aligned(16))) Command {
 	int32_t canvas, flags; // canvas slot; bit 0 = affine (orthogonal camera)
 };
 
+// A (tile, command) pair of the packed lists: the command's slot in the tile's list, from the
+// tile count's atomic in the setup kernel.
+struct PairRecord { int32_t tile, slot, command; };
+
 struct FrameDev {
 	const TaskDev *tasks;
 	int32_t taskCount;
@@ -78,16 +82,18 @@ struct FrameDev {
 	int32_t commandCapacity;
 	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flags, [3] tiles with commands,
 	                              // [4] raster head, [5] backward head, [6] longest tile list when a
-	                              // bin overflowed, [16..23] per-XCD raster heads
+	                              // bin overflowed, [7] pair records, [16..23] per-XCD raster heads
 	int32_t *tileCount;           // tileTotal + 1
 	// Tile lists.  With binCapacity > 0 tile t owns tileList[t * binCapacity, ...) and the setup
 	// kernel appends to it directly; otherwise the lists are packed at tileOffset (an exclusive
-	// scan of tileCount) by the scan and fill kernels, pairCapacity entries in all.
+	// scan of tileCount) by the scan and fill kernels, pairCapacity entries in all, from the
+	// setup kernel's pair records.
 	int32_t binCapacity;
 	int32_t *tileOffset;          // tileTotal + 1 (exclusive scan of tileCount)
 	int32_t *tileFill;            // tileTotal
 	int32_t *tileList;
 	int32_t pairCapacity;
+	PairRecord *pairRecords;      // pairCapacity (packed lists only)
 	int32_t *tileQueue;           // tiles with at least one command, in tile order (counters[3] of them)
 	int32_t *canvasCommandBase;   // per canvas: index of its first draw command (visibility ids count from it)
 	int32_t *queueHead;           // next queue slot for persistent raster workgroups

@@ -220,6 +220,39 @@ def test_direct_bins_grow_to_the_longest_tile_list(api):
     assert_parity(sc, first, H.oracle_render(sc))
 
 
+def test_packed_lists_grow_past_their_first_pair_capacity(api):
+    """Commands whose bounds span the whole canvas touch every tile, so the frame makes more
+    (tile, command) pairs than the packed lists' first capacity (four per command slot): the
+    setup kernel's pair records stop at the capacity, the frame is drawn again with room for its
+    pair count (two passes), and the next frame fits in one.  Canvases and pair counts equal
+    those of the direct bins, and the oracle's bar holds."""
+    w, h = 640, 480
+    tris = []
+    for k in range(12):
+        if k % 2 == 0:
+            tris.append([(-20.0, -20.0), (w + 20.0, -20.0), (-20.0, h + 20.0)])
+        else:
+            tris.append([(w + 20.0, -20.0), (w + 20.0, h + 20.0), (-20.0, h + 20.0)])
+    sc = S.single_triangles(w, h, tris, depth=np.linspace(2.0, 3.0, 12), name="wide_pairs")
+    print(f"[gpu] packed pair capacity {sc.width}x{sc.height}", flush=True)
+    direct = api.renderer_create()
+    api.renderer_setTuning(direct, api.tuning(binning=0))
+    want = api.render_scene(sc, direct)
+    want_stats = direct.stats()
+    r = api.renderer_create()
+    api.renderer_setTuning(r, api.tuning(binning=1))
+    first = api.render_scene(sc, r)
+    first_stats = r.stats()
+    second = api.render_scene(sc, r)
+    second_stats = r.stats()
+    assert first_stats[1] > 4 * (len(tris) + 1024)
+    assert first_stats[3] == 2 and second_stats[3] == 1
+    for got in (first, second):
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1].view(np.uint32), want[1].view(np.uint32))
+    assert list(first_stats[:3]) == list(want_stats[:3]) == list(second_stats[:3])
+    assert_parity(sc, first, H.oracle_render(sc))
+
+
 def test_setup_offsets_scanned_for_many_fan_triangles(api):
     """Above 2048 setup blocks (524288 fan triangles) the command offsets come from the block
     scan kernel instead of each block summing the totals before it; both give the oracle's
