@@ -1462,17 +1462,32 @@ __device__ __forceinline__ void waveSync() {
 // interpolants of the (command, lane) that drew it -- the same rows and the same weight
 // stepping as the forward -- and scatters dL/dcolour into the four bilinear texels and the
 // three corners' RGB multipliers with float atomics.  Workgroups pull tiles from a queue
-// (all tiles of all views) until it is empty.  The multiplier gradients of the pixels of one
-// command are first added up in LDS (a small open-addressed table of the tile's commands), so
-// a tile makes one global atomic per (command, corner, channel) instead of one per pixel;
-// commands beyond the table's room add theirs directly.  Texture gradients are gathered the
-// same way per texel (neighbouring pixels of a magnified texture share their taps); taps that
-// find no room are staged per wave in LDS and added with one lane per channel, so that one
-// atomic instruction covers the four channels of 16 texels (L2 serves float atomics per cache
-// line and request: four channels in one request instead of four requests).
+// (all tiles of all views) until it is empty.  In tiles of 512 pixels and more a thread takes
+// two horizontally adjacent pixels of one quad: drawn by the same command they share the
+// quad's evaluation and one set of multiplier terms, and with the same four texels one set of
+// taps.  The multiplier
+// gradients of the pixels of one command are first added up in LDS (a small open-addressed
+// table of the tile's commands), so a tile makes one global atomic per (command, corner,
+// channel) instead of one per pixel; commands beyond the table's room add theirs directly.
+// Texture gradients are gathered the same way per texel (neighbouring pixels of a magnified
+// texture share their taps); taps that find no room are staged per wave in LDS and added with
+// one lane per channel, so that one atomic instruction covers the four channels of 16 texels
+// (L2 serves float atomics per cache line and request: four channels in one request instead
+// of four requests).
 constexpr int32_t BACKWARD_SLOTS = 64;
 constexpr int32_t TEXEL_SLOTS = 256;
 constexpr int32_t TEXEL_PROBES = 8;
+
+// One pixel's share of the adjoint: its command (-1: none), the corners' points, its bilinear
+// taps, dL/dtexel per channel before the tap weights, and dL/dmultiplier per corner and channel.
+struct AdjointTerms {
+	int32_t c;
+	int32_t point[3];
+	uint32_t texel[4];
+	float weight[4];
+	float gt[4];
+	float mul[9];
+};
 
 __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 	__shared__ int32_t tileLds;
@@ -1484,6 +1499,73 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 	__shared__ int32_t stageTexel[256 * 4];
 	__shared__ float stageGrad[256 * 4 * 4];
 	const int32_t t = (int32_t)threadIdx.x;
+
+	// The taps of P (plus Q's, which then hit the same texels) into the tile's texel table, or
+	// staged and added one lane per channel.  Every lane of the wave calls it.
+	auto addTaps = [&](const AdjointTerms &P, const AdjointTerms &Q, bool any, bool withQ) {
+		for (int k = 0; k < 4; k++) stageTexel[t * 4 + k] = -1;
+		bool staged = false;
+		if (any)
+			for (int k = 0; k < 4; k++) {
+				if (P.weight[k] == 0.0f && !(withQ && Q.weight[k] != 0.0f)) continue;
+				float value[4];
+				for (int ch = 0; ch < 4; ch++) value[ch] = P.gt[ch] * P.weight[k] + (withQ ? Q.gt[ch] * Q.weight[k] : 0.0f);
+				const int32_t texel = (int32_t)P.texel[k];
+				int32_t slot = -1;
+				const int32_t h = (int32_t)(((uint32_t)texel * 2654435761u) >> 24);
+				for (int32_t q = 0; q < TEXEL_PROBES; q++) {
+					const int32_t i = (h + q) & (TEXEL_SLOTS - 1);
+					const int32_t old = atomicCAS(&texelKey[i], -1, texel);
+					if (old == -1 || old == texel) {
+						slot = i;
+						break;
+					}
+				}
+				if (slot >= 0) {
+					float *acc = (float *)&texelGrad[slot];
+					for (int ch = 0; ch < 4; ch++) atomicAdd(acc + ch, value[ch]);
+				} else {
+					stageTexel[t * 4 + k] = texel;
+					staged = true;
+					for (int ch = 0; ch < 4; ch++) stageGrad[(t * 4 + k) * 4 + ch] = value[ch];
+				}
+			}
+		if (__any(staged)) {
+			// The wave's 256 staged taps in 16 rounds of 16 taps x 4 channels; the stage is
+			// the wave's own, so a wave-scope barrier orders its writes and reads.
+			waveSync();
+			const int32_t lane = t & 63, ch = lane & 3, waveBase = (t & ~63) * 4;
+			for (int32_t r = 0; r < 16; r++) {
+				const int32_t e = waveBase + r * 16 + (lane >> 2);
+				const int32_t texel = stageTexel[e];
+				if (texel >= 0) unsafeAtomicAdd(f.gradTexture + (size_t)texel * 4 + ch, stageGrad[e * 4 + ch]);
+			}
+		}
+		waveSync();
+	};
+	auto addMultipliers = [&](const AdjointTerms &P) {
+		int32_t slot = -1;
+		const int32_t h = (int32_t)(((uint32_t)P.c * 2654435761u) >> 26);
+		for (int32_t k = 0; k < BACKWARD_SLOTS; k++) {
+			const int32_t i = (h + k) & (BACKWARD_SLOTS - 1);
+			const int32_t old = atomicCAS(&slotCommand[i], -1, P.c);
+			if (old == -1) {
+				for (int c3 = 0; c3 < 3; c3++) slotPoint[i][c3] = P.point[c3];
+				slot = i;
+				break;
+			}
+			if (old == P.c) {
+				slot = i;
+				break;
+			}
+		}
+		for (int c3 = 0; c3 < 3; c3++)
+			for (int ch = 0; ch < 3; ch++) {
+				if (slot >= 0) atomicAdd(&slotGrad[slot][c3 * 3 + ch], P.mul[c3 * 3 + ch]);
+				else unsafeAtomicAdd(f.gradMultipliers + P.point[c3] * 3 + ch, P.mul[c3 * 3 + ch]);
+			}
+	};
+
 	while (true) {
 		if (t == 0) tileLds = atomicAdd(head, 1);
 		if (t < BACKWARD_SLOTS) slotCommand[t] = -1;
@@ -1498,106 +1580,90 @@ __global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
 		const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
 		const CanvasDev cv = f.canvases[ci];
 		const int32_t local = tile - cv.tileBase;
-		// Tiles hold a multiple of 256 pixels, so every wave takes each round of this loop.
-		for (int32_t pix = t; pix < f.tileW * f.tileH; pix += 256) {
-		for (int k = 0; k < 4; k++) stageTexel[t * 4 + k] = -1;
-		bool staged = false;
-		do {
-			const int32_t x = (local % cv.tilesX) * f.tileW + pix % f.tileW;
-			const int32_t y = (local / cv.tilesX) * f.tileH + pix / f.tileW;
-			if (x >= cv.width || y >= cv.height) break;
-			const size_t p = (size_t)y * cv.diffStride + x;
-			const int32_t id = cv.visibility[p];
-			if (id < 0) break;
-			const int32_t c = (id >> 2) + f.canvasCommandBase[ci], lane = id & 3;
-			const Command cmd = f.commands[c];
-			const TaskDev &task = f.tasks[cmd.task];
-			const TriangleRows &tr = f.triangleRows[c];
-			const int32_t py = y & ~1, px = x & ~1;
-			RowInterval upper = triangleRow(tr, py);
-			RowInterval lower = triangleRow(tr, py + 1);
+		// Pixel pairs (x even, x + 1) per thread and round, or single pixels in tiles of 256
+		// pixels (every thread busy): 256 or 512 items a tile, whole rounds for every wave.
+		const int32_t span = f.tileW * f.tileH >= 512 ? 2 : 1;
+		const int32_t across = f.tileW / span;
+		for (int32_t item = t; item < across * f.tileH; item += 256) {
+			const int32_t x = (local % cv.tilesX) * f.tileW + span * (item % across);
+			const int32_t y = (local / cv.tilesX) * f.tileH + item / across;
+			AdjointTerms a[2];
+			a[0].c = -1;
+			a[1].c = -1;
 			QuadEval e;
-			if (!evalQuad(cmd, upper, lower, px, py, cv.height, e)) break;
-			const bool affine = (cmd.flags & 1) != 0;
-			float wa, wbw, wc;
-			laneWeights(e, lane, affine, wa, wbw, wc);
-			const DiffInputs d = diffInputs(task, cmd.fan);
-			float u = interp(d.u, wa, wbw, wc), v = interp(d.v, wa, wbw, wc);
-			BilinearTaps taps = bilinearTaps(f.ftexLog2w, f.ftexLog2h, u, v);
-			float tx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-			for (int k = 0; k < 4; k++) {
-				const float4 texel = *(const float4 *)(f.ftexTexels + (size_t)taps.index[k] * 4);
-				tx[0] += taps.weight[k] * texel.x;
-				tx[1] += taps.weight[k] * texel.y;
-				tx[2] += taps.weight[k] * texel.z;
-				tx[3] += taps.weight[k] * texel.w;
-			}
-			const float4 g = *(const float4 *)(cv.gradColor + p * 4);
-			const float w3[3] = {wa, wbw, wc};
-			float mixed[3];
-			for (int ch = 0; ch < 3; ch++)
-				mixed[ch] = wa * multiplier(f, d.point[0], ch) + wbw * multiplier(f, d.point[1], ch) + wc * multiplier(f, d.point[2], ch);
-			if (f.gradTexture) {
-				const float gt[4] = {g.x * mixed[0], g.y * mixed[1], g.z * mixed[2], g.w};
-				for (int k = 0; k < 4; k++) {
-					if (taps.weight[k] == 0.0f) continue;
-					const int32_t texel = (int32_t)taps.index[k];
-					int32_t slot = -1;
-					const int32_t h = (int32_t)(((uint32_t)texel * 2654435761u) >> 24);
-					for (int32_t q = 0; q < TEXEL_PROBES; q++) {
-						const int32_t i = (h + q) & (TEXEL_SLOTS - 1);
-						const int32_t old = atomicCAS(&texelKey[i], -1, texel);
-						if (old == -1 || old == texel) {
-							slot = i;
-							break;
-						}
-					}
-					if (slot >= 0) {
-						float *acc = (float *)&texelGrad[slot];
-						for (int ch = 0; ch < 4; ch++) atomicAdd(acc + ch, gt[ch] * taps.weight[k]);
-					} else {
-						stageTexel[t * 4 + k] = texel;
-						staged = true;
-						for (int ch = 0; ch < 4; ch++) stageGrad[(t * 4 + k) * 4 + ch] = gt[ch] * taps.weight[k];
-					}
+			int32_t evaluated = -1;  // the command whose quad e holds
+#pragma unroll
+			for (int i = 0; i < 2; i++) {
+				if (i >= span || x + i >= cv.width || y >= cv.height) continue;
+				const size_t p = (size_t)y * cv.diffStride + x + i;
+				const int32_t id = cv.visibility[p];
+				if (id < 0) continue;
+				const int32_t c = (id >> 2) + f.canvasCommandBase[ci], lane = id & 3;
+				const Command cmd = f.commands[c];
+				if (c != evaluated) {
+					evaluated = -1;
+					const TriangleRows &tr = f.triangleRows[c];
+					const int32_t py = y & ~1;
+					RowInterval upper = triangleRow(tr, py);
+					RowInterval lower = triangleRow(tr, py + 1);
+					if (!evalQuad(cmd, upper, lower, x & ~1, py, cv.height, e)) continue;
+					evaluated = c;
 				}
+				const bool affine = (cmd.flags & 1) != 0;
+				float wa, wbw, wc;
+				laneWeights(e, lane, affine, wa, wbw, wc);
+				const DiffInputs d = diffInputs(f.tasks[cmd.task], cmd.fan);
+				const float u = interp(d.u, wa, wbw, wc), v = interp(d.v, wa, wbw, wc);
+				const BilinearTaps taps = bilinearTaps(f.ftexLog2w, f.ftexLog2h, u, v);
+				float tx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+				for (int k = 0; k < 4; k++) {
+					const float4 texel = *(const float4 *)(f.ftexTexels + (size_t)taps.index[k] * 4);
+					tx[0] += taps.weight[k] * texel.x;
+					tx[1] += taps.weight[k] * texel.y;
+					tx[2] += taps.weight[k] * texel.z;
+					tx[3] += taps.weight[k] * texel.w;
+				}
+				const float4 g = *(const float4 *)(cv.gradColor + p * 4);
+				const float w3[3] = {wa, wbw, wc};
+				float mixed[3];
+				for (int ch = 0; ch < 3; ch++)
+					mixed[ch] = wa * multiplier(f, d.point[0], ch) + wbw * multiplier(f, d.point[1], ch) + wc * multiplier(f, d.point[2], ch);
+				AdjointTerms &o = a[i];
+				o.c = c;
+				for (int c3 = 0; c3 < 3; c3++) o.point[c3] = d.point[c3];
+				for (int k = 0; k < 4; k++) {
+					o.texel[k] = taps.index[k];
+					o.weight[k] = taps.weight[k];
+				}
+				o.gt[0] = g.x * mixed[0];
+				o.gt[1] = g.y * mixed[1];
+				o.gt[2] = g.z * mixed[2];
+				o.gt[3] = g.w;
+				const float gm[3] = {g.x * tx[0], g.y * tx[1], g.z * tx[2]};
+				for (int c3 = 0; c3 < 3; c3++)
+					for (int ch = 0; ch < 3; ch++) o.mul[c3 * 3 + ch] = gm[ch] * w3[c3];
+			}
+			const bool v0 = a[0].c >= 0, v1 = a[1].c >= 0;
+			if (f.gradTexture) {
+				bool same = v0 && v1;
+				for (int k = 0; k < 4; k++) same = same && a[0].texel[k] == a[1].texel[k];
+				// First the left pixel's taps (with the right one's when they hit the same
+				// texels) or the right pixel's alone, then the right pixel's when still due.
+				AdjointTerms first = a[1];
+				if (v0) first = a[0];
+				addTaps(first, a[1], v0 || v1, same);
+				const bool second = v0 && v1 && !same;
+				if (__any(second)) addTaps(a[1], a[1], second, false);
 			}
 			if (f.gradMultipliers) {
-				const float gm[3] = {g.x * tx[0], g.y * tx[1], g.z * tx[2]};
-				int32_t slot = -1;
-				const int32_t h = (int32_t)(((uint32_t)c * 2654435761u) >> 26);
-				for (int32_t k = 0; k < BACKWARD_SLOTS; k++) {
-					const int32_t i = (h + k) & (BACKWARD_SLOTS - 1);
-					const int32_t old = atomicCAS(&slotCommand[i], -1, c);
-					if (old == -1) {
-						for (int c3 = 0; c3 < 3; c3++) slotPoint[i][c3] = d.point[c3];
-						slot = i;
-						break;
-					}
-					if (old == c) {
-						slot = i;
-						break;
-					}
+				if (v0 && v1 && a[0].c == a[1].c) {
+					for (int m = 0; m < 9; m++) a[0].mul[m] += a[1].mul[m];
+					addMultipliers(a[0]);
+				} else {
+					if (v0) addMultipliers(a[0]);
+					if (v1) addMultipliers(a[1]);
 				}
-				for (int c3 = 0; c3 < 3; c3++)
-					for (int ch = 0; ch < 3; ch++) {
-						if (slot >= 0) atomicAdd(&slotGrad[slot][c3 * 3 + ch], gm[ch] * w3[c3]);
-						else unsafeAtomicAdd(f.gradMultipliers + d.point[c3] * 3 + ch, gm[ch] * w3[c3]);
-					}
 			}
-		} while (false);
-		if (__any(staged)) {
-			// The wave's 256 staged taps in 16 rounds of 16 taps x 4 channels; the stage is
-			// the wave's own, so a wave-scope barrier orders its writes and reads.
-			waveSync();
-			const int32_t lane = t & 63, ch = lane & 3, waveBase = (t & ~63) * 4;
-			for (int32_t r = 0; r < 16; r++) {
-				const int32_t e = waveBase + r * 16 + (lane >> 2);
-				const int32_t texel = stageTexel[e];
-				if (texel >= 0) unsafeAtomicAdd(f.gradTexture + (size_t)texel * 4 + ch, stageGrad[e * 4 + ch]);
-			}
-		}
-		waveSync();
 		}
 		__syncthreads();
 		if (f.gradTexture)
