@@ -168,6 +168,38 @@ def test_adjoint_matches_oracle_and_dot_product(api):
     assert abs(dot_rgb - float((got_m * mul).sum())) <= 1e-4 * abs(dot_rgb)
 
 
+def test_adjoint_matches_oracle_on_ragged_canvases(api):
+    """Canvases of odd width and of height that is no multiple of the tile height: the pixel
+    pairs at the right edge hold one pixel only, and edge tiles hold partial rows.  Every tile
+    shape's adjoint meets the oracle's bar."""
+    views = S.sphere_views(12, 2, 203, 118, seed=6, radius=1.8)  # the sphere fills the edge columns and rows
+    tex = S.float_texture(64, 8)
+    mul = (1.0 + 0.3 * S.Rng(12).uniform(views[0].points.shape[0] * 3, -1, 1)).reshape(-1, 3).astype(np.float32)
+    rng = np.random.default_rng(8)
+    grads = [rng.standard_normal((v.height, v.width, 4)).astype(np.float32) for v in views]
+    ref_t = np.zeros(tex.shape)
+    ref_m = np.zeros(mul.shape)
+    for i, sc in enumerate(views):
+        col, vis, dep, w = H.oracle_diff_forward(sc, tex, mul)
+        a, b = H.oracle_diff_backward(sc, tex, mul, vis, w, grads[i], mul.shape[0])
+        ref_t += a
+        ref_m += b
+    tex_d, mul_d = torch.from_numpy(tex).cuda(), torch.from_numpy(mul).cuda()
+    for tile_config in (0, 1, 2):
+        print(f"[gpu] ragged adjoint 203x118 tile_config {tile_config}", flush=True)
+        gv = api.GpuViews(views)
+        for g, buf in zip(grads, gv.grad):
+            buf.copy_(torch.from_numpy(g))
+        r = api.DiffRenderer()
+        r.forward(gv.model, gv.views, tex_d, mul_d, api.tuning(clear=True, tile_config=tile_config))
+        gt, gm = torch.zeros_like(tex_d), torch.zeros_like(mul_d)
+        r.backward(gt, gm, api.tuning())
+        torch.cuda.synchronize()
+        got_t, got_m = gt.cpu().numpy().astype(np.float64), gm.cpu().numpy().astype(np.float64)
+        assert np.linalg.norm(got_t - ref_t) <= ADJOINT_RTOL * np.linalg.norm(ref_t)
+        assert np.linalg.norm(got_m - ref_m) <= ADJOINT_RTOL * np.linalg.norm(ref_m)
+
+
 @pytest.fixture(scope="module")
 def fit_setup(api):
     """Config 3: 8 views at 1280x720 of a 50,000-triangle displaced geodesic sphere (frequency 50),
