@@ -448,8 +448,9 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Binning into screen tiles.  Lists are filled with atomics and sorted by command index in
-// the raster kernel, which restores submission order per pixel.
+// Binning into screen tiles.  List slots come from atomics on the tile counts, so a list is
+// in no particular order; the raster kernel sorts it by command index, which restores
+// submission order per pixel.
 // ---------------------------------------------------------------------------------------
 
 // Tile offsets and the ordered queue of non-empty tiles come from one scan of
