@@ -265,18 +265,30 @@ __global__ void k_setup_count(FrameDev f) {
 	if (threadIdx.x == 0) f.blockSums[blockIdx.x] = total;
 }
 
-// Exclusive scan of the per-block sums by one workgroup; writes the grand total.
+// Exclusive scan of the per-block sums by one workgroup; writes the grand total.  Each thread
+// takes a run of PER consecutive sums per step, so a step covers SCAN_BLOCK * PER of them.
 __global__ void k_scan_block_sums(int32_t *blockSums, int32_t blocks, int32_t *total) {
+	constexpr int32_t PER = 16;
 	__shared__ int32_t carry;
 	if (threadIdx.x == 0) carry = 0;
 	__syncthreads();
-	for (int32_t base = 0; base < blocks; base += SCAN_BLOCK) {
-		int32_t i = base + threadIdx.x;
-		int32_t v = (i < blocks) ? blockSums[i] : 0;
+	for (int32_t base = 0; base < blocks; base += SCAN_BLOCK * PER) {
+		const int32_t first = base + (int32_t)threadIdx.x * PER;
+		int32_t v[PER], run = 0;
+#pragma unroll
+		for (int k = 0; k < PER; k++) {
+			v[k] = first + k < blocks ? blockSums[first + k] : 0;
+			run += v[k];
+		}
 		int32_t ex;
-		int32_t sum = blockExclusiveScan(v, ex);
-		int32_t c = carry;
-		if (i < blocks) blockSums[i] = c + ex;
+		const int32_t sum = blockExclusiveScan(run, ex);
+		const int32_t c = carry;
+		int32_t at = c + ex;
+#pragma unroll
+		for (int k = 0; k < PER; k++) {
+			if (first + k < blocks) blockSums[first + k] = at;
+			at += v[k];
+		}
 		__syncthreads();
 		if (threadIdx.x == 0) carry = c + sum;
 		__syncthreads();
