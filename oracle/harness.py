@@ -105,6 +105,7 @@ def load_ref():
     lib = _load(REF_SO, "ref")
     lib.ref_render_scene.argtypes = [C.POINTER(SceneDesc), C.c_void_p, C.c_void_p]
     lib.ref_time_scene.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    lib.ref_render_depth_scene.argtypes = [C.POINTER(SceneDesc), C.c_void_p]
     return lib
 
 
@@ -112,6 +113,8 @@ def load_oracle():
     lib = _load(ORACLE_SO, "oracle")
     lib.oracle_render_scene.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p]
     lib.oracle_render_scene.restype = C.c_int32
+    lib.oracle_render_depth_scene.argtypes = [C.POINTER(SceneDesc), C.c_void_p]
+    lib.oracle_render_depth_scene.restype = C.c_int32
     lib.oracle_time_scene.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     lib.oracle_time_scene.restype = C.c_int32
     lib.oracle_texture_query.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
@@ -235,6 +238,35 @@ def oracle_render(scene, threads: int = 1):
     if rc != 0:
         raise RuntimeError(f"oracle failed with {rc}")
     return color, depth
+
+
+def _initial_depth(scene):
+    if scene.initial_depth is None:
+        return np.zeros((scene.height, scene.width), dtype=np.float32)
+    return np.ascontiguousarray(scene.initial_depth, dtype=np.float32).copy()
+
+
+def ref_render_depth(scene):
+    """model_renderDepth of every instance with the compiled reference (depth canvas only)."""
+    lib = load_ref()
+    d, keep = make_desc(scene, False)
+    d.has_depth = 1
+    depth = _initial_depth(scene)
+    if lib.ref_render_depth_scene(C.byref(d), depth.ctypes.data_as(C.c_void_p)) != 0:
+        raise RuntimeError("reference depth renderer failed")
+    return depth
+
+
+def oracle_render_depth(scene):
+    """model_renderDepth of every instance with the oracle (depth canvas only)."""
+    lib = load_oracle()
+    d, keep = make_desc(scene, False)
+    d.has_depth = 1
+    depth = _initial_depth(scene)
+    rc = lib.oracle_render_depth_scene(C.byref(d), depth.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"oracle failed with {rc}")
+    return depth
 
 
 def oracle_time(scene, threads: int, warmup: int, iterations: int) -> float:

@@ -1174,9 +1174,90 @@ void executeAll(const Frame &fr, int32_t threads) {
 	for (auto &t : pool) t.join();
 }
 
+// executeTriangleDrawingDepth + drawTriangleDepth (renderCore.cpp:344-394): rows of the aligned
+// bound over the whole canvas, one depth value per row start, then one add of the x step per
+// pixel; perspective keeps the greater depth, orthogonal the smaller.  Rows at or below the
+// canvas height are skipped: the reference's aligned bound rounds the bottom up to an even row,
+// which on an odd-height canvas lies outside the image.
+void drawTriangleDepth(const Camera &cam, float *depth, int32_t width, int32_t height, const Tri &t) {
+	if (!frontFacing(t)) return;
+	Rect clip = rectFromSides(0, 0, width, height);
+	if (!overlaps(t.wholeBound, clip)) return;
+	Rect bound = alignedRasterBound(t, clip);
+	if (bound.h <= 0) return;
+	std::vector<Row> rows((size_t)bound.h);
+	rasterize(t.p[0].flat, t.p[1].flat, t.p[2].flat, rows.data(), bound);
+	Projection proj = getProjection(t, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, cam.perspective);
+	for (int32_t y = bound.t; y < bound.t + bound.h && y < height; y++) {
+		Row r = rows[(size_t)(y - bound.t)];
+		float value = weightAtPixel(proj, r.left, y).x;
+		float *row = depth + (size_t)y * (size_t)width;
+		for (int32_t x = r.left; x < r.right; x++) {
+			if (proj.affine ? (value < row[x]) : (value > row[x])) row[x] = value;
+			value += proj.dx.x;
+		}
+	}
+}
+
+// renderTriangleFromDataDepth (renderCore.cpp:405-443): cull, draw whole or clip into a fan.
+void giveTriangleDepth(const Camera &cam, float *depth, int32_t width, int32_t height, const Projected &a,
+                       const Projected &b, const Projected &c) {
+	Tri tri = makeTri(a, b, c);
+	if (triangleVisibility(tri, cam.cull) == Hidden) return;
+	if (triangleVisibility(tri, cam.clip) == Full) {
+		if (frontFacing(tri)) drawTriangleDepth(cam, depth, width, height, tri);
+		return;
+	}
+	Polygon9 poly;
+	poly.v[0].cs = tri.p[0].cs; poly.v[0].subB = 0.0f; poly.v[0].subC = 0.0f;
+	poly.v[1].cs = tri.p[1].cs; poly.v[1].subB = 1.0f; poly.v[1].subC = 0.0f;
+	poly.v[2].cs = tri.p[2].cs; poly.v[2].subB = 0.0f; poly.v[2].subC = 1.0f;
+	poly.count = 3;
+	for (int32_t s = 0; s < cam.clip.count; s++) poly.clip(cam.clip.planes[s]);
+	for (int32_t i = 0; i < poly.count - 2; i++) {
+		Tri sub = makeTri(cameraToScreen(cam, poly.v[0].cs), cameraToScreen(cam, poly.v[1 + i].cs), cameraToScreen(cam, poly.v[2 + i].cs));
+		drawTriangleDepth(cam, depth, width, height, sub);
+	}
+}
+
+// ModelImpl::renderDepth + Part::renderDepth (Model.cpp:164-210): box cull, project every point,
+// then each polygon as the fan (0,1,2), (0,2,3) in polygon order.
+void renderModelDepth(const scene_desc *s, const Camera &cam, const Transform &modelToWorld, float *depth) {
+	V3 mn{0.0f, 0.0f, 0.0f}, mx{0.0f, 0.0f, 0.0f};
+	for (int32_t i = 0; i < s->point_count; i++) {
+		const float *p = s->points + i * 3;
+		mn.x = std::min(mn.x, p[0]); mn.y = std::min(mn.y, p[1]); mn.z = std::min(mn.z, p[2]);
+		mx.x = std::max(mx.x, p[0]); mx.y = std::max(mx.y, p[1]); mx.z = std::max(mx.z, p[2]);
+	}
+	if (boxSeen(cam, mn, mx, modelToWorld) == 0) return;
+	std::vector<Projected> proj((size_t)s->point_count);
+	for (int32_t i = 0; i < s->point_count; i++) {
+		const float *p = s->points + i * 3;
+		proj[(size_t)i] = cameraToScreen(cam, transformPointTransposedInverse(cam.location, transformPoint(modelToWorld, V3{p[0], p[1], p[2]})));
+	}
+	for (int32_t part = 0; part < s->part_count; part++) {
+		for (int32_t poly = s->part_polygon_start[part]; poly < s->part_polygon_start[part + 1]; poly++) {
+			const int32_t *idx = s->indices + poly * 4;
+			int32_t fan = (idx[3] >= 0) ? 2 : 1;
+			for (int32_t k = 0; k < fan; k++)
+				giveTriangleDepth(cam, depth, s->width, s->height, proj[(size_t)idx[0]], proj[(size_t)idx[1 + k]], proj[(size_t)idx[2 + k]]);
+		}
+	}
+}
+
 }  // namespace
 
 extern "C" {
+
+int32_t oracle_render_depth_scene(const scene_desc *s, float *outDepth) {
+	if (s->width < 1 || s->height < 1 || !outDepth) return -1;
+	Camera cam = makeCamera(s);
+	size_t n = (size_t)s->width * (size_t)s->height;
+	if (s->initial_depth) memcpy(outDepth, s->initial_depth, n * 4); else memset(outDepth, 0, n * 4);
+	for (int32_t i = 0; i < s->instance_count; i++)
+		renderModelDepth(s, cam, transformFrom(s->instance_transforms + i * 12), outDepth);
+	return 0;
+}
 
 int32_t oracle_render_scene(const scene_desc *s, int32_t threads, uint32_t *outColor, float *outDepth) {
 	if (s->width < 1 || s->height < 1) return -1;

@@ -23,6 +23,10 @@ extern "C" {
  * worker threads.  Both give identical images.  Returns 0 on success. */
 int32_t oracle_render_scene(const scene_desc *scene, int32_t threads, uint32_t *out_color, float *out_depth);
 
+/* model_renderDepth (modelAPI.cpp:202-206) of every instance into the depth canvas alone,
+ * starting from initial_depth (or zeros). */
+int32_t oracle_render_depth_scene(const scene_desc *scene, float *out_depth);
+
 /* Median seconds of the begin..end frame over `iterations` (after `warmup`). */
 int32_t oracle_time_scene(const scene_desc *scene, int32_t threads, int32_t warmup, int32_t iterations, double *out_seconds);
 

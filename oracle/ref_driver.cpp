@@ -156,6 +156,24 @@ int ref_render_scene(const scene_desc *s, uint32_t *outColor, float *outDepth) {
 	}
 }
 
+// model_renderDepth (modelAPI.cpp:202-206) of every instance into the depth canvas alone.
+int ref_render_depth_scene(const scene_desc *s, float *outDepth) {
+	try {
+		if (!s->has_depth) return -1;
+		BuiltScene b;
+		buildScene(s, b);
+		loadCanvas(s, b);
+		ImageF32 depth = b.depth;
+		for (int32_t i = 0; i < s->instance_count; i++) {
+			model_renderDepth(b.model, makeTransform(s->instance_transforms + i * 12), depth, b.camera);
+		}
+		readCanvas(s, b, nullptr, outDepth);
+		return 0;
+	} catch (...) {
+		return -1;
+	}
+}
+
 // Median wall time of renderer_begin..renderer_end over `iterations` frames,
 // with the canvas reloaded (untimed) before each.
 int ref_time_scene(const scene_desc *s, int32_t warmup, int32_t iterations, double *outSeconds) {

@@ -158,3 +158,33 @@ def fixture_scenes():
     out = micro_scenes() + clip_scenes() + [variant(s) for s in range(40)] + filter_scenes()
     out += [id_color(s) for s in clip_scenes()]
     return out
+
+
+def depth_scenes():
+    """Scenes for model_renderDepth (depth canvas only): random soups in perspective and
+    orthogonal views, the clipping scenes, and a preloaded depth canvas.  Heights are even:
+    on an odd-height canvas the reference's depth-only filler rounds the last row bound up
+    past the image (renderCore.cpp:345-349 with ITriangle2D.cpp:70-75), so those rows are
+    not compared with it."""
+    out = []
+    for seed in range(14):
+        sc = variant(seed)
+        sc.height = max(2, sc.height - (sc.height % 2))
+        if sc.initial_depth is not None:
+            sc.initial_depth = np.ascontiguousarray(sc.initial_depth[:sc.height])
+        sc.initial_color = None
+        sc.has_color, sc.has_depth = False, True
+        if seed % 4 == 1:
+            sc.perspective = False
+            sc.width_slope = 0.5 + 0.1 * seed
+        if not sc.perspective and sc.initial_depth is None:
+            # Orthogonal views keep the smaller depth: start from a far plane.
+            sc.initial_depth = np.full((sc.height, sc.width), 1.0e6, np.float32)
+        sc.name = f"depth_{sc.name}"
+        out.append(sc)
+    for sc in clip_scenes():
+        sc.height -= sc.height % 2
+        sc.has_color, sc.has_depth = False, True
+        sc.name = f"depth_{sc.name}"
+        out.append(sc)
+    return out

@@ -315,3 +315,50 @@ def test_depth_is_the_inverse_depth_plane():
     exact = plane[0] * (xs + 0.5) + plane[1] * (ys + 0.5) + plane[2]
     assert len(xs) > 500
     assert np.max(np.abs(depth[ys, xs] - exact) / exact) < 1e-4
+
+
+@pytest.fixture(scope="module")
+def depth_goldens():
+    return np.load(os.path.join(GOLDEN, "depth_scenes.npz"))
+
+
+@pytest.mark.parametrize("scene", fixture_scenes.depth_scenes(), ids=lambda s: s.name)
+def test_depth_only_golden(scene, depth_goldens):
+    """model_renderDepth: the oracle leaves the reference's depth canvas bit for bit."""
+    got = H.oracle_render_depth(scene)
+    want = depth_goldens[scene.name]
+    bad = got.view(np.uint32) != want.view(np.uint32)
+    assert not bad.any(), f"{int(bad.sum())} depth pixels differ, first at {np.argwhere(bad)[:3].tolist()}"
+
+
+def test_depth_only_covers_what_the_renderer_covers():
+    """Both paths cut the same row intervals (ITriangle2D::getShape): on a perspective scene
+    cleared to zero, the depth-only path writes exactly the pixels the renderer's depth test
+    writes, with the same inverse depth up to the order of the plane's additions."""
+    checked = 0
+    for sc in fixture_scenes.depth_scenes():
+        if not sc.perspective or sc.initial_depth is not None:
+            continue
+        only = H.oracle_render_depth(sc)
+        _, full = H.oracle_render(sc, threads=1)
+        assert np.array_equal(only > 0, full > 0), sc.name
+        np.testing.assert_allclose(only, full, rtol=1e-5, atol=0, err_msg=sc.name)
+        checked += 1
+    assert checked >= 5
+
+
+def test_depth_only_keeps_nearest_in_both_projections():
+    """Perspective keeps the greater inverse depth, orthogonal the smaller depth: the same
+    screen triangle at depth 2 then at depth 4 leaves the depth-2 values wherever it drew."""
+    tri = [(2, 2), (30, 2), (2, 30)]
+    for perspective in (True, False):
+        scenes = [S.single_triangles(32, 32, [tri] * len(d), depth=d) for d in ([2.0], [4.0], [2.0, 4.0])]
+        for sc in scenes:
+            sc.perspective = perspective
+            if not perspective:
+                sc.initial_depth = np.full((32, 32), 1.0e6, np.float32)
+        near, far, both = (H.oracle_render_depth(sc) for sc in scenes)
+        written = near != (0.0 if perspective else 1.0e6)
+        assert written.sum() > 50
+        assert np.array_equal(both[written], near[written])
+        assert np.all((near[written] > far[written]) if perspective else (near[written] < far[written]))
