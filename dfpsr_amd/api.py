@@ -104,6 +104,12 @@ def model_getBoundingBox(model: Model):
     return mn, mx
 
 
+def model_renderDepth(model: Model, transform, depth: Image, camera: Camera) -> None:
+    """model_renderDepth (modelAPI.h:291): the model's depth alone into `depth`, synchronously."""
+    t = np.ascontiguousarray(transform, dtype=np.float32)
+    check(lib().dfpsr_hip_model_render_depth(model.handle, t.ctypes.data_as(C.c_void_p), C.byref(depth), C.byref(camera)))
+
+
 def camera_create(perspective: bool, location, image_width: float, image_height: float, width_slope: float,
                   near_clip: float = 0.01, far_clip: float = 1000.0) -> Camera:
     """Camera::createPerspective (perspective=True) or Camera::createOrthogonal."""
@@ -258,6 +264,22 @@ def render_scene(scene, renderer: Optional[Renderer] = None, gpu_scene: Optional
     g.give(r)
     renderer_end(r)
     return color, depth
+
+
+def render_scene_depth(scene, gpu_scene: Optional[GpuScene] = None, on_device: bool = False):
+    """model_renderDepth of every instance of `scene` into its depth canvas alone; returns it
+    as a numpy array.  on_device: draw into a device tensor instead of a host array."""
+    g = gpu_scene or GpuScene(scene)
+    depth = np.zeros((scene.height, scene.width), np.float32) if scene.initial_depth is None else scene.initial_depth.astype(np.float32).copy()
+    if on_device:
+        import torch
+        dev = torch.from_numpy(depth).cuda()
+        for t in scene.instances:
+            model_renderDepth(g.model, t, device_image(dev), g.camera)
+        return dev.cpu().numpy()
+    for t in scene.instances:
+        model_renderDepth(g.model, t, host_image(depth), g.camera)
+    return depth
 
 
 # ---------------------------------------------------------------------------------------

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdfpsr_hip.so")
-SOURCES = ["api.hip", "frame.hip", "kernels.hip"]
+SOURCES = ["api.hip", "frame.hip", "kernels.hip", "depth.hip"]
 HEADERS = ["geometry.h", "shading.h", "pipeline.h", "frame.h"]
 
 HIPCC_FLAGS = [

@@ -132,6 +132,7 @@ struct dfpsr_hip_model {
 	std::vector<int32_t> fanPolygon, polygonPart;
 	std::vector<const TextureDesc *> partDiffuse, partLight;
 	DevBuf dPoints, dIndices, dColors, dTexcoords, dFanPolygon, dPolygonPart;
+	DevBuf dProjected, dDepthStage;  // working memory of model_render_depth
 	bool dirty = true;
 
 	ModelView view() const {
@@ -279,6 +280,49 @@ static int checkCamera(const dfpsr_hip_camera *c) {
 	return DFPSR_HIP_OK;
 }
 
+static bool hasImage(const dfpsr_hip_image *im) { return im && im->pixels; }
+
+// model_renderDepth (modelAPI.cpp:202-206): box cull, project the points, draw every fan
+// triangle's depth.  Synchronous like the reference: the canvas is final on return.
+extern "C" int dfpsr_hip_model_render_depth(const dfpsr_hip_model *cm, const float transform[12], const dfpsr_hip_image *depth,
+                                            const dfpsr_hip_camera *camera) {
+	if (!cm || !transform) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_render_depth: null argument");
+	if (!hasImage(depth)) return DFPSR_HIP_OK;  // no target buffer: nothing to draw (renderCore.cpp:409)
+	if (depth->width < 1 || depth->height < 1 || depth->stride_bytes < depth->width * 4 || (depth->stride_bytes & 3))
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_render_depth: bad canvas size or stride");
+	int rc = checkCamera(camera);
+	if (rc) return rc;
+	CameraParams cam = cameraFrom(camera);
+	const Transform modelToWorld = transformFrom(transform);
+	if (boxSeen(cam, cm->minBound, cm->maxBound, modelToWorld) == 0) return DFPSR_HIP_OK;
+	dfpsr_hip_model *m = const_cast<dfpsr_hip_model *>(cm);
+	rc = m->upload();
+	if (rc) return rc;
+	const int32_t pointCount = (int32_t)(m->points.size() / 3), fanCount = (int32_t)m->fanPolygon.size();
+	if (pointCount == 0 || fanCount == 0) return DFPSR_HIP_OK;
+	for (int32_t ix : m->indices)
+		if (ix >= pointCount) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_render_depth: a polygon names a point the model does not have");
+	const size_t canvasBytes = (size_t)depth->stride_bytes * depth->height;
+	if (m->dProjected.ensure((size_t)pointCount * sizeof(Projected)) != hipSuccess ||
+	    (!depth->on_device && m->dDepthStage.ensure(canvasBytes) != hipSuccess))
+		return fail(DFPSR_HIP_ERROR_MEMORY, "model_render_depth: device allocation failed");
+	DepthCanvas cv;
+	cv.depth = depth->on_device ? (float *)depth->pixels : m->dDepthStage.as<float>();
+	cv.width = depth->width;
+	cv.height = depth->height;
+	cv.stride = depth->stride_bytes / 4;
+	if (!depth->on_device) HIP_TRY(hipMemcpy(cv.depth, depth->pixels, canvasBytes, hipMemcpyHostToDevice));
+	hipLaunchKernelGGL(k_depth_project, dim3((pointCount + 255) / 256), dim3(256), 0, 0, m->dPoints.as<float>(), pointCount,
+	                   modelToWorld, cam, m->dProjected.as<Projected>());
+	HIP_TRY(hipGetLastError());
+	hipLaunchKernelGGL(k_depth_draw, dim3((fanCount + 3) / 4), dim3(256), 0, 0, m->dProjected.as<Projected>(),
+	                   m->dIndices.as<int32_t>(), m->dFanPolygon.as<int32_t>(), fanCount, cam, cv);
+	HIP_TRY(hipGetLastError());
+	if (!depth->on_device) HIP_TRY(hipMemcpy(depth->pixels, cv.depth, canvasBytes, hipMemcpyDeviceToHost));
+	HIP_TRY(hipDeviceSynchronize());
+	return DFPSR_HIP_OK;
+}
+
 extern "C" int dfpsr_hip_renderer_create(dfpsr_hip_renderer **out) {
 	if (!out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_create: null output");
 	std::unique_ptr<dfpsr_hip_renderer> r(new (std::nothrow) dfpsr_hip_renderer());
@@ -290,8 +334,6 @@ extern "C" int dfpsr_hip_renderer_create(dfpsr_hip_renderer **out) {
 }
 
 extern "C" void dfpsr_hip_renderer_destroy(dfpsr_hip_renderer *r) { delete r; }
-
-static bool hasImage(const dfpsr_hip_image *im) { return im && im->pixels; }
 
 extern "C" int dfpsr_hip_renderer_begin(dfpsr_hip_renderer *r, const dfpsr_hip_image *color, const dfpsr_hip_image *depth) {
 	if (!r) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_begin: null renderer");

@@ -148,6 +148,16 @@ __global__ void k_fill(int32_t *p, int32_t n, int32_t value);
 __global__ void k_clear_canvases(FrameDev f, uint32_t color, float depth);
 __global__ void k_downsample(uint32_t *texels, TextureDesc t, uint32_t targetLevel);
 
+// Depth-only drawing (model_renderDepth): a depth canvas alone, stride in elements.
+struct DepthCanvas {
+	float *depth;
+	int32_t width, height, stride;
+};
+// Projects a model's points; then one wavefront per fan triangle (256-lane workgroups).
+__global__ void k_depth_project(const float *points, int32_t n, Transform modelToWorld, CameraParams cam, Projected *out);
+__global__ void k_depth_draw(const Projected *projected, const int32_t *indices, const int32_t *fanPolygon, int32_t fanCount,
+                             CameraParams cam, DepthCanvas cv);
+
 constexpr int32_t SCAN_BLOCK = 256;
 constexpr int32_t COUNTERS = 32;
 

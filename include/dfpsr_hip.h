@@ -105,6 +105,14 @@ int dfpsr_hip_model_add_part(dfpsr_hip_model *model, int32_t count, const int32_
                              const float *texcoords, const dfpsr_hip_texture *diffuse, const dfpsr_hip_texture *light);
 /* model_getBoundingBox (api/modelAPI.h:138). */
 int dfpsr_hip_model_bound(const dfpsr_hip_model *model, float minimum[3], float maximum[3]);
+/* model_renderDepth (api/modelAPI.h:291, modelAPI.cpp:202-206; renderCore.cpp:344-443): draws
+ * the model's front-facing triangles into a depth canvas alone, keeping the greater inverse
+ * depth for a perspective camera and the smaller depth for an orthogonal one.  A NULL or
+ * pixel-less canvas draws nothing.  Synchronous: the canvas holds the result on return.
+ * Rows that the reference's even-rounded row bound puts past an odd canvas height are not
+ * drawn (the reference writes them past its image). */
+int dfpsr_hip_model_render_depth(const dfpsr_hip_model *model, const float transform[12], const dfpsr_hip_image *depth,
+                                 const dfpsr_hip_camera *camera);
 void dfpsr_hip_model_destroy(dfpsr_hip_model *model);
 
 /* ---- Renderer (api/rendererAPI.h) ------------------------------------------------------- */
