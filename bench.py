@@ -24,6 +24,8 @@ Two more objects ride on the same line (skip them with --no-batch / --no-diff):
   "differentiable": forward + adjoint steps of the 8-view 1280x720 fit (50,000-triangle
       sphere, 512^2 float texture, per-point RGB multipliers) and of the 256-view 256x256
       batch (20,480 triangles, 1024^2 texture).
+  "depth_only": model_renderDepth of the 1920x1080 terrain into a device depth canvas (skip
+      with --no-depth), per synchronous call.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-batch] [--no-diff]
 """
@@ -214,6 +216,31 @@ def differentiable_form(steps: int = 10) -> dict:
     return out
 
 
+def depth_only_form(calls: int = 20) -> dict:
+    """model_renderDepth of the 1080p terrain into a device depth canvas, per synchronous call."""
+    import torch
+    from dfpsr_amd import api, scenes
+    sc = scenes.terrain()
+    sc.has_color = False
+    g = api.GpuScene(sc)
+    depth = torch.zeros((sc.height, sc.width), dtype=torch.float32, device="cuda")
+    im = api.device_image(depth)
+    for _ in range(3):
+        depth.zero_()
+        api.model_renderDepth(g.model, sc.instances[0], im, g.camera)
+    total = 0.0
+    for _ in range(calls):
+        depth.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        api.model_renderDepth(g.model, sc.instances[0], im, g.camera)
+        total += time.perf_counter() - t0
+    ms = total / calls * 1e3
+    return {"workload": "terrain 1920x1080, 130,050 triangles, depth canvas only", "ms_per_call": round(ms, 4),
+            "mpix_per_s": round(sc.width * sc.height / (ms * 1e-3) / 1e6, 1),
+            "mtri_per_s": round(int((sc.indices[:, 3] >= 0).sum() + sc.indices.shape[0]) / (ms * 1e-3) / 1e6, 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +250,7 @@ def main() -> int:
     ap.add_argument("--cpu-baseline-only", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-depth", action="store_true")
     ap.add_argument("--tile-config", type=int, default=-1, help="renderer tile shape 0..2 (default: fastest in a sweep)")
     ap.add_argument("--graph", type=int, default=-1, help="1/0: replay the frame as a hipGraph (default: fastest)")
     ap.add_argument("--persistent", type=int, default=0, help="persistent raster workgroups per CU (0 = one per tile)")
@@ -353,6 +381,9 @@ def main() -> int:
         torch.cuda.empty_cache()
     if not args.no_diff:
         line["differentiable"] = differentiable_form()
+        torch.cuda.empty_cache()
+    if not args.no_depth:
+        line["depth_only"] = depth_only_form()
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         try:
