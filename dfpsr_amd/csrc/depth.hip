@@ -2,11 +2,11 @@
 // reference runs on the calling thread through ModelImpl::renderDepth / Part::renderDepth
 // (Model.cpp:164-210) and renderTriangleFromDataDepth (renderCore.cpp:344-443).
 //
-// One wavefront per fan triangle.  Every lane repeats the triangle's culling, clipping and
-// weight-plane setup (uniform values), then the lanes take the rows of each front-facing
-// (sub-)triangle.  A row keeps the reference's arithmetic: the depth at the row's first pixel
-// from the plane, then one float add of the x step per pixel, so each lane walks its row in
-// order.  The keep-nearest write (greater inverse depth for perspective, smaller depth for
+// A group of 2^groupShift lanes (8 to 64, within one wavefront) per fan triangle.  Every lane
+// repeats the triangle's culling, clipping and weight-plane setup (values uniform over the
+// group), then the group's lanes take the rows of each front-facing (sub-)triangle in turn.
+// A row keeps the reference's arithmetic: the depth at the row's first pixel from the plane,
+// then one float add of the x step per pixel, so each lane walks its row in order.  The keep-nearest write (greater inverse depth for perspective, smaller depth for
 // orthogonal) is a compare-and-swap loop with the reference's strict comparison; the result
 // does not depend on the order of the triangles except between +0 and -0.
 #include "pipeline.h"
@@ -36,7 +36,7 @@ __device__ __forceinline__ void keepNearest(float *cell, float value, bool great
 // executeTriangleDrawingDepth + drawTriangleDepth (renderCore.cpp:344-394) for one lane's rows.
 // Rows from the aligned bound's even bottom past an odd canvas height are skipped.
 __device__ void drawDepthTriangle(const Projected &A, const Projected &B, const Projected &C, bool perspective,
-                                  const DepthCanvas &cv, int32_t lane) {
+                                  const DepthCanvas &cv, int32_t lane, int32_t lanes) {
 	if (!frontFacing(A.flat, B.flat, C.flat)) return;
 	const Rect canvas = rectSides(0, 0, cv.width, cv.height);
 	if (!overlaps(triangleBound(A.flat, B.flat, C.flat), canvas)) return;
@@ -45,7 +45,7 @@ __device__ void drawDepthTriangle(const Projected &A, const Projected &B, const 
 	const f3 zero = make3(0.0f, 0.0f, 0.0f);
 	const Projection pr = weightPlanes(is, make3(A.cs.z, B.cs.z, C.cs.z), zero, zero, perspective);
 	const int32_t bottom = tmin(tr.alB, cv.height);
-	for (int32_t y = tr.alT + lane; y < bottom; y += 64) {
+	for (int32_t y = tr.alT + lane; y < bottom; y += lanes) {
 		const RowInterval row = triangleRow(tr, y);
 		if (row.left >= row.right) continue;
 		float value = weightAtPixel(pr, row.left, y).x;
@@ -58,11 +58,14 @@ __device__ void drawDepthTriangle(const Projected &A, const Projected &B, const 
 	}
 }
 
-// renderTriangleFromDataDepth (renderCore.cpp:405-443) for fan triangle (poly, k) of each wave.
+// renderTriangleFromDataDepth (renderCore.cpp:405-443) for fan triangle (poly, k) of each lane
+// group.  The grid holds fanCount << groupShift threads or more, in workgroups of 256.
 __global__ void __launch_bounds__(256) k_depth_draw(const Projected *projected, const int32_t *indices, const int32_t *fanPolygon,
-                                                    int32_t fanCount, CameraParams cam, DepthCanvas cv) {
-	const int32_t g = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-	const int32_t lane = threadIdx.x & 63;
+                                                    int32_t fanCount, int32_t groupShift, CameraParams cam, DepthCanvas cv) {
+	const int32_t thread = blockIdx.x * blockDim.x + threadIdx.x;
+	const int32_t g = thread >> groupShift;
+	const int32_t lanes = 1 << groupShift;
+	const int32_t lane = thread & (lanes - 1);
 	if (g >= fanCount) return;
 	const int32_t fp = fanPolygon[g];
 	const int32_t poly = fp >> 1, k = fp & 1;
@@ -72,7 +75,7 @@ __global__ void __launch_bounds__(256) k_depth_draw(const Projected *projected, 
 	const bool perspective = cam.perspective != 0;
 	if (triangleVisibility(A.cs, B.cs, C.cs, cam.cull) == VIS_HIDDEN) return;
 	if (triangleVisibility(A.cs, B.cs, C.cs, cam.clip) == VIS_FULL) {
-		drawDepthTriangle(A, B, C, perspective, cv, lane);
+		drawDepthTriangle(A, B, C, perspective, cv, lane, lanes);
 		return;
 	}
 	ClipPolygon poly9;
@@ -82,7 +85,7 @@ __global__ void __launch_bounds__(256) k_depth_draw(const Projected *projected, 
 		const Projected a = cameraToScreen(cam, poly9.v[0].cs);
 		const Projected b = cameraToScreen(cam, poly9.v[1 + i].cs);
 		const Projected c = cameraToScreen(cam, poly9.v[2 + i].cs);
-		drawDepthTriangle(a, b, c, perspective, cv, lane);
+		drawDepthTriangle(a, b, c, perspective, cv, lane, lanes);
 	}
 }
 

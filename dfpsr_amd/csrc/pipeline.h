@@ -153,10 +153,11 @@ struct DepthCanvas {
 	float *depth;
 	int32_t width, height, stride;
 };
-// Projects a model's points; then one wavefront per fan triangle (256-lane workgroups).
+// Projects a model's points; then 2^groupShift lanes (3 <= groupShift <= 6) per fan triangle,
+// in workgroups of 256 threads.
 __global__ void k_depth_project(const float *points, int32_t n, Transform modelToWorld, CameraParams cam, Projected *out);
 __global__ void k_depth_draw(const Projected *projected, const int32_t *indices, const int32_t *fanPolygon, int32_t fanCount,
-                             CameraParams cam, DepthCanvas cv);
+                             int32_t groupShift, CameraParams cam, DepthCanvas cv);
 
 constexpr int32_t SCAN_BLOCK = 256;
 constexpr int32_t COUNTERS = 32;
