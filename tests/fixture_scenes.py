@@ -188,3 +188,16 @@ def depth_scenes():
         sc.name = f"depth_{sc.name}"
         out.append(sc)
     return out
+
+
+def special_depth_canvas(scene, seed):
+    """The scene with its depth canvas preloaded, on half of the pixels, with -0, +0, -1, NaN
+    of both signs, +inf, the smallest subnormal and the largest finite float."""
+    rng = np.random.default_rng(seed)
+    special = np.array([0x80000000, 0x00000000, 0xBF800000, 0x7FC00000, 0xFFC00000, 0x7F800000,
+                        0x00000001, 0x7F7FFFFF], np.uint32).view(np.float32)
+    plain = scene.initial_depth if scene.initial_depth is not None else np.zeros((scene.height, scene.width), np.float32)
+    pick = rng.integers(0, 16, size=plain.shape)
+    scene.initial_depth = np.where(pick < 8, special[np.minimum(pick, 7)], plain).astype(np.float32)
+    return scene
+

@@ -362,3 +362,18 @@ def test_depth_only_keeps_nearest_in_both_projections():
         assert written.sum() > 50
         assert np.array_equal(both[written], near[written])
         assert np.all((near[written] > far[written]) if perspective else (near[written] < far[written]))
+
+
+@pytest.mark.reference
+@pytest.mark.skipif(not (H.ref_available() and os.path.isdir("/root/reference/Source")),
+                    reason="the reference renderer is not built from source here")
+@pytest.mark.parametrize("index", [0, 1])
+def test_depth_only_special_canvas_values_match_reference(index):
+    """model_renderDepth over special canvas values (perspective keeps the greater, orthogonal
+    the smaller): the oracle and the compiled reference leave the same bits."""
+    sc = fixture_scenes.special_depth_canvas(fixture_scenes.depth_scenes()[index], 100 + index)
+    got = H.oracle_render_depth(sc)
+    want = H.ref_render_depth(sc)
+    assert (got.view(np.uint32) != sc.initial_depth.view(np.uint32)).sum() > 50
+    bad = got.view(np.uint32) != want.view(np.uint32)
+    assert not bad.any(), f"{int(bad.sum())} depth pixels differ, first at {np.argwhere(bad)[:3].tolist()}"
