@@ -76,3 +76,16 @@ def test_depth_only_without_canvas_draws_nothing(api):
     sc = fixture_scenes.depth_scenes()[0]
     g = api.GpuScene(sc)
     api.model_renderDepth(g.model, sc.instances[0], api.host_image(None), g.camera)
+
+
+@pytest.mark.parametrize("index", [0, 1])
+def test_depth_only_special_canvas_values(api, index):
+    """A canvas preloaded with -0, +0, negative numbers, NaN of both signs, +inf and tiny and
+    huge positives: the keep-nearest write follows the reference's strict float comparison
+    on each of them (perspective keeps the greater value, orthogonal the smaller)."""
+    sc = fixture_scenes.special_depth_canvas(fixture_scenes.depth_scenes()[index], 100 + index)
+    print(f"depth-only special canvas {sc.name} perspective={sc.perspective}", flush=True)
+    got = api.render_scene_depth(sc)
+    want = H.oracle_render_depth(sc)
+    assert (got.view(np.uint32) != sc.initial_depth.view(np.uint32)).sum() > 50
+    assert_same_depth(sc.name, got, want)
