@@ -226,7 +226,10 @@ DH void finishShaderInputs(ShaderInputs &s) {
 	s.blue = scale3(s.blue, k);
 	s.alpha = scale3(s.alpha, k);
 	s.fading = !(almostSame3(s.red) && almostSame3(s.green) && almostSame3(s.blue) && almostSame3(s.alpha));
-	s.colorless = almostOne3(s.red) && almostOne3(s.green) && almostOne3(s.blue) && almostOne3(s.alpha);
+	// The texture-only shaders are chosen only for triangles without vertex fading: corners
+	// within 0.001 of 1 that still differ by 0.001 or more take the texture x interpolated
+	// vertex colour shader (DiffuseVertex / LightVertex, RgbaMultiply.h:130-136, :155-156).
+	s.colorless = !s.fading && almostOne3(s.red) && almostOne3(s.green) && almostOne3(s.blue) && almostOne3(s.alpha);
 }
 
 // shaderMethods::interpolate (shaderMethods.h:39-44)

@@ -701,7 +701,10 @@ ShaderChoice chooseShader(const TriangleInput &in) {
 	if (s.light) k *= 1.0f / 255.0f;
 	s.red = scale(in.red, k); s.green = scale(in.green, k); s.blue = scale(in.blue, k); s.alpha = scale(in.alpha, k);
 	s.fading = !(almostSame3(s.red) && almostSame3(s.green) && almostSame3(s.blue) && almostSame3(s.alpha));
-	s.colorless = almostOne3(s.red) && almostOne3(s.green) && almostOne3(s.blue) && almostOne3(s.alpha);
+	// The "without normalization" (texture-only) branches sit under !hasVertexFade
+	// (RgbaMultiply.h:137-149, :157-162): a colourless triangle that fades takes the
+	// texture x interpolated vertex colour shader instead.
+	s.colorless = !s.fading && almostOne3(s.red) && almostOne3(s.green) && almostOne3(s.blue) && almostOne3(s.alpha);
 	return s;
 }
 

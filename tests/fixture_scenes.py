@@ -134,6 +134,71 @@ def filter_scenes():
     return out
 
 
+SHADER_BRANCHES = [
+    # (name, diffuse: None / "pyramid" / "single", light, colours, filter)
+    ("diffuse_light_vertex_mip", "pyramid", True, "fade", 0),
+    ("diffuse_light_vertex_single", "single", True, "fade", 0),
+    ("diffuse_light_mip", "pyramid", True, "const", 0),
+    ("diffuse_light_single", "single", True, "const", 0),
+    ("diffuse_vertex_mip", "pyramid", False, "fade", 0),
+    ("diffuse_vertex_single", "single", False, "fade", 0),
+    ("diffuse_colorless_mip", "pyramid", False, "white", 0),
+    ("diffuse_colorless_single", "single", False, "white", 0),
+    ("diffuse_const_mip", "pyramid", False, "const", 0),
+    ("diffuse_const_single", "single", False, "const", 0),
+    ("light_vertex", None, True, "fade", 0),
+    ("light_colorless", None, True, "white", 0),
+    ("light_const", None, True, "const", 0),
+    ("vertex_only", None, False, "fade", 0),
+    ("single_color", None, False, "const", 0),
+    # Colourless and fading at once: every corner channel within 0.001 of 1, but corners differ
+    # by more than 0.001, so the reference takes the texture x vertex colour shader.
+    ("diffuse_white_fade_mip", "pyramid", False, "white_fade", 0),
+    ("diffuse_white_fade_single", "single", False, "white_fade", 0),
+    ("light_white_fade", None, True, "white_fade", 0),
+    ("diffuse_white_fade_alpha", "pyramid", False, "white_fade", 1),
+    ("light_white_fade_alpha", None, True, "white_fade", 1),
+    ("diffuse_light_white_fade", "pyramid", True, "white_fade", 0),
+]
+
+
+def shader_branch_scenes():
+    """One scene per shader template that processTriangle_RgbaMultiply can pick
+    (RgbaMultiply.h:110-175): diffuse map with or without a pyramid, light map, both or none,
+    crossed with fading, constant and white (colourless) vertex colours, plus corners that are
+    colourless and fading at once, under the Solid and the Alpha filter."""
+    out = []
+    for i, (name, diffuse, light, colours, filt) in enumerate(SHADER_BRANCHES):
+        n = 160
+        rng = S.Rng(500 + i)
+        sc = S.cube_soup(width=96, height=72, triangles=n, seed=300 + i, spread=0.35, name="shader_" + name)
+        if colours == "const":
+            c = rng.uniform(n * 4, 0.05, 0.95).reshape(n, 1, 4).astype(np.float32)
+            sc.colors[:] = c
+            if filt == 0:
+                sc.colors[:, :, 3] = 1.0
+        elif colours == "white":
+            sc.colors[:] = 1.0
+        elif colours == "white_fade":
+            sign = rng.integers(n * 16, 0, 2).reshape(n, 4, 4).astype(np.float32) * 2 - 1
+            sc.colors[:] = (1.0 + sign * np.float32(0.0008)).astype(np.float32)
+        elif filt == 1:
+            sc.colors[:, :, 3] = rng.uniform(n * 4, 0.2, 1.0).reshape(n, 4)
+        sc.textures = []
+        d = li = -1
+        if diffuse is not None:
+            sc.textures.append(S.noise_checker_texture(64, 40 + i, 7 if diffuse == "pyramid" else 1, check=8))
+            d = 0
+        if light:
+            sc.textures.append(S.noise_checker_texture(16, 60 + i, 1, check=2))
+            li = len(sc.textures) - 1
+        sc.parts = [S.Part(0, n, diffuse=d, light=li)]
+        sc.texcoords[:, :3, :] = rng.uniform(n * 12, -1.5, 2.5).reshape(n, 3, 4)
+        sc.filter = filt
+        out.append(sc)
+    return out
+
+
 def id_color(scene: S.Scene) -> S.Scene:
     """Same geometry with one constant colour per polygon and no textures: selects the
     "Single color" shader (RgbaMultiply.h:170-171), so the colour buffer is a triangle-id map
@@ -157,6 +222,7 @@ def id_color(scene: S.Scene) -> S.Scene:
 def fixture_scenes():
     out = micro_scenes() + clip_scenes() + [variant(s) for s in range(40)] + filter_scenes()
     out += [id_color(s) for s in clip_scenes()]
+    out += shader_branch_scenes()
     return out
 
 
