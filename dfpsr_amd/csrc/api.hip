@@ -7,11 +7,13 @@
 #include "frame.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -135,8 +137,11 @@ struct dfpsr_hip_model {
 	std::vector<int32_t> fanPolygon, polygonPart;
 	std::vector<const TextureDesc *> partDiffuse, partLight;
 	DevBuf dPoints, dIndices, dColors, dTexcoords, dFanPolygon, dPolygonPart;
-	DevBuf dProjected, dDepthStage;  // working memory of model_render_depth
-	bool dirty = true;
+	std::atomic<bool> dirty{true};
+	// Guards the upload of changed arrays: several threads may draw one model at once (the
+	// reference's model_renderDepth is const and documented for background threads,
+	// modelAPI.h:280-291), and each draw first makes the model's arrays resident.
+	std::mutex uploadLock;
 
 	ModelView view() const {
 		ModelView v;
@@ -150,12 +155,14 @@ struct dfpsr_hip_model {
 		v.fanCount = (int32_t)fanPolygon.size();
 		v.partCount = (int32_t)partDiffuse.size();
 		v.filter = filter;
+		v.maxIndex = maxIndex;
 		v.partDiffuse = partDiffuse.data();
 		v.partLight = partLight.data();
 		return v;
 	}
 
 	int upload() {
+		std::lock_guard<std::mutex> hold(uploadLock);
 		if (!dirty) return DFPSR_HIP_OK;
 		struct { DevBuf *buf; const void *src; size_t bytes; } items[] = {
 			{&dPoints, points.data(), points.size() * 4}, {&dIndices, indices.data(), indices.size() * 4},
@@ -286,6 +293,14 @@ static int checkCamera(const dfpsr_hip_camera *c) {
 
 static bool hasImage(const dfpsr_hip_image *im) { return im && im->pixels; }
 
+// model_set_points may replace the points with fewer after polygons were added; drawing such a
+// model would read projected points past the model's own.  Checked before any device work.
+static int checkModelIndices(const dfpsr_hip_model *m, const char *who) {
+	if (m->maxIndex >= (int32_t)(m->points.size() / 3))
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": a polygon names a point the model does not have");
+	return DFPSR_HIP_OK;
+}
+
 // Lanes per fan triangle in the depth-only draw, as a power-of-two exponent: 32 lanes (fastest
 // on the 1080p terrain: 0.160 ms per call against 0.234, 0.198 and 0.187 ms for 8, 16 and 64)
 // unless DFPSR_HIP_DEPTH_LANES names 8, 16 or 64 (a tuning override, read once).
@@ -310,30 +325,35 @@ extern "C" int dfpsr_hip_model_render_depth(const dfpsr_hip_model *cm, const flo
 	if (rc) return rc;
 	CameraParams cam = cameraFrom(camera);
 	const Transform modelToWorld = transformFrom(transform);
+	rc = checkModelIndices(cm, "model_render_depth");
+	if (rc) return rc;
 	if (boxSeen(cam, cm->minBound, cm->maxBound, modelToWorld) == 0) return DFPSR_HIP_OK;
 	dfpsr_hip_model *m = const_cast<dfpsr_hip_model *>(cm);
-	rc = m->upload();
-	if (rc) return rc;
 	const int32_t pointCount = (int32_t)(m->points.size() / 3), fanCount = (int32_t)m->fanPolygon.size();
 	if (pointCount == 0 || fanCount == 0) return DFPSR_HIP_OK;
-	if (m->maxIndex >= pointCount) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_render_depth: a polygon names a point the model does not have");
+	rc = m->upload();
+	if (rc) return rc;
 	const size_t canvasBytes = (size_t)depth->stride_bytes * depth->height;
-	if (m->dProjected.ensure((size_t)pointCount * sizeof(Projected)) != hipSuccess ||
-	    (!depth->on_device && m->dDepthStage.ensure(canvasBytes) != hipSuccess))
+	// Working memory belongs to the calling thread, not to the model: threads drawing one
+	// model into their own canvases never share it, and a call has finished with it (the
+	// call is synchronous) before the same thread's next call can grow it.
+	static thread_local DevBuf projected, stage;
+	if (projected.ensure((size_t)pointCount * sizeof(Projected)) != hipSuccess ||
+	    (!depth->on_device && stage.ensure(canvasBytes) != hipSuccess))
 		return fail(DFPSR_HIP_ERROR_MEMORY, "model_render_depth: device allocation failed");
 	DepthCanvas cv;
-	cv.depth = depth->on_device ? (float *)depth->pixels : m->dDepthStage.as<float>();
+	cv.depth = depth->on_device ? (float *)depth->pixels : stage.as<float>();
 	cv.width = depth->width;
 	cv.height = depth->height;
 	cv.stride = depth->stride_bytes / 4;
 	if (!depth->on_device) HIP_TRY(hipMemcpy(cv.depth, depth->pixels, canvasBytes, hipMemcpyHostToDevice));
 	hipLaunchKernelGGL(k_depth_project, dim3((pointCount + 255) / 256), dim3(256), 0, 0, m->dPoints.as<float>(), pointCount,
-	                   modelToWorld, cam, m->dProjected.as<Projected>());
+	                   modelToWorld, cam, projected.as<Projected>());
 	HIP_TRY(hipGetLastError());
 	const int32_t groupShift = depthGroupShift();
 	const int64_t drawThreads = (int64_t)fanCount << groupShift;
 	if (drawThreads > INT32_MAX - 255) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_render_depth: too many polygons for one launch");
-	hipLaunchKernelGGL(k_depth_draw, dim3((uint32_t)((drawThreads + 255) / 256)), dim3(256), 0, 0, m->dProjected.as<Projected>(),
+	hipLaunchKernelGGL(k_depth_draw, dim3((uint32_t)((drawThreads + 255) / 256)), dim3(256), 0, 0, projected.as<Projected>(),
 	                   m->dIndices.as<int32_t>(), m->dFanPolygon.as<int32_t>(), fanCount, groupShift, cam, cv);
 	HIP_TRY(hipGetLastError());
 	if (!depth->on_device) HIP_TRY(hipMemcpy(depth->pixels, cv.depth, canvasBytes, hipMemcpyDeviceToHost));
@@ -397,6 +417,8 @@ extern "C" int dfpsr_hip_renderer_give_model(dfpsr_hip_renderer *r, const dfpsr_
 	if (!r || !m || !transform) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_give_model: null argument");
 	if (!r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "Cannot call renderer_giveTask before renderer_begin!");
 	int rc = checkCamera(camera);
+	if (rc) return rc;
+	rc = checkModelIndices(m, "renderer_give_model");
 	if (rc) return rc;
 	CameraParams cam = cameraFrom(camera);
 	// Camera::isBoxSeen on the model bound (modelAPI.cpp:227).
@@ -531,6 +553,8 @@ extern "C" int dfpsr_hip_batch_render(dfpsr_hip_batch *b, const dfpsr_hip_job *j
 				if (hasImage(im) && (im->width < 1 || im->height < 1 || im->stride_bytes < im->width * 4 || (im->stride_bytes & 3)))
 					return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_render: bad canvas size or stride");
 			if (!j.model) return fail(DFPSR_HIP_ERROR_ARGUMENT, "batch_render: null model");
+			rc = checkModelIndices(j.model, "batch_render");
+			if (rc) return rc;
 			rc = checkCamera(&j.camera);
 			if (rc) return rc;
 			CanvasHost cv;
@@ -653,6 +677,8 @@ extern "C" int dfpsr_hip_diff_forward(dfpsr_hip_diff *d, const dfpsr_hip_model *
 	            (count == 0 || std::memcmp(d->lastViews.data(), views, (size_t)count * sizeof(dfpsr_hip_view)) == 0);
 	d->frame.setDifferentiable(texture, log2w, log2h, multipliers);
 	if (!same) {
+		rc = checkModelIndices(model, "diff_forward");
+		if (rc) return rc;
 		d->frame.reset(o.tileConfig);
 		rc = const_cast<dfpsr_hip_model *>(model)->upload();
 		if (rc) return rc;

@@ -53,6 +53,7 @@ struct ModelView {
 	const int32_t *fanPolygon;
 	const int32_t *polygonPart;
 	int32_t pointCount, fanCount, partCount, filter;
+	int32_t maxIndex;  // the largest point index of any polygon (-1 without polygons)
 	const TextureDesc *const *partDiffuse;  // per part, null when absent
 	const TextureDesc *const *partLight;
 };

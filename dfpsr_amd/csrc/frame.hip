@@ -100,6 +100,9 @@ int32_t Frame::textureSlot(const TextureDesc *desc) {
 
 int Frame::addTask(const ModelView &m, const float *transform, const CameraParams &camera, int32_t canvas, bool cameraSpacePoints) {
 	if (m.fanCount == 0 || m.pointCount == 0) return 0;
+	// The points may have been replaced by fewer after the polygons were added: a polygon
+	// naming a missing point would send the setup kernel past the projected points.
+	if (m.maxIndex >= m.pointCount) return fail(-1, "giveTask: a polygon names a point the model does not have");
 	TaskDev t;
 	std::memset(&t, 0, sizeof(t));
 	t.points = m.points;

@@ -12,6 +12,9 @@
  *     failure on the calling thread.
  *   - Plain pointers and sizes only.  Arrays are tightly packed, little-endian.
  *   - A renderer is used from one host thread at a time, like the reference's Renderer.
+ *   - A model may be drawn from several threads at once (renderers or batches on different
+ *     threads, dfpsr_hip_model_render_depth into different canvases) while no thread changes
+ *     it; model_set_points and model_add_part must not overlap any draw of that model.
  *   - Colours are packed RGBA8 (red in the lowest byte), the OrderedImageRgbaU8 pack order.
  *   - Arithmetic follows the reference operation for operation: coverage and depth are
  *     bit-exact; RGBA8 colours may differ by 1 LSB where the reference's SSE reciprocal
@@ -110,7 +113,11 @@ int dfpsr_hip_model_bound(const dfpsr_hip_model *model, float minimum[3], float 
  * depth for a perspective camera and the smaller depth for an orthogonal one.  A NULL or
  * pixel-less canvas draws nothing.  Synchronous: the canvas holds the result on return.
  * Rows that the reference's even-rounded row bound puts past an odd canvas height are not
- * drawn (the reference writes them past its image). */
+ * drawn (the reference writes them past its image).  Like the reference's const
+ * model_renderDepth, it may run on several threads at once for one model: its working memory
+ * belongs to the calling thread.  Fails with DFPSR_HIP_ERROR_ARGUMENT, before any device
+ * work, when a polygon names a point that model_set_points has since removed (as do every
+ * other draw of a model: give_model, batch_render, diff_forward). */
 int dfpsr_hip_model_render_depth(const dfpsr_hip_model *model, const float transform[12], const dfpsr_hip_image *depth,
                                  const dfpsr_hip_camera *camera);
 void dfpsr_hip_model_destroy(dfpsr_hip_model *model);

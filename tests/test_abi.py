@@ -65,3 +65,27 @@ def test_null_handles(lib):
     t = C.c_void_p()
     assert lib.dfpsr_hip_texture_create(C.byref(t), 0, 4, 1) == _lib.ERROR_ARGUMENT
     assert lib.dfpsr_hip_texture_create(C.byref(t), 4, 4, 0) == _lib.ERROR_ARGUMENT
+
+
+def test_points_replaced_by_fewer_are_refused_before_drawing(lib):
+    """model_set_points after model_add_part can leave polygons naming missing points; a draw
+    must refuse that before it reaches the device (the depth-only draw is checked here; the
+    renderer, batch and differentiable paths in tests/test_gpu_parity.py)."""
+    m = C.c_void_p()
+    assert lib.dfpsr_hip_model_create(C.byref(m), 0) == 0
+    pts = np.array([[0, 0, 1], [1, 0, 1], [0, 1, 1]], np.float32)
+    assert lib.dfpsr_hip_model_set_points(m, 3, pts.ctypes.data_as(C.c_void_p)) == 0
+    idx = np.array([[0, 1, 2, -1]], np.int32)
+    col = np.ones((1, 16), np.float32)
+    tex = np.zeros((1, 16), np.float32)
+    assert lib.dfpsr_hip_model_add_part(m, 1, idx.ctypes.data_as(C.c_void_p), col.ctypes.data_as(C.c_void_p),
+                                        tex.ctypes.data_as(C.c_void_p), None, None) == 0
+    assert lib.dfpsr_hip_model_set_points(m, 2, pts.ctypes.data_as(C.c_void_p)) == 0
+    canvas = np.zeros((8, 8), np.float32)
+    img = _lib.Image(canvas.ctypes.data, 8, 8, 32, 0)
+    cam = _lib.Camera(1, (C.c_float * 12)(0, 0, -2, 1, 0, 0, 0, 1, 0, 0, 0, 1), 8.0, 8.0, 1.0, 0.01, 100.0)
+    t = (C.c_float * 12)(0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1)
+    rc = lib.dfpsr_hip_model_render_depth(m, t, C.byref(img), C.byref(cam))
+    assert rc == _lib.ERROR_ARGUMENT and b"does not have" in lib.dfpsr_hip_last_error()
+    assert not canvas.any()
+    lib.dfpsr_hip_model_destroy(m)

@@ -296,3 +296,36 @@ def test_setup_block_with_more_commands_than_its_binning_table(api):
     ref = api.render_scene(sc, packed)
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1].view(np.uint32), ref[1].view(np.uint32))
     assert_parity(sc, got, H.oracle_render(sc))
+
+
+def test_points_replaced_by_fewer_are_refused_by_every_draw(api):
+    """A model whose points were replaced by fewer after its polygons were added is refused with
+    an argument error by renderer_giveTask, the batch and the differentiable forward, and the
+    canvases stay untouched."""
+    import torch
+    sc = S.cube_soup(width=32, height=24, triangles=20, seed=4)
+    jobs = api.GpuJobs([sc])
+    model = jobs.gpu[0]
+    api.model_setPoints(model, sc.points[:-3])
+    r = api.renderer_create()
+    color = np.zeros((sc.height, sc.width), np.uint32)
+    api.renderer_begin(r, api.host_image(color), api.host_image(None))
+    cam = api.camera_create(sc.perspective, sc.cam_location, sc.width, sc.height, sc.width_slope, sc.near_clip, sc.far_clip)
+    with pytest.raises(api.DfpsrError) as e:
+        api.renderer_giveTask(r, model, sc.instances[0], cam)
+    assert e.value.code == -1 and "does not have" in str(e.value)
+    api.renderer_end(r)
+    assert not color.any()
+    batch = api.Batch()
+    with pytest.raises(api.DfpsrError) as e:
+        batch.render(jobs.jobs, api.tuning())
+    assert e.value.code == -1
+    torch.cuda.synchronize()
+    assert not jobs.colors[0].any().item()
+    views = api.GpuViews([sc])
+    api.model_setPoints(views.model, sc.points[:-3])
+    diff = api.DiffRenderer()
+    tex = torch.full((8, 8, 4), 0.5, dtype=torch.float32, device="cuda")
+    with pytest.raises(api.DfpsrError) as e:
+        diff.forward(views.model, views.views, tex)
+    assert e.value.code == -1
