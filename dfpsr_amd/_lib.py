@@ -72,6 +72,7 @@ _SIGNATURES = {
     "dfpsr_hip_model_add_part": ([_P, _I, _P, _P, _P, _P, _P], C.c_int),
     "dfpsr_hip_model_bound": ([_P, _P, _P], C.c_int),
     "dfpsr_hip_model_render_depth": ([_P, _P, _P, _P], C.c_int),
+    "dfpsr_hip_model_render_depth_noalloc": ([_P, _P, _P, _P], C.c_int),
     "dfpsr_hip_model_destroy": ([_P], None),
     "dfpsr_hip_renderer_create": ([C.POINTER(_P)], C.c_int),
     "dfpsr_hip_renderer_destroy": ([_P], None),

@@ -313,52 +313,116 @@ static int32_t depthGroupShift() {
 	return shift;
 }
 
+// Working memory of the scratch-free depth-only draw: projected points of one model, in a
+// device array of fixed size that every call on a GPU shares (each GPU holds its own copy
+// until the process ends).  Host code never passes the array's name to a kernel: the address
+// comes from hipGetSymbolAddress, once per device, with that device current.
+static constexpr int32_t NOALLOC_MAX_POINTS = 1 << 18;  // 10 MiB of 40-byte projected points
+static __device__ __align__(256) unsigned char scratch[(size_t)NOALLOC_MAX_POINTS * sizeof(Projected)];
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratchLock;  // serialises set-up and every scratch-free call in the process
+static void *g_scratchAddress[NOALLOC_MAX_DEVICES] = {};
+
+// The calling thread's current device's copy of `scratch` (g_scratchLock held).
+static int scratchAddress(Projected **out) {
+	int device = 0;
+	HIP_TRY(hipGetDevice(&device));
+	if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(DFPSR_HIP_ERROR_DEVICE, "model_render_depth_noalloc: device index beyond 63");
+	if (!g_scratchAddress[device]) {
+		void *ptr = nullptr;
+		HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+		if (!ptr) return fail(DFPSR_HIP_ERROR_DEVICE, "model_render_depth_noalloc: no device address for the scratch array");
+		g_scratchAddress[device] = ptr;
+	}
+	*out = (Projected *)g_scratchAddress[device];
+	return DFPSR_HIP_OK;
+}
+
 // model_renderDepth (modelAPI.cpp:202-206): box cull, project the points, draw every fan
 // triangle's depth.  Synchronous like the reference: the canvas is final on return.
-extern "C" int dfpsr_hip_model_render_depth(const dfpsr_hip_model *cm, const float transform[12], const dfpsr_hip_image *depth,
-                                            const dfpsr_hip_camera *camera) {
-	if (!cm || !transform) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_render_depth: null argument");
+// noAlloc = false: the projected points and a staged host canvas live in the calling
+// thread's growable buffers.  noAlloc = true: the projected points go to the shared scratch
+// array (at most NOALLOC_MAX_POINTS), the canvas must be device memory and the model
+// resident, and nothing is allocated; the lock keeps two calls in one process from using
+// the array at once.
+static int renderDepth(const dfpsr_hip_model *cm, const float transform[12], const dfpsr_hip_image *depth,
+                       const dfpsr_hip_camera *camera, const char *who, bool noAlloc) {
+	if (!cm || !transform) return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": null argument");
 	if (!hasImage(depth)) return DFPSR_HIP_OK;  // no target buffer: nothing to draw (renderCore.cpp:409)
 	if (depth->width < 1 || depth->height < 1 || depth->stride_bytes < depth->width * 4 || (depth->stride_bytes & 3))
-		return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_render_depth: bad canvas size or stride");
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": bad canvas size or stride");
+	if (noAlloc && !depth->on_device)
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": the canvas must be device memory (a host canvas needs a staging allocation)");
 	int rc = checkCamera(camera);
 	if (rc) return rc;
 	CameraParams cam = cameraFrom(camera);
 	const Transform modelToWorld = transformFrom(transform);
-	rc = checkModelIndices(cm, "model_render_depth");
+	rc = checkModelIndices(cm, who);
 	if (rc) return rc;
+	const int32_t pointCount = (int32_t)(cm->points.size() / 3), fanCount = (int32_t)cm->fanPolygon.size();
+	if (noAlloc && pointCount > NOALLOC_MAX_POINTS)
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": the model has " + std::to_string(pointCount) +
+		                                          " points, more than the scratch array's " + std::to_string(NOALLOC_MAX_POINTS));
 	if (boxSeen(cam, cm->minBound, cm->maxBound, modelToWorld) == 0) return DFPSR_HIP_OK;
-	dfpsr_hip_model *m = const_cast<dfpsr_hip_model *>(cm);
-	const int32_t pointCount = (int32_t)(m->points.size() / 3), fanCount = (int32_t)m->fanPolygon.size();
 	if (pointCount == 0 || fanCount == 0) return DFPSR_HIP_OK;
-	rc = m->upload();
-	if (rc) return rc;
+	dfpsr_hip_model *m = const_cast<dfpsr_hip_model *>(cm);
+	if (noAlloc) {
+		if (m->dirty)
+			return fail(DFPSR_HIP_ERROR_STATE, std::string(who) + ": the model has changes that are not on the device yet "
+			                                                      "(any allocating draw of it uploads them)");
+	} else {
+		rc = m->upload();
+		if (rc) return rc;
+	}
+	const int32_t groupShift = depthGroupShift();
+	const int64_t drawThreads = (int64_t)fanCount << groupShift;
+	if (drawThreads > INT32_MAX - 255) return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": too many polygons for one launch");
 	const size_t canvasBytes = (size_t)depth->stride_bytes * depth->height;
-	// Working memory belongs to the calling thread, not to the model: threads drawing one
-	// model into their own canvases never share it, and a call has finished with it (the
-	// call is synchronous) before the same thread's next call can grow it.
-	static thread_local DevBuf projected, stage;
-	if (projected.ensure((size_t)pointCount * sizeof(Projected)) != hipSuccess ||
-	    (!depth->on_device && stage.ensure(canvasBytes) != hipSuccess))
-		return fail(DFPSR_HIP_ERROR_MEMORY, "model_render_depth: device allocation failed");
+	std::unique_lock<std::mutex> hold(g_scratchLock, std::defer_lock);
+	Projected *projected = nullptr;
+	float *stagedCanvas = nullptr;
+	if (noAlloc) {
+		hold.lock();
+		rc = scratchAddress(&projected);
+		if (rc) return rc;
+	} else {
+		// Working memory belongs to the calling thread, not to the model: threads drawing one
+		// model into their own canvases never share it, and a call has finished with it (the
+		// call is synchronous) before the same thread's next call can grow it.
+		static thread_local DevBuf ownProjected, stage;
+		if (ownProjected.ensure((size_t)pointCount * sizeof(Projected)) != hipSuccess ||
+		    (!depth->on_device && stage.ensure(canvasBytes) != hipSuccess))
+			return fail(DFPSR_HIP_ERROR_MEMORY, std::string(who) + ": device allocation failed");
+		projected = ownProjected.as<Projected>();
+		stagedCanvas = stage.as<float>();
+	}
 	DepthCanvas cv;
-	cv.depth = depth->on_device ? (float *)depth->pixels : stage.as<float>();
+	cv.depth = depth->on_device ? (float *)depth->pixels : stagedCanvas;
 	cv.width = depth->width;
 	cv.height = depth->height;
 	cv.stride = depth->stride_bytes / 4;
 	if (!depth->on_device) HIP_TRY(hipMemcpy(cv.depth, depth->pixels, canvasBytes, hipMemcpyHostToDevice));
+	// Every projected point the draw reads ([0, pointCount)) is written first: nothing in the
+	// shared array needs clearing between calls.
 	hipLaunchKernelGGL(k_depth_project, dim3((pointCount + 255) / 256), dim3(256), 0, 0, m->dPoints.as<float>(), pointCount,
-	                   modelToWorld, cam, projected.as<Projected>());
+	                   modelToWorld, cam, projected);
 	HIP_TRY(hipGetLastError());
-	const int32_t groupShift = depthGroupShift();
-	const int64_t drawThreads = (int64_t)fanCount << groupShift;
-	if (drawThreads > INT32_MAX - 255) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_render_depth: too many polygons for one launch");
-	hipLaunchKernelGGL(k_depth_draw, dim3((uint32_t)((drawThreads + 255) / 256)), dim3(256), 0, 0, projected.as<Projected>(),
+	hipLaunchKernelGGL(k_depth_draw, dim3((uint32_t)((drawThreads + 255) / 256)), dim3(256), 0, 0, projected,
 	                   m->dIndices.as<int32_t>(), m->dFanPolygon.as<int32_t>(), fanCount, groupShift, cam, cv);
 	HIP_TRY(hipGetLastError());
 	if (!depth->on_device) HIP_TRY(hipMemcpy(depth->pixels, cv.depth, canvasBytes, hipMemcpyDeviceToHost));
 	HIP_TRY(hipDeviceSynchronize());
 	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_model_render_depth(const dfpsr_hip_model *model, const float transform[12], const dfpsr_hip_image *depth,
+                                            const dfpsr_hip_camera *camera) {
+	return renderDepth(model, transform, depth, camera, "model_render_depth", false);
+}
+
+extern "C" int dfpsr_hip_model_render_depth_noalloc(const dfpsr_hip_model *model, const float transform[12],
+                                                    const dfpsr_hip_image *depth, const dfpsr_hip_camera *camera) {
+	return renderDepth(model, transform, depth, camera, "model_render_depth_noalloc", true);
 }
 
 extern "C" int dfpsr_hip_renderer_create(dfpsr_hip_renderer **out) {

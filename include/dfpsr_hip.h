@@ -120,6 +120,21 @@ int dfpsr_hip_model_bound(const dfpsr_hip_model *model, float minimum[3], float 
  * other draw of a model: give_model, batch_render, diff_forward). */
 int dfpsr_hip_model_render_depth(const dfpsr_hip_model *model, const float transform[12], const dfpsr_hip_image *depth,
                                  const dfpsr_hip_camera *camera);
+/* No counterpart in the reference: dfpsr_hip_model_render_depth for callers that cannot
+ * allocate device memory (code under HIP graph capture elsewhere in the process, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same kernels and the same canvas as
+ * dfpsr_hip_model_render_depth, but it calls no hipMalloc/hipFree: the projected points go to
+ * a fixed device array of 2^18 entries (10 MiB per GPU, held until the process ends).
+ *   - The canvas must be device memory (on_device = 1); a host canvas fails with
+ *     DFPSR_HIP_ERROR_ARGUMENT.
+ *   - The model must already be resident: a model with changes not yet uploaded (never
+ *     drawn, or changed since its last draw) fails with DFPSR_HIP_ERROR_STATE; any allocating
+ *     draw of it uploads them.
+ *   - A model of more than 262 144 points fails with DFPSR_HIP_ERROR_ARGUMENT.
+ * All calls on one GPU share the array and would race on it if they overlapped; the library
+ * serialises them with a process-wide lock.  Synchronous, like model_render_depth. */
+int dfpsr_hip_model_render_depth_noalloc(const dfpsr_hip_model *model, const float transform[12],
+                                         const dfpsr_hip_image *depth, const dfpsr_hip_camera *camera);
 void dfpsr_hip_model_destroy(dfpsr_hip_model *model);
 
 /* ---- Renderer (api/rendererAPI.h) ------------------------------------------------------- */
