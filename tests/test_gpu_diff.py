@@ -26,7 +26,7 @@ FORWARD_FLIP_FRACTION = 1e-3  # ... except where u or v sits on an 8-bit sub-tex
 FORWARD_FLIP_ATOL = 1.0 / 128  # tap weights move by one step of 1/256 (texels and multipliers < 2);
                                # the same 99.9% bar as the RGBA8 path's colours
 ADJOINT_RTOL = 1e-3          # GPU float32 atomics vs the oracle's float64 adjoint
-FD_RTOL = 1e-3               # analytic gradient vs central finite differences (h = 1e-3)
+FD_RTOL = 1e-3               # analytic gradient vs central finite differences (h = 1e-3), no floor
 BATCHED_GRAD_RTOL = 1e-4     # 256-view batched adjoint vs the sum of per-view adjoints
 
 
@@ -216,6 +216,13 @@ def fit_setup(api):
 
 
 def test_config3_finite_differences(api, fit_setup):
+    """Texture gradients of the config-3 loss against central finite differences on 1,000
+    seeded (texel, channel) pairs, h = 1e-3.  The colour is linear in the texture and
+    visibility does not depend on it, so the loss is exactly quadratic along each pair: the
+    only error left is float32 rounding of the rendered images.  The step is the stored
+    difference of the perturbed texel values.  Every pair with a non-zero gradient must agree
+    to 1e-3 relative error, with no floor; texels that no pixel samples must give exactly zero
+    on both sides."""
     gv, r, t, target = fit_setup
     texture = torch.full((512, 512, 4), 0.5, dtype=torch.float32, device="cuda")
     mul = torch.ones((gv.point_count, 3), dtype=torch.float32, device="cuda")
@@ -229,32 +236,90 @@ def test_config3_finite_differences(api, fit_setup):
         g.copy_(2.0 * (img[i] - target[i]))
     gt = torch.zeros_like(texture)
     r.backward(gt, None, api.tuning())
-    analytic = gt.cpu().numpy().reshape(-1)
+    analytic = gt.cpu().numpy().reshape(-1).astype(np.float64)
     rng = np.random.default_rng(1000)
     texels = rng.integers(0, 512 * 512, 1000)
     channels = rng.integers(0, 4, 1000)
     h = 1e-3
-    scale = np.abs(analytic).max()
     target64 = target.double()
-    errors = []
+    errors, zeros = [], 0
     flat = texture.view(-1)
     for tx, ch in zip(texels, channels):
         k = int(tx) * 4 + int(ch)
-        flat[k] += h
+        flat[k] = 0.5 + h
+        plus = float(flat[k].item())
         ip = render()
-        flat[k] -= 2 * h
+        flat[k] = 0.5 - h
+        minus = float(flat[k].item())
         im = render()
-        flat[k] += h
+        flat[k] = 0.5
         # L(T+h) - L(T-h) = sum (ip - im)(ip + im - 2 target): exactly zero where nothing changed.
         num = float(((ip.double() - im.double()) * (ip.double() + im.double() - 2.0 * target64)).sum())
-        fd = num / (2 * h)
+        fd = num / (plus - minus)
         an = float(analytic[k])
-        errors.append(abs(fd - an) / max(abs(an), 1e-3 * scale))
+        if an == 0.0:
+            assert fd == 0.0, f"texel {tx} channel {ch}: no analytic gradient but a finite difference of {fd}"
+            zeros += 1
+            continue
+        errors.append(abs(fd - an) / abs(an))
     errors = np.array(errors)
-    visible = int((np.abs(analytic[texels * 4 + channels]) > 0).sum())
-    print(f"[gpu] fd: max rel err {errors.max():.2e}, median {np.median(errors):.2e}, {visible} of 1000 texels visible",
-          flush=True)
-    assert visible > 100
+    print(f"[gpu] texel fd: max rel err {errors.max():.2e}, median {np.median(errors):.2e}, "
+          f"{len(errors)} of 1000 texels sampled ({zeros} not sampled by any pixel)", flush=True)
+    assert len(errors) > 100
+    assert errors.max() <= FD_RTOL
+
+
+def test_config3_multiplier_finite_differences(api, fit_setup):
+    """Per-point RGB multiplier gradients of the config-3 loss (8 views at 1280x720, texture
+    0.5, multipliers 1) against central finite differences on 1,000 seeded (point, channel)
+    pairs.  The colour is linear in the multipliers and visibility does not depend on them, so
+    the loss is exactly quadratic along each pair and the central difference is the
+    derivative: the only error left is float32 rounding of the rendered images.  The step is
+    the stored difference of the perturbed multipliers (h = 1e-3 before rounding to float32).
+    Every pair with a non-zero gradient must agree to 1e-3 relative error; pairs whose point no
+    view draws must give exactly zero on both sides."""
+    gv, r, t, target = fit_setup
+    texture = torch.full((512, 512, 4), 0.5, dtype=torch.float32, device="cuda")
+    mul = torch.ones((gv.point_count, 3), dtype=torch.float32, device="cuda")
+
+    def render():
+        r.forward(gv.model, gv.views, texture, mul, t)
+        return torch.stack(gv.color).clone()
+
+    img = render()
+    for i, g in enumerate(gv.grad):
+        g.copy_(2.0 * (img[i] - target[i]))
+    gm = torch.zeros_like(mul)
+    r.backward(None, gm, api.tuning())
+    analytic = gm.cpu().numpy().reshape(-1).astype(np.float64)
+    rng = np.random.default_rng(2000)
+    points = rng.integers(0, gv.point_count, 1000)
+    channels = rng.integers(0, 3, 1000)
+    h = 1e-3
+    target64 = target.double()
+    flat = mul.view(-1)
+    errors, zeros = [], 0
+    for p, ch in zip(points, channels):
+        k = int(p) * 3 + int(ch)
+        flat[k] = 1.0 + h
+        plus = float(flat[k].item())
+        ip = render()
+        flat[k] = 1.0 - h
+        minus = float(flat[k].item())
+        im = render()
+        flat[k] = 1.0
+        num = float(((ip.double() - im.double()) * (ip.double() + im.double() - 2.0 * target64)).sum())
+        fd = num / (plus - minus)
+        an = float(analytic[k])
+        if an == 0.0:
+            assert fd == 0.0, f"point {p} channel {ch}: no analytic gradient but a finite difference of {fd}"
+            zeros += 1
+            continue
+        errors.append(abs(fd - an) / abs(an))
+    errors = np.array(errors)
+    print(f"[gpu] multiplier fd: max rel err {errors.max():.2e}, median {np.median(errors):.2e}, "
+          f"{len(errors)} of 1000 pairs drawn ({zeros} not drawn in any view)", flush=True)
+    assert len(errors) > 500
     assert errors.max() <= FD_RTOL
 
 
