@@ -79,6 +79,7 @@ _SIGNATURES = {
     "dfpsr_hip_renderer_begin": ([_P, C.POINTER(Image), C.POINTER(Image)], C.c_int),
     "dfpsr_hip_renderer_give_model": ([_P, _P, _P, C.POINTER(Camera)], C.c_int),
     "dfpsr_hip_renderer_give_triangles": ([_P, _I, _P, _P, _P, _P, _P, _I, C.POINTER(Camera)], C.c_int),
+    "dfpsr_hip_renderer_give_projected_triangles": ([_P, _I, _P, _P, _P, _P, _P, _I, C.POINTER(Camera)], C.c_int),
     "dfpsr_hip_renderer_takes_triangles": ([_P, _P], C.c_int),
     "dfpsr_hip_renderer_end": ([_P], C.c_int),
     "dfpsr_hip_renderer_stats": ([_P, _P], C.c_int),

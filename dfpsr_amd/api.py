@@ -202,6 +202,25 @@ def renderer_giveTask_triangles(renderer: Renderer, camera_space, colors, texcoo
                                                   filter, C.byref(camera)))
 
 
+PROJECTED_DTYPE = np.dtype({"names": ["cs", "is", "flat"], "formats": [(np.float32, 3), (np.float32, 2), (np.int64, 2)],
+                            "offsets": [0, 12, 24], "itemsize": 40})
+
+
+def renderer_giveTask_projected(renderer: Renderer, corners, colors, texcoords, diffuse: Optional[Texture],
+                                light: Optional[Texture], filter: int, camera: Camera) -> None:
+    """renderer_giveTask_triangle with the caller's ProjectedPoints: corners is an (N, 3) array
+    of PROJECTED_DTYPE records (cs, is, flat)."""
+    pp = np.ascontiguousarray(corners)
+    assert pp.dtype == PROJECTED_DTYPE and pp.ndim == 2 and pp.shape[1] == 3
+    co = np.ascontiguousarray(colors, dtype=np.float32).reshape(-1, 12)
+    tc = np.ascontiguousarray(texcoords, dtype=np.float32).reshape(-1, 12)
+    assert co.shape[0] == pp.shape[0] and tc.shape[0] == pp.shape[0]
+    check(lib().dfpsr_hip_renderer_give_projected_triangles(renderer.handle, pp.shape[0], pp.ctypes.data_as(C.c_void_p),
+                                                            co.ctypes.data_as(C.c_void_p), tc.ctypes.data_as(C.c_void_p),
+                                                            diffuse.handle if diffuse else None,
+                                                            light.handle if light else None, filter, C.byref(camera)))
+
+
 def renderer_takesTriangles(renderer: Renderer) -> bool:
     v = C.c_int32()
     check(lib().dfpsr_hip_renderer_takes_triangles(renderer.handle, C.byref(v)))
@@ -238,7 +257,16 @@ class GpuScene:
                                     scene.near_clip, scene.far_clip)
 
     def give(self, renderer: Renderer) -> None:
-        for t in self.scene.instances:
+        sc = self.scene
+        if sc.projected is not None:
+            # One renderer_giveTask_triangle batch per part, in polygon order.
+            for p in sc.parts:
+                sl = slice(p.polygon_start, p.polygon_end)
+                renderer_giveTask_projected(renderer, sc.projected[sl], sc.colors[sl, :3], sc.texcoords[sl, :3],
+                                            self.textures[p.diffuse] if p.diffuse >= 0 else None,
+                                            self.textures[p.light] if p.light >= 0 else None, sc.filter, self.camera)
+            return
+        for t in sc.instances:
             renderer_giveTask(renderer, self.model, t, self.camera)
 
 

@@ -10,6 +10,7 @@
 #include <atomic>
 #include <chrono>
 #include <climits>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -130,6 +131,8 @@ extern "C" void dfpsr_hip_texture_destroy(dfpsr_hip_texture *t) { delete t; }
 struct dfpsr_hip_model {
 	int32_t filter = 0;
 	std::vector<float> points;
+	int32_t pointFloats = 3;  // floats per point: 3 (x, y, z), or 10 for 40-byte ProjectedPoint records
+	int32_t pointCount() const { return (int32_t)(points.size() / (size_t)pointFloats); }
 	f3 minBound = make3(0.0f, 0.0f, 0.0f), maxBound = make3(0.0f, 0.0f, 0.0f);
 	std::vector<int32_t> indices;
 	int32_t maxIndex = -1;  // the largest entry of indices, checked against the point count per draw
@@ -151,7 +154,7 @@ struct dfpsr_hip_model {
 		v.texcoords = dTexcoords.as<float>();
 		v.fanPolygon = dFanPolygon.as<int32_t>();
 		v.polygonPart = dPolygonPart.as<int32_t>();
-		v.pointCount = (int32_t)(points.size() / 3);
+		v.pointCount = pointCount();
 		v.fanCount = (int32_t)fanPolygon.size();
 		v.partCount = (int32_t)partDiffuse.size();
 		v.filter = filter;
@@ -210,7 +213,7 @@ extern "C" int dfpsr_hip_model_add_part(dfpsr_hip_model *m, int32_t count, const
                                         const float *texcoords, const dfpsr_hip_texture *diffuse, const dfpsr_hip_texture *light) {
 	if (!m || count < 0 || (count > 0 && (!indices || !colors || !texcoords)))
 		return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_add_part: bad argument");
-	int32_t points = (int32_t)(m->points.size() / 3);
+	int32_t points = m->pointCount();
 	for (int32_t i = 0; i < count; i++) {
 		const int32_t *ix = indices + (size_t)i * 4;
 		int32_t corners = ix[3] < 0 ? 3 : 4;
@@ -296,7 +299,7 @@ static bool hasImage(const dfpsr_hip_image *im) { return im && im->pixels; }
 // model_set_points may replace the points with fewer after polygons were added; drawing such a
 // model would read projected points past the model's own.  Checked before any device work.
 static int checkModelIndices(const dfpsr_hip_model *m, const char *who) {
-	if (m->maxIndex >= (int32_t)(m->points.size() / 3))
+	if (m->maxIndex >= m->pointCount())
 		return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": a polygon names a point the model does not have");
 	return DFPSR_HIP_OK;
 }
@@ -359,7 +362,7 @@ static int renderDepth(const dfpsr_hip_model *cm, const float transform[12], con
 	const Transform modelToWorld = transformFrom(transform);
 	rc = checkModelIndices(cm, who);
 	if (rc) return rc;
-	const int32_t pointCount = (int32_t)(cm->points.size() / 3), fanCount = (int32_t)cm->fanPolygon.size();
+	const int32_t pointCount = cm->pointCount(), fanCount = (int32_t)cm->fanPolygon.size();
 	if (noAlloc && pointCount > NOALLOC_MAX_POINTS)
 		return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": the model has " + std::to_string(pointCount) +
 		                                          " points, more than the scratch array's " + std::to_string(NOALLOC_MAX_POINTS));
@@ -489,36 +492,86 @@ extern "C" int dfpsr_hip_renderer_give_model(dfpsr_hip_renderer *r, const dfpsr_
 	if (boxSeen(cam, m->minBound, m->maxBound, transformFrom(transform)) == 0) return DFPSR_HIP_OK;
 	rc = const_cast<dfpsr_hip_model *>(m)->upload();
 	if (rc) return rc;
-	return r->frame.addTask(m->view(), transform, cam, 0, false);
+	return r->frame.addTask(m->view(), transform, cam, 0, POINTS_MODEL);
+}
+
+// The per-call model behind both triangle entries: `count` triangles of corners (3t, 3t + 1,
+// 3t + 2) in one part, with the given corner colours and texture coordinates (3 x 4 floats per
+// triangle).  The points are set by the caller.
+static int triangleModel(int32_t count, const float *colors, const float *texcoords, const dfpsr_hip_texture *diffuse,
+                         const dfpsr_hip_texture *light, int32_t filter, std::unique_ptr<dfpsr_hip_model> &out) {
+	std::unique_ptr<dfpsr_hip_model> m(new (std::nothrow) dfpsr_hip_model());
+	if (!m) return fail(DFPSR_HIP_ERROR_MEMORY, "renderer_give_triangles: out of host memory");
+	m->filter = filter;
+	m->indices.resize((size_t)count * 4);
+	m->colors.assign((size_t)count * 16, 0.0f);
+	m->texcoords.assign((size_t)count * 16, 0.0f);
+	for (int32_t t = 0; t < count; t++) {
+		int32_t *ix = &m->indices[(size_t)t * 4];
+		ix[0] = t * 3; ix[1] = t * 3 + 1; ix[2] = t * 3 + 2; ix[3] = -1;
+		std::memcpy(&m->colors[(size_t)t * 16], colors + (size_t)t * 12, 12 * sizeof(float));
+		std::memcpy(&m->texcoords[(size_t)t * 16], texcoords + (size_t)t * 12, 12 * sizeof(float));
+		m->fanPolygon.push_back(t * 2);
+		m->polygonPart.push_back(0);
+	}
+	m->maxIndex = count * 3 - 1;
+	m->partDiffuse.push_back(diffuse ? &diffuse->desc : nullptr);
+	m->partLight.push_back(light ? &light->desc : nullptr);
+	out = std::move(m);
+	return DFPSR_HIP_OK;
+}
+
+static int checkTriangleCall(dfpsr_hip_renderer *r, int32_t count, const void *corners, const float *colors,
+                             const float *texcoords, int32_t filter, const dfpsr_hip_camera *camera, const char *who) {
+	if (!r || count < 0 || (count > 0 && (!corners || !colors || !texcoords)))
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": bad argument");
+	if (count > INT32_MAX / 8) return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": too many triangles for one call");
+	if (!r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "Cannot call renderer_giveTask_triangle before renderer_begin!");
+	if (filter != DFPSR_HIP_FILTER_SOLID && filter != DFPSR_HIP_FILTER_ALPHA) return fail(DFPSR_HIP_ERROR_ARGUMENT, "unknown filter");
+	return checkCamera(camera);
 }
 
 extern "C" int dfpsr_hip_renderer_give_triangles(dfpsr_hip_renderer *r, int32_t count, const float *cs, const float *colors,
                                                  const float *texcoords, const dfpsr_hip_texture *diffuse,
                                                  const dfpsr_hip_texture *light, int32_t filter, const dfpsr_hip_camera *camera) {
-	if (!r || count < 0 || (count > 0 && (!cs || !colors || !texcoords)))
-		return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_give_triangles: bad argument");
-	if (!r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "Cannot call renderer_giveTask_triangle before renderer_begin!");
-	if (filter != DFPSR_HIP_FILTER_SOLID && filter != DFPSR_HIP_FILTER_ALPHA) return fail(DFPSR_HIP_ERROR_ARGUMENT, "unknown filter");
-	int rc = checkCamera(camera);
+	int rc = checkTriangleCall(r, count, cs, colors, texcoords, filter, camera, "renderer_give_triangles");
+	if (rc || count == 0) return rc;
+	std::unique_ptr<dfpsr_hip_model> m;
+	rc = triangleModel(count, colors, texcoords, diffuse, light, filter, m);
 	if (rc) return rc;
-	if (count == 0) return DFPSR_HIP_OK;
-	std::unique_ptr<dfpsr_hip_model> m(new (std::nothrow) dfpsr_hip_model());
-	if (!m) return fail(DFPSR_HIP_ERROR_MEMORY, "renderer_give_triangles: out of host memory");
-	m->filter = filter;
 	m->points.assign(cs, cs + (size_t)count * 9);
-	std::vector<int32_t> idx((size_t)count * 4);
-	std::vector<float> col((size_t)count * 16, 0.0f), tex((size_t)count * 16, 0.0f);
-	for (int32_t t = 0; t < count; t++) {
-		idx[(size_t)t * 4 + 0] = t * 3; idx[(size_t)t * 4 + 1] = t * 3 + 1; idx[(size_t)t * 4 + 2] = t * 3 + 2; idx[(size_t)t * 4 + 3] = -1;
-		std::memcpy(&col[(size_t)t * 16], colors + (size_t)t * 12, 12 * sizeof(float));
-		std::memcpy(&tex[(size_t)t * 16], texcoords + (size_t)t * 12, 12 * sizeof(float));
-	}
-	rc = dfpsr_hip_model_add_part(m.get(), count, idx.data(), col.data(), tex.data(), diffuse, light);
-	if (rc) return rc;
 	rc = m->upload();
 	if (rc) return rc;
 	static const float identity[12] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1};
-	rc = r->frame.addTask(m->view(), identity, cameraFrom(camera), 0, true);
+	rc = r->frame.addTask(m->view(), identity, cameraFrom(camera), 0, POINTS_CAMERA_SPACE);
+	if (rc) return rc;
+	r->triangleModels.push_back(std::move(m));
+	return DFPSR_HIP_OK;
+}
+
+static_assert(sizeof(dfpsr_hip_projected_point) == sizeof(Projected), "ProjectedPoint layout");
+static_assert(offsetof(dfpsr_hip_projected_point, is) == offsetof(Projected, is), "ProjectedPoint layout");
+static_assert(offsetof(dfpsr_hip_projected_point, flat) == offsetof(Projected, flat), "ProjectedPoint layout");
+
+extern "C" int dfpsr_hip_renderer_give_projected_triangles(dfpsr_hip_renderer *r, int32_t count,
+                                                           const dfpsr_hip_projected_point *corners, const float *colors,
+                                                           const float *texcoords, const dfpsr_hip_texture *diffuse,
+                                                           const dfpsr_hip_texture *light, int32_t filter,
+                                                           const dfpsr_hip_camera *camera) {
+	int rc = checkTriangleCall(r, count, corners, colors, texcoords, filter, camera, "renderer_give_projected_triangles");
+	if (rc || count == 0) return rc;
+	std::unique_ptr<dfpsr_hip_model> m;
+	rc = triangleModel(count, colors, texcoords, diffuse, light, filter, m);
+	if (rc) return rc;
+	// The 40-byte records travel as they are: the projection kernel copies them in place of
+	// projecting (POINTS_PROJECTED), so the setup sees exactly the caller's cs, is and flat.
+	const float *raw = reinterpret_cast<const float *>(corners);
+	m->points.assign(raw, raw + (size_t)count * 3 * (sizeof(Projected) / sizeof(float)));
+	m->pointFloats = sizeof(Projected) / sizeof(float);
+	rc = m->upload();
+	if (rc) return rc;
+	static const float identity[12] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+	rc = r->frame.addTask(m->view(), identity, cameraFrom(camera), 0, POINTS_PROJECTED);
 	if (rc) return rc;
 	r->triangleModels.push_back(std::move(m));
 	return DFPSR_HIP_OK;
@@ -634,7 +687,7 @@ extern "C" int dfpsr_hip_batch_render(dfpsr_hip_batch *b, const dfpsr_hip_job *j
 			if (boxSeen(cam, j.model->minBound, j.model->maxBound, transformFrom(j.transform)) == 0) continue;
 			rc = const_cast<dfpsr_hip_model *>(j.model)->upload();
 			if (rc) return rc;
-			rc = b->frame.addTask(j.model->view(), j.transform, cam, slot, false);
+			rc = b->frame.addTask(j.model->view(), j.transform, cam, slot, POINTS_MODEL);
 			if (rc) return rc;
 		}
 		rc = b->frame.prepare();
@@ -766,7 +819,7 @@ extern "C" int dfpsr_hip_diff_forward(dfpsr_hip_diff *d, const dfpsr_hip_model *
 			int32_t slot = d->frame.addCanvas(cv);
 			CameraParams cam = cameraFrom(&v.camera);
 			if (boxSeen(cam, model->minBound, model->maxBound, transformFrom(v.transform)) == 0) continue;
-			rc = d->frame.addTask(model->view(), v.transform, cam, slot, false);
+			rc = d->frame.addTask(model->view(), v.transform, cam, slot, POINTS_MODEL);
 			if (rc) return rc;
 		}
 		rc = d->frame.prepare();

@@ -84,7 +84,7 @@ public:
 	// Starts a new frame whose canvases are cut into tiles of the given configuration.
 	void reset(int32_t tileConfig = 0);
 	int32_t addCanvas(const CanvasHost &canvas);
-	int addTask(const ModelView &model, const float *transform, const CameraParams &camera, int32_t canvas, bool cameraSpacePoints);
+	int addTask(const ModelView &model, const float *transform, const CameraParams &camera, int32_t canvas, int32_t pointKind);
 	// Packs and uploads the frame's tables, then runs the pipeline and waits for it.
 	int render(const RenderOptions &options) {
 		int rc = prepare();

@@ -98,7 +98,7 @@ int32_t Frame::textureSlot(const TextureDesc *desc) {
 	return (int32_t)textures_.size() - 1;
 }
 
-int Frame::addTask(const ModelView &m, const float *transform, const CameraParams &camera, int32_t canvas, bool cameraSpacePoints) {
+int Frame::addTask(const ModelView &m, const float *transform, const CameraParams &camera, int32_t canvas, int32_t pointKind) {
 	if (m.fanCount == 0 || m.pointCount == 0) return 0;
 	// The points may have been replaced by fewer after the polygons were added: a polygon
 	// naming a missing point would send the setup kernel past the projected points.
@@ -118,7 +118,7 @@ int Frame::addTask(const ModelView &m, const float *transform, const CameraParam
 	t.fanBase = taskFanEnd_.empty() ? 0 : taskFanEnd_.back();
 	t.filter = m.filter;
 	t.canvas = canvas;
-	t.cameraSpacePoints = cameraSpacePoints ? 1 : 0;
+	t.pointKind = pointKind;
 	std::memcpy(t.transform, transform, sizeof(t.transform));
 	t.camera = camera;
 	partTextureOffset_.push_back(partTexture_.size());

@@ -18,8 +18,12 @@ __device__ void projectPoint(const FrameDev &f, int32_t i) {
 	int32_t ti = findSegment(f.taskPointEnd, f.taskCount, i);
 	const TaskDev &t = f.tasks[ti];
 	int32_t local = i - t.pointBase;
+	if (t.pointKind == POINTS_PROJECTED) {
+		f.projected[i] = reinterpret_cast<const Projected *>(t.points)[local];
+		return;
+	}
 	f3 p = make3(t.points[local * 3], t.points[local * 3 + 1], t.points[local * 3 + 2]);
-	if (t.cameraSpacePoints) {
+	if (t.pointKind == POINTS_CAMERA_SPACE) {
 		f.projected[i] = cameraToScreen(t.camera, p);
 	} else {
 		Transform m = transformFrom(t.transform);

@@ -16,6 +16,11 @@ DH int32_t tileConfigH(int32_t c) { return c == 2 ? 32 : 16; }
 constexpr int32_t TILE_SORT_CAP = 1024;
 
 // One renderer_giveTask(model, transform, camera) call, as the kernels see it.
+// What a task's points hold: model-space positions (renderer_giveTask of a model), camera-space
+// corners (renderer_giveTask_triangle with the corners' cs), or the caller's ProjectedPoint
+// records of 40 bytes (renderer_giveTask_triangle as the reference takes it).
+enum { POINTS_MODEL = 0, POINTS_CAMERA_SPACE = 1, POINTS_PROJECTED = 2 };
+
 struct TaskDev {
 	const float *points;       // pointCount x 3
 	const int32_t *indices;    // polygons x 4
@@ -28,7 +33,7 @@ struct TaskDev {
 	int32_t pointBase, fanBase;  // offsets into the frame's projected points / fan numbering
 	int32_t filter;              // 0 Solid, 1 Alpha
 	int32_t canvas;              // canvas slot (the batched form renders many canvases)
-	int32_t cameraSpacePoints;   // 1: points are already in camera space (renderer_giveTask_triangle)
+	int32_t pointKind;           // POINTS_MODEL, POINTS_CAMERA_SPACE or POINTS_PROJECTED
 	float transform[12];
 	CameraParams camera;
 };

@@ -96,6 +96,9 @@ class Scene:
     initial_color: Optional[np.ndarray] = None   # (H, W) uint32
     initial_depth: Optional[np.ndarray] = None   # (H, W) float32
     seed: int = 0
+    # Optional (M, 3) records of the caller's ProjectedPoints (cs, is, flat) per triangle corner:
+    # the frame then gives the triangles with renderer_giveTask_triangle instead of the model.
+    projected: Optional[np.ndarray] = None
 
     @property
     def triangle_count(self) -> int:

@@ -158,6 +158,28 @@ int dfpsr_hip_renderer_give_model(dfpsr_hip_renderer *renderer, const dfpsr_hip_
 int dfpsr_hip_renderer_give_triangles(dfpsr_hip_renderer *renderer, int32_t count, const float *camera_space,
                                       const float *colors, const float *texcoords, const dfpsr_hip_texture *diffuse,
                                       const dfpsr_hip_texture *light, int32_t filter, const dfpsr_hip_camera *camera);
+/* ProjectedPoint (implementation/render/ProjectedPoint.h:33-49) in the reference's own 40-byte
+ * layout: the camera-space position, the image-space position in pixels and the position in
+ * 1/256 sub-pixels (truncated toward zero by Camera::cameraToScreen, Camera.h:160-187). */
+typedef struct {
+	float cs[3];
+	float is[2];
+	int64_t flat[2];
+} dfpsr_hip_projected_point;
+
+/* renderer_giveTask_triangle for `count` triangles exactly as the reference takes them
+ * (api/rendererAPI.h:123-128, rendererAPI.cpp:503-519 -> renderTriangleFromData,
+ * renderCore.cpp:284-341): corners = count x 3 of the caller's ProjectedPoints, used as given
+ * (cs for culling and clipping, is for the weight planes, flat for coverage and the
+ * front-face test), so a caller whose flat or is is not cameraToScreen(cs) draws as the
+ * reference draws it.  Sub-triangles of a clipped triangle are projected again from their
+ * camera-space corners, as in the reference (renderCore.cpp:220-240).
+ * colors / texcoords: count x 3 corners x 4 floats. */
+int dfpsr_hip_renderer_give_projected_triangles(dfpsr_hip_renderer *renderer, int32_t count,
+                                                const dfpsr_hip_projected_point *corners, const float *colors,
+                                                const float *texcoords, const dfpsr_hip_texture *diffuse,
+                                                const dfpsr_hip_texture *light, int32_t filter,
+                                                const dfpsr_hip_camera *camera);
 /* renderer_takesTriangles (api/rendererAPI.h:73). */
 int dfpsr_hip_renderer_takes_triangles(const dfpsr_hip_renderer *renderer, int32_t *out);
 /* renderer_end (api/rendererAPI.h:135, RendererImpl::endFrame rendererAPI.cpp:352-402):

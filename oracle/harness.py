@@ -18,6 +18,11 @@ ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libdfpsr_ref.so")
 
 
+# ProjectedPoint (ProjectedPoint.h:33-49) as 40-byte records: cs xyz, is xy, flat xy.
+PROJECTED_DTYPE = np.dtype({"names": ["cs", "is", "flat"], "formats": [(np.float32, 3), (np.float32, 2), (np.int64, 2)],
+                            "offsets": [0, 12, 24], "itemsize": 40})
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("width", C.c_int32), ("height", C.c_int32),
@@ -34,6 +39,7 @@ class SceneDesc(C.Structure):
         ("texture_count", C.c_int32), ("texture_dims", C.c_void_p), ("texture_level0", C.c_void_p),
         ("instance_count", C.c_int32), ("instance_transforms", C.c_void_p),
         ("threaded", C.c_int32),
+        ("projected", C.c_void_p),
     ]
 
 
@@ -82,6 +88,12 @@ def make_desc(scene, threaded: bool = True) -> Tuple[SceneDesc, list]:
     d.instance_count = scene.instances.shape[0]
     d.instance_transforms = _ptr(arr(scene.instances, np.float32))
     d.threaded = int(threaded)
+    proj = getattr(scene, "projected", None)
+    if proj is not None:
+        a = np.ascontiguousarray(proj)
+        assert a.dtype == PROJECTED_DTYPE and a.shape == (scene.indices.shape[0], 3)
+        keep.append(a)
+        d.projected = a.ctypes.data_as(C.c_void_p)
     return d, keep
 
 

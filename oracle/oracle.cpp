@@ -1142,9 +1142,46 @@ void loadCanvas(const scene_desc *s, Frame &fr) {
 	}
 }
 
+// renderer_giveTask_triangle (rendererAPI.cpp:503-519) per polygon with the caller's
+// ProjectedPoints: no box cull and no projection, straight into renderTriangleFromData.
+void giveProjectedTriangles(Frame &fr, const scene_desc *s) {
+	const uint8_t *records = (const uint8_t *)s->projected;
+	int32_t source = 0;
+	for (int32_t part = 0; part < s->part_count; part++) {
+		for (int32_t poly = s->part_polygon_start[part]; poly < s->part_polygon_start[part + 1]; poly++) {
+			Projected p[3];
+			for (int32_t c = 0; c < 3; c++) {
+				const uint8_t *r = records + ((size_t)poly * 3 + c) * 40;
+				float f[5];
+				int64_t flat[2];
+				memcpy(f, r, sizeof(f));
+				memcpy(flat, r + 24, sizeof(flat));
+				p[c].cs = V3{f[0], f[1], f[2]};
+				p[c].is = V2{f[3], f[4]};
+				p[c].flat = L2{flat[0], flat[1]};
+			}
+			const float *col = s->colors + poly * 16;
+			const float *tc = s->texcoords + poly * 16;
+			TriangleInput in;
+			in.red = V3{col[0], col[4], col[8]}; in.green = V3{col[1], col[5], col[9]};
+			in.blue = V3{col[2], col[6], col[10]}; in.alpha = V3{col[3], col[7], col[11]};
+			in.u1 = V3{tc[0], tc[4], tc[8]}; in.v1 = V3{tc[1], tc[5], tc[9]};
+			in.u2 = V3{tc[2], tc[6], tc[10]}; in.v2 = V3{tc[3], tc[7], tc[11]};
+			in.diffuse = s->part_diffuse ? s->part_diffuse[part] : -1;
+			in.light = s->part_light ? s->part_light[part] : -1;
+			in.point[0] = poly * 3; in.point[1] = poly * 3 + 1; in.point[2] = poly * 3 + 2;
+			giveTriangle(fr, p[0], p[1], p[2], in, source++);
+		}
+	}
+}
+
 void giveAll(const scene_desc *s, Frame &fr) {
 	fr.commands.clear();
 	fr.inputs.clear();
+	if (s->projected) {
+		giveProjectedTriangles(fr, s);
+		return;
+	}
 	int32_t source = 0;
 	for (int32_t i = 0; i < s->instance_count; i++) giveModel(fr, s, transformFrom(s->instance_transforms + i * 12), i, source);
 }

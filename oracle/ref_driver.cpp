@@ -109,8 +109,43 @@ static void loadCanvas(const scene_desc *s, BuiltScene &b) {
 	}
 }
 
+// renderer_giveTask_triangle per polygon with the scene's own ProjectedPoints.
+static void giveProjectedTriangles(const scene_desc *s, BuiltScene &b, Renderer &renderer) {
+	const uint8_t *records = (const uint8_t *)s->projected;
+	TextureRgbaU8 none;
+	for (int32_t part = 0; part < s->part_count; part++) {
+		const TextureRgbaU8 &diffuse = (s->part_diffuse && s->part_diffuse[part] >= 0) ? b.textures[s->part_diffuse[part]] : none;
+		const TextureRgbaU8 &light = (s->part_light && s->part_light[part] >= 0) ? b.textures[s->part_light[part]] : none;
+		for (int32_t poly = s->part_polygon_start[part]; poly < s->part_polygon_start[part + 1]; poly++) {
+			ProjectedPoint p[3];
+			FVector4D col[3], tc[3];
+			for (int32_t c = 0; c < 3; c++) {
+				const uint8_t *r = records + ((size_t)poly * 3 + c) * 40;
+				float f[5];
+				int64_t flat[2];
+				memcpy(f, r, sizeof(f));
+				memcpy(flat, r + 24, sizeof(flat));
+				p[c] = ProjectedPoint(FVector3D(f[0], f[1], f[2]), FVector2D(f[3], f[4]), LVector2D(flat[0], flat[1]));
+				const float *cc = s->colors + poly * 16 + c * 4;
+				const float *tt = s->texcoords + poly * 16 + c * 4;
+				col[c] = FVector4D(cc[0], cc[1], cc[2], cc[3]);
+				tc[c] = FVector4D(tt[0], tt[1], tt[2], tt[3]);
+			}
+			renderer_giveTask_triangle(renderer, p[0], p[1], p[2], col[0], col[1], col[2], tc[0], tc[1], tc[2], diffuse, light,
+			                           s->filter ? Filter::Alpha : Filter::Solid, b.camera);
+		}
+	}
+}
+
 static void renderFrame(const scene_desc *s, BuiltScene &b) {
-	if (s->threaded) {
+	if (s->projected) {
+		Renderer renderer = renderer_create();
+		ImageRgbaU8 color = b.color;
+		ImageF32 depth = b.depth;
+		renderer_begin(renderer, color, depth);
+		giveProjectedTriangles(s, b, renderer);
+		renderer_end(renderer);
+	} else if (s->threaded) {
 		Renderer renderer = renderer_create();
 		ImageRgbaU8 color = b.color;
 		ImageF32 depth = b.depth;

@@ -54,6 +54,12 @@ typedef struct {
 	const float *instance_transforms;  /* instance_count x 12, same layout as cam_location */
 	/* 1 = renderer_create/begin/giveTask/end (12 row slices), 0 = model_render (one thread). */
 	int32_t threaded;
+	/* Optional: the polygons (triangles only) are given as renderer_giveTask_triangle calls
+	 * (rendererAPI.h:123-128) with the caller's ProjectedPoints instead of renderer_giveTask of
+	 * the model: polygon_count x 3 records of 40 bytes in the ProjectedPoint layout (cs xyz and
+	 * is xy as float, flat xy as int64).  Colours, texture coordinates and each part's textures
+	 * come from the model's arrays; points and instances are not used.  NULL: the model path. */
+	const void *projected;
 } scene_desc;
 
 #ifdef __cplusplus

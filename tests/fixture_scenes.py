@@ -12,6 +12,10 @@ import numpy as np
 
 from dfpsr_amd import scenes as S
 
+# ProjectedPoint records (ProjectedPoint.h:33-49): cs xyz, is xy, flat xy; 40 bytes.
+H_PROJECTED = np.dtype({"names": ["cs", "is", "flat"], "formats": [(np.float32, 3), (np.float32, 2), (np.int64, 2)],
+                        "offsets": [0, 12, 24], "itemsize": 40})
+
 
 def variant(seed: int) -> S.Scene:
     """Random small scene; the kind of edge case is picked by seed % 7."""
@@ -199,6 +203,70 @@ def shader_branch_scenes():
     return out
 
 
+def camera_space_corners(scene: S.Scene) -> np.ndarray:
+    """(M, 3, 3) camera-space corners of the scene's triangles (first instance)."""
+    loc = np.asarray(scene.cam_location, dtype=np.float64)
+    axes = loc[3:].reshape(3, 3)
+    world = scene.points.astype(np.float64)[scene.indices[:, :3]]
+    return ((world - loc[:3]) @ axes.T).astype(np.float32)
+
+
+def projected_triangles(scene: S.Scene, seed: int, moved: float = 0.4) -> S.Scene:
+    """The scene given as renderer_giveTask_triangle calls with the caller's ProjectedPoints:
+    cs from the camera, is and flat from the pinhole projection (Camera.h:160-187) in float64,
+    then on a `moved` share of the corners flat moved by up to 64 sub-pixels and is by up to a
+    fifth of a pixel, so that coverage and weights no longer follow cameraToScreen(cs)."""
+    import copy
+    sc = copy.copy(scene)
+    rng = np.random.default_rng(seed)
+    cs = camera_space_corners(scene).astype(np.float64)
+    w, h = scene.width, scene.height
+    ws = scene.width_slope
+    hs = ws * h / w
+    z = cs[..., 2]
+    inv = np.where(z > 0, 1.0 / np.where(z > 0, z, 1.0), 0.0)
+    x = (cs[..., 0] * (0.5 / ws) + z * 0.5) * w * inv
+    y = (-cs[..., 1] * (0.5 / hs) + z * 0.5) * h * inv
+    rec = np.zeros(cs.shape[:2], dtype=H_PROJECTED)
+    rec["cs"] = cs.astype(np.float32)
+    rec["is"][..., 0] = x.astype(np.float32)
+    rec["is"][..., 1] = y.astype(np.float32)
+    flat = np.stack([np.trunc(x * 256.0), np.trunc(y * 256.0)], -1)
+    flat = np.clip(np.nan_to_num(flat), -2.0 ** 40, 2.0 ** 40).astype(np.int64)
+    pick = rng.uniform(size=cs.shape[:2]) < moved
+    flat += np.where(pick[..., None], rng.integers(-64, 65, size=flat.shape), 0)
+    rec["flat"] = flat
+    pick = rng.uniform(size=cs.shape[:2]) < moved
+    rec["is"] += np.where(pick[..., None], rng.uniform(-0.2, 0.2, size=(*cs.shape[:2], 2)), 0.0).astype(np.float32)
+    sc.projected = rec
+    sc.instances = S.identity_transform()[None, :]
+    sc.name = scene.name + "_projected"
+    return sc
+
+
+def projected_scenes():
+    """renderer_giveTask_triangle with ProjectedPoints that differ from cameraToScreen(cs):
+    a vertex-coloured soup, a textured soup over a diffuse and a light part, triangles crossing
+    the near plane and the sides (clipped sub-triangles are projected again from cs), and the
+    Alpha filter."""
+    out = []
+    out.append(projected_triangles(S.cube_soup(width=120, height=90, triangles=300, seed=41, spread=0.3, name="proj_soup"), 1))
+    n = 240
+    sc = S.cube_soup(width=112, height=84, triangles=n, seed=42, spread=0.35, name="proj_textured")
+    rng = S.Rng(43)
+    sc.textures = [S.noise_checker_texture(64, 44, 7, check=8), S.noise_checker_texture(16, 45, 1, check=2)]
+    sc.parts = [S.Part(0, n // 2, diffuse=0), S.Part(n // 2, n, light=1)]
+    sc.texcoords[:, :3, :] = rng.uniform(n * 12, -1.5, 2.5).reshape(n, 3, 4)
+    out.append(projected_triangles(sc, 2))
+    out.append(projected_triangles(S.cube_soup(width=80, height=60, triangles=400, seed=79, spread=0.4, cam_distance=1.1,
+                                               width_slope=0.6, name="proj_clipped"), 3))
+    sc = S.cube_soup(width=96, height=72, triangles=200, seed=46, spread=0.3, name="proj_alpha")
+    sc.filter = 1
+    sc.colors[:, :, 3] = S.Rng(47).uniform(200 * 4).reshape(200, 4)
+    out.append(projected_triangles(sc, 4))
+    return out
+
+
 def id_color(scene: S.Scene) -> S.Scene:
     """Same geometry with one constant colour per polygon and no textures: selects the
     "Single color" shader (RgbaMultiply.h:170-171), so the colour buffer is a triangle-id map
@@ -223,6 +291,7 @@ def fixture_scenes():
     out = micro_scenes() + clip_scenes() + [variant(s) for s in range(40)] + filter_scenes()
     out += [id_color(s) for s in clip_scenes()]
     out += shader_branch_scenes()
+    out += projected_scenes()
     return out
 
 

@@ -118,6 +118,47 @@ def test_give_triangles_matches_model_path(api, renderer):
     assert np.array_equal(color, model_color)
 
 
+def projected_by_oracle(sc):
+    """The scene's triangles as ProjectedPoint records that the oracle's cameraToScreen makes
+    from their camera-space corners (bit-exact to the reference's own projection)."""
+    import copy
+    cs, is_, flat = H.oracle_project(sc, S.identity_transform(), sc.points)
+    n = sc.polygon_count
+    rec = np.zeros((n, 3), fixture_scenes.H_PROJECTED)
+    rec["cs"], rec["is"], rec["flat"] = cs.reshape(n, 3, 3), is_.reshape(n, 3, 2), flat.reshape(n, 3, 2)
+    out = copy.copy(sc)
+    out.projected = rec
+    return out
+
+
+def test_give_triangles_matches_oracle(api, renderer):
+    """renderer_giveTask_triangle with camera-space corners (the device projects them) draws
+    what the oracle's renderer_giveTask_triangle draws from the reference's projection of the
+    same corners; the ProjectedPoint entry given those projections draws the same canvases."""
+    n = 300
+    sc = S.cube_soup(width=160, height=120, triangles=n, seed=21, cam_distance=1.2, spread=0.5)
+    sc.textures = [S.noise_checker_texture(64, 21, 7, check=8)]
+    sc.parts = [S.Part(0, n, diffuse=0)]
+    sc.texcoords[:, :3, :2] = S.Rng(22).uniform(n * 6, -1, 2).reshape(n, 3, 2)
+    ref = projected_by_oracle(sc)
+    want = H.oracle_render(ref)
+    g = api.GpuScene(sc)
+    cam = api.camera_create(True, sc.cam_location, sc.width, sc.height, sc.width_slope, sc.near_clip, sc.far_clip)
+    for entry in ("camera_space", "projected"):
+        color = np.zeros((sc.height, sc.width), np.uint32)
+        depth = np.zeros((sc.height, sc.width), np.float32)
+        api.renderer_begin(renderer, api.host_image(color), api.host_image(depth))
+        if entry == "camera_space":
+            api.renderer_giveTask_triangles(renderer, ref.projected["cs"], sc.colors[:, :3, :], sc.texcoords[:, :3, :],
+                                            g.textures[0], None, 0, cam)
+        else:
+            api.renderer_giveTask_projected(renderer, ref.projected, sc.colors[:, :3, :], sc.texcoords[:, :3, :],
+                                            g.textures[0], None, 0, cam)
+        api.renderer_end(renderer)
+        assert (depth > 0).mean() > 0.2
+        assert_parity(sc, (color, depth), want)
+
+
 def test_device_canvases_match_host_canvases(api, renderer):
     import torch
     sc = fixture_scenes.clip_scenes()[2]

@@ -377,3 +377,23 @@ def test_depth_only_special_canvas_values_match_reference(index):
     assert (got.view(np.uint32) != sc.initial_depth.view(np.uint32)).sum() > 50
     bad = got.view(np.uint32) != want.view(np.uint32)
     assert not bad.any(), f"{int(bad.sum())} depth pixels differ, first at {np.argwhere(bad)[:3].tolist()}"
+
+
+def test_projected_scenes_move_coverage():
+    """The ProjectedPoint fixtures really give corners whose flat and is differ from
+    cameraToScreen(cs): their canvases differ from the same triangles projected by the
+    reference's own cameraToScreen (the oracle's, pinned by the goldens above)."""
+    import copy
+    for sc in fixture_scenes.projected_scenes():
+        world = sc.points[sc.indices[:, :3]].reshape(-1, 3)
+        cs, is_, flat = H.oracle_project(sc, S.identity_transform(), world)
+        assert np.allclose(cs.reshape(-1, 3, 3), sc.projected["cs"], atol=1e-5), sc.name
+        exact = copy.copy(sc)
+        rec = sc.projected.copy()
+        rec["is"], rec["flat"] = is_.reshape(-1, 3, 2), flat.reshape(-1, 3, 2)
+        exact.projected = rec
+        moved = int((rec["flat"] != sc.projected["flat"]).any(-1).sum())
+        assert moved > sc.polygon_count // 2, sc.name
+        a = H.oracle_render(sc)
+        b = H.oracle_render(exact)
+        assert not np.array_equal(a[0], b[0]), sc.name
