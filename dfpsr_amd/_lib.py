@@ -80,6 +80,12 @@ _SIGNATURES = {
     "dfpsr_hip_renderer_give_model": ([_P, _P, _P, C.POINTER(Camera)], C.c_int),
     "dfpsr_hip_renderer_give_triangles": ([_P, _I, _P, _P, _P, _P, _P, _I, C.POINTER(Camera)], C.c_int),
     "dfpsr_hip_renderer_give_projected_triangles": ([_P, _I, _P, _P, _P, _P, _P, _I, C.POINTER(Camera)], C.c_int),
+    "dfpsr_hip_renderer_occlude_from_box": ([_P, _P, _P, _P, C.POINTER(Camera)], C.c_int),
+    "dfpsr_hip_renderer_occlude_from_top_rows": ([_P, C.POINTER(Camera)], C.c_int),
+    "dfpsr_hip_renderer_occlude_from_existing_triangles": ([_P], C.c_int),
+    "dfpsr_hip_renderer_is_box_visible": ([_P, _P, _P, _P, C.POINTER(Camera), _P], C.c_int),
+    "dfpsr_hip_renderer_has_occluders": ([_P, _P], C.c_int),
+    "dfpsr_hip_renderer_occlusion_grid": ([_P, _P, _P, _P], C.c_int),
     "dfpsr_hip_renderer_takes_triangles": ([_P, _P], C.c_int),
     "dfpsr_hip_renderer_end": ([_P], C.c_int),
     "dfpsr_hip_renderer_stats": ([_P, _P], C.c_int),

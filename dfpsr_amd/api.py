@@ -221,6 +221,47 @@ def renderer_giveTask_projected(renderer: Renderer, corners, colors, texcoords, 
                                                             light.handle if light else None, filter, C.byref(camera)))
 
 
+def _vec3(v):
+    return np.ascontiguousarray(v, dtype=np.float32).reshape(3)
+
+
+def renderer_occludeFromBox(renderer: Renderer, minimum, maximum, transform, camera: Camera) -> None:
+    mn, mx, t = _vec3(minimum), _vec3(maximum), np.ascontiguousarray(transform, dtype=np.float32)
+    check(lib().dfpsr_hip_renderer_occlude_from_box(renderer.handle, mn.ctypes.data_as(C.c_void_p), mx.ctypes.data_as(C.c_void_p),
+                                                    t.ctypes.data_as(C.c_void_p), C.byref(camera)))
+
+
+def renderer_occludeFromTopRows(renderer: Renderer, camera: Camera) -> None:
+    check(lib().dfpsr_hip_renderer_occlude_from_top_rows(renderer.handle, C.byref(camera)))
+
+
+def renderer_occludeFromExistingTriangles(renderer: Renderer) -> None:
+    check(lib().dfpsr_hip_renderer_occlude_from_existing_triangles(renderer.handle))
+
+
+def renderer_isBoxVisible(renderer: Renderer, minimum, maximum, transform, camera: Camera) -> bool:
+    mn, mx, t = _vec3(minimum), _vec3(maximum), np.ascontiguousarray(transform, dtype=np.float32)
+    out = C.c_int32()
+    check(lib().dfpsr_hip_renderer_is_box_visible(renderer.handle, mn.ctypes.data_as(C.c_void_p), mx.ctypes.data_as(C.c_void_p),
+                                                  t.ctypes.data_as(C.c_void_p), C.byref(camera), C.byref(out)))
+    return bool(out.value)
+
+
+def renderer_hasOccluders(renderer: Renderer) -> bool:
+    out = C.c_int32()
+    check(lib().dfpsr_hip_renderer_has_occluders(renderer.handle, C.byref(out)))
+    return bool(out.value)
+
+
+def renderer_occlusionGrid(renderer: Renderer) -> np.ndarray:
+    """The renderer's occlusion cells (16 x 16 pixels each) as they are now."""
+    w, h = C.c_int32(), C.c_int32()
+    check(lib().dfpsr_hip_renderer_occlusion_grid(renderer.handle, None, C.byref(w), C.byref(h)))
+    out = np.zeros((h.value, w.value), np.float32)
+    check(lib().dfpsr_hip_renderer_occlusion_grid(renderer.handle, out.ctypes.data_as(C.c_void_p), C.byref(w), C.byref(h)))
+    return out
+
+
 def renderer_takesTriangles(renderer: Renderer) -> bool:
     v = C.c_int32()
     check(lib().dfpsr_hip_renderer_takes_triangles(renderer.handle, C.byref(v)))
@@ -266,8 +307,26 @@ class GpuScene:
                                             self.textures[p.diffuse] if p.diffuse >= 0 else None,
                                             self.textures[p.light] if p.light >= 0 else None, sc.filter, self.camera)
             return
-        for t in sc.instances:
+        ops = sc.occluders or []
+        self.visible = []
+
+        def occluders_before(i):
+            for op in ops:
+                if op[1] != i:
+                    continue
+                if op[0] == "box":
+                    renderer_occludeFromBox(renderer, op[2], op[3], op[4], self.camera)
+                elif op[0] == "top_rows":
+                    renderer_occludeFromTopRows(renderer, self.camera)
+                elif op[0] == "existing":
+                    renderer_occludeFromExistingTriangles(renderer)
+                elif op[0] == "query":
+                    self.visible.append(renderer_isBoxVisible(renderer, op[2], op[3], op[4], self.camera))
+
+        for i, t in enumerate(sc.instances):
+            occluders_before(i)
             renderer_giveTask(renderer, self.model, t, self.camera)
+        occluders_before(len(sc.instances))
 
 
 def _tile_level0(t) -> np.ndarray:

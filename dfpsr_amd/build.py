@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdfpsr_hip.so")
 SOURCES = ["api.hip", "frame.hip", "kernels.hip", "depth.hip"]
-HEADERS = ["geometry.h", "shading.h", "pipeline.h", "frame.h"]
+HEADERS = ["geometry.h", "occlusion.h", "shading.h", "pipeline.h", "frame.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

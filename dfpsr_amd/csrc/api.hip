@@ -5,6 +5,7 @@
 // per-triangle and per-pixel work runs in the kernels of kernels.hip at renderer_end.
 #include "../../include/dfpsr_hip.h"
 #include "frame.h"
+#include "occlusion.h"
 
 #include <algorithm>
 #include <atomic>
@@ -259,6 +260,52 @@ struct dfpsr_hip_renderer {
 	DevBuf colorStage, depthStage;
 	std::vector<std::unique_ptr<dfpsr_hip_model>> triangleModels;  // giveTask_triangle batches of this frame
 	RenderOptions options;
+	// Occlusion grid (RendererImpl's depthGrid, rendererAPI.cpp:140-188).  Like the reference's
+	// grid image it outlives the frame: it is made at the first occluder call that finds none
+	// large enough, and refilled with +inf by the first occluder call of each frame, so a box
+	// query before that call reads what an earlier frame left (or 0 while there is none).
+	// The frame's cells are the top-left gridW x gridH of the image (imageW cells per row).
+	std::vector<float> gridImage;
+	int32_t imageW = 0, imageH = 0;
+	int32_t gridW = 0, gridH = 0;
+	bool occluded = false;  // an occluder call since renderer_begin (renderer_hasOccluders)
+	std::vector<float> cells;  // the frame's cells packed for the device
+	DevBuf dGrid;
+
+	const float *gridRead() const { return gridImage.empty() ? nullptr : gridImage.data(); }
+	// prepareForOcclusion (rendererAPI.cpp:179-188).
+	void prepareForOcclusion() {
+		if (!occluded) {
+			if (!(!gridImage.empty() && imageW >= gridW && imageH >= gridH)) {
+				imageW = gridW;
+				imageH = gridH;
+				gridImage.assign((size_t)imageW * imageH, 0.0f);
+			}
+			std::fill(gridImage.begin(), gridImage.end(), __builtin_huge_valf());
+		}
+		occluded = true;
+	}
+	float *deviceCanvasDepth() const {
+		return depth.pixels ? (float *)(depth.on_device ? depth.pixels : depthStage.ptr) : nullptr;
+	}
+	// The frame's cells to the device (packed gridW x gridH) and back into the image.
+	int uploadGrid() {
+		cells.resize((size_t)gridW * gridH);
+		for (int32_t y = 0; y < gridH; y++)
+			std::memcpy(&cells[(size_t)y * gridW], &gridImage[(size_t)y * imageW], (size_t)gridW * sizeof(float));
+		const size_t bytes = cells.size() * sizeof(float);
+		if (dGrid.ensure(bytes) != hipSuccess) return fail(DFPSR_HIP_ERROR_MEMORY, "occlusion grid: device allocation failed");
+		if (bytes) HIP_TRY(hipMemcpyAsync(dGrid.ptr, cells.data(), bytes, hipMemcpyHostToDevice, frame.stream));
+		return DFPSR_HIP_OK;
+	}
+	int downloadGrid() {
+		const size_t bytes = cells.size() * sizeof(float);
+		if (bytes) HIP_TRY(hipMemcpyAsync(cells.data(), dGrid.ptr, bytes, hipMemcpyDeviceToHost, frame.stream));
+		HIP_TRY(hipStreamSynchronize(frame.stream));
+		for (int32_t y = 0; y < gridH; y++)
+			std::memcpy(&gridImage[(size_t)y * imageW], &cells[(size_t)y * gridW], (size_t)gridW * sizeof(float));
+		return DFPSR_HIP_OK;
+	}
 };
 
 static RenderOptions optionsFrom(const dfpsr_hip_tuning *t) {
@@ -475,6 +522,10 @@ extern "C" int dfpsr_hip_renderer_begin(dfpsr_hip_renderer *r, const dfpsr_hip_i
 	r->frame.reset(r->options.tileConfig);
 	r->frame.addCanvas(cv);
 	r->triangleModels.clear();
+	// One grid cell per 16 x 16 pixels, rounded up (rendererAPI.cpp:162-163).
+	r->gridW = (r->width + (OCCLUSION_CELL - 1)) / OCCLUSION_CELL;
+	r->gridH = (r->height + (OCCLUSION_CELL - 1)) / OCCLUSION_CELL;
+	r->occluded = false;
 	r->receiving = true;
 	return DFPSR_HIP_OK;
 }
@@ -490,6 +541,9 @@ extern "C" int dfpsr_hip_renderer_give_model(dfpsr_hip_renderer *r, const dfpsr_
 	CameraParams cam = cameraFrom(camera);
 	// Camera::isBoxSeen on the model bound (modelAPI.cpp:227).
 	if (boxSeen(cam, m->minBound, m->maxBound, transformFrom(transform)) == 0) return DFPSR_HIP_OK;
+	// Whole models behind the occluders given so far are skipped (modelAPI.cpp:229-233).
+	if (r->occluded && boxOccluded(r->gridRead(), r->imageW, r->gridW, r->gridH, m->minBound, m->maxBound, transformFrom(transform), cam))
+		return DFPSR_HIP_OK;
 	rc = const_cast<dfpsr_hip_model *>(m)->upload();
 	if (rc) return rc;
 	return r->frame.addTask(m->view(), transform, cam, 0, POINTS_MODEL);
@@ -587,6 +641,13 @@ extern "C" int dfpsr_hip_renderer_end(dfpsr_hip_renderer *r) {
 	if (!r) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_end: null renderer");
 	if (!r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "Called renderer_end without renderer_begin!");
 	r->receiving = false;
+	// completeOcclusion (rendererAPI.cpp:190-215) runs inside the setup kernels: with occluders,
+	// each draw command is tested against the frame's final grid.
+	if (r->occluded) {
+		int rc = r->uploadGrid();
+		if (rc) return rc;
+		r->frame.setOcclusion(r->dGrid.as<float>(), r->gridW, r->gridH);
+	}
 	int rc = r->frame.render(r->options);
 	if (rc) return rc;
 	if (hasImage(&r->color) && !r->color.on_device)
@@ -594,6 +655,84 @@ extern "C" int dfpsr_hip_renderer_end(dfpsr_hip_renderer *r) {
 	if (hasImage(&r->depth) && !r->depth.on_device)
 		HIP_TRY(hipMemcpy(r->depth.pixels, r->depthStage.ptr, (size_t)r->depth.stride_bytes * r->depth.height, hipMemcpyDeviceToHost));
 	r->triangleModels.clear();
+	return DFPSR_HIP_OK;
+}
+
+// ---- Occlusion (rendererAPI.h:77-95) ----
+
+static int checkBoxCall(const dfpsr_hip_renderer *r, const float *mn, const float *mx, const float *transform,
+                        const dfpsr_hip_camera *camera, const char *who) {
+	if (!r || !mn || !mx || !transform) return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": null argument");
+	if (!r->receiving) return fail(DFPSR_HIP_ERROR_STATE, std::string("Cannot call ") + who + " without first calling renderer_begin!");
+	return checkCamera(camera);
+}
+
+extern "C" int dfpsr_hip_renderer_occlude_from_box(dfpsr_hip_renderer *r, const float minimum[3], const float maximum[3],
+                                                   const float transform[12], const dfpsr_hip_camera *camera) {
+	int rc = checkBoxCall(r, minimum, maximum, transform, camera, "renderer_occludeFromBox");
+	if (rc) return rc;
+	r->prepareForOcclusion();
+	occludeFromBox(r->gridImage.data(), r->imageW, r->gridW, r->gridH, make3(minimum[0], minimum[1], minimum[2]),
+	               make3(maximum[0], maximum[1], maximum[2]), transformFrom(transform), cameraFrom(camera));
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_occlude_from_top_rows(dfpsr_hip_renderer *r, const dfpsr_hip_camera *camera) {
+	if (!r) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_occludeFromTopRows: null renderer");
+	if (!r->receiving) return fail(DFPSR_HIP_ERROR_STATE, "Cannot call renderer_occludeFromTopRows without first calling renderer_begin!");
+	if (!hasImage(&r->depth))
+		return fail(DFPSR_HIP_ERROR_STATE, "Cannot call renderer_occludeFromTopRows without having given a depth buffer in renderer_begin!");
+	int rc = checkCamera(camera);
+	if (rc) return rc;
+	r->prepareForOcclusion();
+	const int32_t cells = r->gridW * r->gridH;
+	if (cells == 0) return DFPSR_HIP_OK;
+	rc = r->uploadGrid();
+	if (rc) return rc;
+	hipLaunchKernelGGL(k_occlude_top_rows, dim3((cells + 255) / 256), dim3(256), 0, r->frame.stream, r->deviceCanvasDepth(),
+	                   r->depth.width, r->depth.height, r->depth.stride_bytes / 4, r->dGrid.as<float>(), r->gridW, r->gridH,
+	                   camera->perspective ? 1 : 0);
+	HIP_TRY(hipGetLastError());
+	return r->downloadGrid();
+}
+
+extern "C" int dfpsr_hip_renderer_occlude_from_existing_triangles(dfpsr_hip_renderer *r) {
+	if (!r) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_occludeFromExistingTriangles: null renderer");
+	if (!r->receiving)
+		return fail(DFPSR_HIP_ERROR_STATE, "Cannot call renderer_occludeFromExistingTriangles without first calling renderer_begin!");
+	r->prepareForOcclusion();
+	if (r->gridW * r->gridH == 0) return DFPSR_HIP_OK;
+	int rc = r->uploadGrid();
+	if (rc) return rc;
+	rc = r->frame.occludeExisting(r->dGrid.as<float>(), r->gridW, r->gridH);
+	if (rc) return rc;
+	return r->downloadGrid();
+}
+
+extern "C" int dfpsr_hip_renderer_is_box_visible(const dfpsr_hip_renderer *r, const float minimum[3], const float maximum[3],
+                                                 const float transform[12], const dfpsr_hip_camera *camera, int32_t *out) {
+	int rc = checkBoxCall(r, minimum, maximum, transform, camera, "renderer_isBoxVisible");
+	if (rc) return rc;
+	if (!out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_isBoxVisible: null output");
+	*out = boxOccluded(r->gridRead(), r->imageW, r->gridW, r->gridH, make3(minimum[0], minimum[1], minimum[2]),
+	                   make3(maximum[0], maximum[1], maximum[2]), transformFrom(transform), cameraFrom(camera)) ? 0 : 1;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_has_occluders(const dfpsr_hip_renderer *r, int32_t *out) {
+	if (!r || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_hasOccluders: null argument");
+	*out = r->occluded ? 1 : 0;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_renderer_occlusion_grid(const dfpsr_hip_renderer *r, float *out, int32_t *width, int32_t *height) {
+	if (!r || !width || !height) return fail(DFPSR_HIP_ERROR_ARGUMENT, "renderer_occlusion_grid: null argument");
+	*width = r->gridW;
+	*height = r->gridH;
+	if (out)
+		for (int32_t y = 0; y < r->gridH; y++)
+			for (int32_t x = 0; x < r->gridW; x++)
+				out[(size_t)y * r->gridW + x] = r->gridImage.empty() ? 0.0f : r->gridImage[(size_t)y * r->imageW + x];
 	return DFPSR_HIP_OK;
 }
 

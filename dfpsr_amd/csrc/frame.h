@@ -105,6 +105,16 @@ public:
 	}
 	// Adjoint of the last differentiable run: adds into the gradient buffers (either may be null).
 	int backward(float *gradTexture, float *gradMultipliers, const RenderOptions &options);
+	// Occlusion grid (device, gridW x gridH cells) the next run drops covered commands with;
+	// null turns the test off.  reset() turns it off.
+	void setOcclusion(float *deviceGrid, int32_t gridW, int32_t gridH) {
+		occlusionGrid_ = deviceGrid;
+		gridW_ = gridW;
+		gridH_ = gridH;
+	}
+	// renderer_occludeFromExistingTriangles: projects the tasks given so far and lowers the device
+	// grid under the hulls of their Solid draw commands.  Waits for the device.
+	int occludeExisting(float *deviceGrid, int32_t gridW, int32_t gridH);
 
 private:
 	int32_t tileConfig_ = 0;
@@ -147,6 +157,8 @@ private:
 	const float *ftex_ = nullptr;
 	int32_t ftexLog2w_ = 0, ftexLog2h_ = 0;
 	const float *multipliers_ = nullptr;
+	float *occlusionGrid_ = nullptr;
+	int32_t gridW_ = 0, gridH_ = 0;
 	FrameDev lastRun_;
 	bool lastRunValid_ = false;
 	hipGraph_t bwdGraph_ = nullptr;

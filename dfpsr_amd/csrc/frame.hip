@@ -62,6 +62,8 @@ void Frame::reset(int32_t tileConfig) {
 	partTextureOffset_.clear();
 	alphaAny_ = false;
 	orthogonalAny_ = false;
+	occlusionGrid_ = nullptr;
+	gridW_ = gridH_ = 0;
 	oddCanvas_ = false;
 	wideCanvas_ = false;
 }
@@ -325,6 +327,9 @@ int Frame::run(const RenderOptions &o) {
 		f.ftexLog2w = ftexLog2w_;
 		f.ftexLog2h = ftexLog2h_;
 		f.multipliers = multipliers_;
+		f.occlusionGrid = occlusionGrid_;
+		f.gridW = gridW_;
+		f.gridH = gridH_;
 		lastRun_ = f;
 		lastRunValid_ = false;
 		// The raster kernel copies counters[0..7] into pinned host memory as it starts; the
@@ -410,6 +415,50 @@ int Frame::run(const RenderOptions &o) {
 		dropGraph();
 	}
 	return fail(-4, "renderer_end: buffers did not converge");
+}
+
+int Frame::occludeExisting(float *deviceGrid, int32_t gridW, int32_t gridH) {
+	int rc = prepare();
+	if (rc || !prepared_) return rc;
+	const int32_t pointTotal = taskPointEnd_.back();
+	const int32_t fanTotal = taskFanEnd_.back();
+	const int32_t tileTotal = canvasTileEnd_.back();
+	if (fanTotal == 0) return 0;
+	const size_t nC = canvases_.size();
+	if (dProjected_.ensure((size_t)pointTotal * sizeof(Projected)) != hipSuccess || dCounters_.ensure(COUNTERS * 4) != hipSuccess ||
+	    dTileCount_.ensure((2 * (size_t)tileTotal + 1) * 4) != hipSuccess || dCanvasBase_.ensure(nC * 4) != hipSuccess)
+		return fail(-4, "device allocation failed");
+	uint8_t *base = (uint8_t *)dTables_.ptr;
+	FrameDev f;
+	std::memset(&f, 0, sizeof(f));
+	f.tasks = (const TaskDev *)(base + offTasks_);
+	f.taskCount = (int32_t)tasks_.size();
+	f.taskFanEnd = (const int32_t *)(base + offFanEnd_);
+	f.taskPointEnd = (const int32_t *)(base + offPointEnd_);
+	f.fanTotal = fanTotal;
+	f.pointTotal = pointTotal;
+	f.canvases = (const CanvasDev *)(base + offCanvases_);
+	f.canvasCount = (int32_t)nC;
+	f.canvasTileEnd = (const int32_t *)(base + offTileEnd_);
+	f.tileTotal = tileTotal;
+	f.textures = (const TextureDesc *)(base + offTextures_);
+	f.textureCount = (int32_t)textures_.size();
+	f.projected = dProjected_.as<Projected>();
+	f.counters = dCounters_.as<int32_t>();
+	f.tileCount = dTileCount_.as<int32_t>();
+	f.canvasCommandBase = dCanvasBase_.as<int32_t>();
+	f.occlusionGrid = deviceGrid;
+	f.gridW = gridW;
+	f.gridH = gridH;
+	int64_t beginThreads = 2 * (int64_t)tileTotal + 1;
+	if (beginThreads < pointTotal) beginThreads = pointTotal;
+	if (beginThreads < (int64_t)nC) beginThreads = (int64_t)nC;
+	if (beginThreads < COUNTERS) beginThreads = COUNTERS;
+	hipLaunchKernelGGL(k_frame_begin, dim3(blocksOf(beginThreads, 256)), dim3(256), 0, stream, f);
+	hipLaunchKernelGGL(k_occlude_existing, dim3(blocksOf(fanTotal, 256)), dim3(256), 0, stream, f);
+	HIP_TRY(hipGetLastError());
+	HIP_TRY(hipStreamSynchronize(stream));
+	return 0;
 }
 
 int Frame::backward(float *gradTexture, float *gradMultipliers, const RenderOptions &o) {

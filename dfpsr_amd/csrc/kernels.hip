@@ -6,6 +6,7 @@
 // planes ITriangle2D.cpp:182-300, the 2x2 filler shader/fillerTemplates.h:92-385, shading
 // shader/RgbaMultiply.h:74-175.
 #include "pipeline.h"
+#include "occlusion.h"
 
 #include <type_traits>
 
@@ -161,10 +162,40 @@ __device__ __forceinline__ TriangleCorners loadCorners(const FrameDev &f, int32_
 	return tc;
 }
 
+// What setupTriangle does with the draw commands it makes: count them, write and bin them, or
+// lower the occlusion grid under their hulls (renderer_occludeFromExistingTriangles).
+enum { SETUP_COUNT = 0, SETUP_EMIT = 1, SETUP_OCCLUDE = 2 };
+
+// occludeFromSortedHull of one Solid draw command's triangle (rendererAPI.cpp:218-239, with
+// the triangle's wholeBound as the pixel bound, :246-258): every cell whose four corners are
+// inside the triangle takes the larger corner depth if that is nearer.  The depth is +0 or
+// more, whose float bits order as integers, so an integer atomic min keeps the nearest and
+// leaves any negative cell value (sign bit set) in place, as the reference's '<' does.
+__device__ void occludeFromCommand(const FrameDev &f, const Projected *P) {
+	const Rect bound = triangleBound(P[0].flat, P[1].flat, P[2].flat);
+	if (!hullCanOcclude(bound)) return;
+	const float z[3] = {P[0].cs.z, P[1].cs.z, P[2].cs.z};
+	const int bits = __float_as_int(hullDistance(z, 3));
+	const l2 hull[3] = {P[0].flat, P[1].flat, P[2].flat};
+	const Rect cells = outerCellBound(bound, f.gridW, f.gridH);
+	for (int32_t cy = cells.t; cy < cells.bottom(); cy++)
+		for (int32_t cx = cells.l; cx < cells.right(); cx++)
+			if (cellInsideOfHull(hull, 3, cx, cy)) atomicMin(reinterpret_cast<int *>(f.occlusionGrid) + cy * f.gridW + cx, bits);
+}
+
+// Whether a command survives completeOcclusion (rendererAPI.cpp:190-215): always without a
+// grid; otherwise when some cell under its bound is not nearer than its nearest corner.
+__device__ __forceinline__ bool survivesOcclusion(const FrameDev &f, const Projected *P) {
+	if (f.occlusionGrid == nullptr) return true;
+	return commandVisibleInGrid(f.occlusionGrid, f.gridW, f.gridH, triangleBound(P[0].flat, P[1].flat, P[2].flat), P[0].cs.z,
+	                            P[1].cs.z, P[2].cs.z);
+}
+
 // corners: the triangle's corners when the caller has loaded them already.
-template <bool EMIT>
+template <int MODE>
 __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset, BinTally &tally,
                                  BinJob *jobs = nullptr, int32_t jobBase = 0, const TriangleCorners *corners = nullptr) {
+	constexpr bool EMIT = MODE == SETUP_EMIT;
 	const TaskDev &t = f.tasks[ti];
 	const CanvasDev &cv = f.canvases[t.canvas];
 	if (cv.color == nullptr && cv.depth == nullptr) return 0;
@@ -185,6 +216,11 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 	if (EMIT && f.mode == 0) si = triangleShader(t, poly, k);
 	if (triangleVisibility(P[0].cs, P[1].cs, P[2].cs, cam.clip) == VIS_FULL) {
 		if (frontFacing(P[0].flat, P[1].flat, P[2].flat)) {
+			if (MODE == SETUP_OCCLUDE) {
+				if (t.filter == 0) occludeFromCommand(f, P);
+				return 0;
+			}
+			if (!survivesOcclusion(f, P)) return 0;
 			if (EMIT) {
 				Command c;
 				const f2 is[3] = {P[0].is, P[1].is, P[2].is};
@@ -204,6 +240,11 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 			const SubVertex &va = poly9.v[0], &vb = poly9.v[1 + i], &vc = poly9.v[2 + i];
 			Projected S[3] = {cameraToScreen(cam, va.cs), cameraToScreen(cam, vb.cs), cameraToScreen(cam, vc.cs)};
 			if (frontFacing(S[0].flat, S[1].flat, S[2].flat)) {
+				if (MODE == SETUP_OCCLUDE) {
+					if (t.filter == 0) occludeFromCommand(f, S);
+					continue;
+				}
+				if (!survivesOcclusion(f, S)) continue;
 				if (EMIT) {
 					Command c;
 					const f2 is[3] = {S[0].is, S[1].is, S[2].is};
@@ -262,7 +303,7 @@ __device__ int32_t blockExclusiveScan(int32_t value, int32_t &exclusive) {
 __global__ void k_setup_count(FrameDev f) {
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	BinTally unused{0, 0};
-	int32_t n = (g < f.fanTotal) ? setupTriangle<false>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), 0, unused) : 0;
+	int32_t n = (g < f.fanTotal) ? setupTriangle<SETUP_COUNT>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), 0, unused) : 0;
 	if (g < f.fanTotal) f.fanCount[g] = n;
 	int32_t ex;
 	int32_t total = blockExclusiveScan(n, ex);
@@ -383,7 +424,7 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 		if (g == f.tasks[ti].fanBase) atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
 		if (n > 0) {
 			if (offset + n > f.commandCapacity) atomicOr(&f.counters[2], 1);
-			else setupTriangle<true>(f, g, ti, offset, tally, together ? jobs : nullptr, jobBase, &corners);
+			else setupTriangle<SETUP_EMIT>(f, g, ti, offset, tally, together ? jobs : nullptr, jobBase, &corners);
 		}
 	}
 	if (together) {
@@ -1705,6 +1746,45 @@ __global__ void k_frame_begin(FrameDev f) {
 	if (i < 2 * f.tileTotal + 1) f.tileCount[i] = 0;
 	if (i < f.canvasCount) f.canvasCommandBase[i] = 0x7FFFFFFF;
 	if (i < f.pointTotal) projectPoint(f, i);
+}
+
+// renderer_occludeFromExistingTriangles (rendererAPI.cpp:241-258): the Solid draw commands of
+// every task given so far lower the grid under their hulls.  One thread per fan triangle,
+// after k_frame_begin has projected the points.
+__global__ void k_occlude_existing(FrameDev f) {
+	const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= f.fanTotal) return;
+	BinTally unused{0, 0};
+	(void)setupTriangle<SETUP_OCCLUDE>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), 0, unused);
+}
+
+// renderer_occludeFromTopRows (rendererAPI.cpp:403-474), one thread per cell: the first row of
+// the cell's pixel rows, over the columns the reference's scan gives that cell (it starts one
+// column early after the first cell and stops at the canvas width), reduced to the farthest
+// depth -- 1 / the smallest inverse depth for a perspective camera, the largest depth for an
+// orthogonal one -- which replaces the cell's depth if nearer.
+__global__ void k_occlude_top_rows(const float *depth, int32_t width, int32_t height, int32_t stride, float *grid, int32_t gridW,
+                                   int32_t gridH, int32_t perspective) {
+	const int32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+	if (cell >= gridW * gridH) return;
+	const int32_t gx = cell % gridW, gy = cell / gridW;
+	const int32_t y = gy * OCCLUSION_CELL;
+	if (y >= height) return;
+	const int32_t x0 = gx == 0 ? 0 : tmin(gx * OCCLUSION_CELL - 1, width);
+	const int32_t x1 = tmin(gx * OCCLUSION_CELL + OCCLUSION_CELL - 1, width);
+	const float *row = depth + (size_t)y * stride;
+	float farthest;
+	if (perspective) {
+		float smallestInverse = __builtin_huge_valf();
+		for (int32_t x = x0; x < x1; x++)
+			if (row[x] < smallestInverse) smallestInverse = row[x];
+		farthest = 1.0f / smallestInverse;
+	} else {
+		farthest = 0.0f;
+		for (int32_t x = x0; x < x1; x++)
+			if (row[x] > farthest) farthest = row[x];
+	}
+	if (farthest < grid[cell]) grid[cell] = farthest;
 }
 
 __global__ void k_fill(int32_t *p, int32_t n, int32_t value) {

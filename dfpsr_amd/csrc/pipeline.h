@@ -118,6 +118,11 @@ struct FrameDev {
 	const float *multipliers;
 	float *gradTexture;
 	float *gradMultipliers;
+	// Occlusion grid of a renderer frame (one canvas), gridW x gridH cells of 16 x 16 pixels;
+	// null when the frame has no occluders.  The setup drops each command it covers
+	// (completeOcclusion); k_occlude_existing lowers it.
+	float *occlusionGrid;
+	int32_t gridW, gridH;
 };
 
 // First entry of tile t's command list.
@@ -140,6 +145,9 @@ __global__ void k_setup_count(FrameDev f);
 __global__ void k_scan_block_sums(int32_t *blockSums, int32_t blocks, int32_t *total);
 __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_t *totalLow, int32_t *totalHigh);
 __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks);
+__global__ void k_occlude_existing(FrameDev f);
+__global__ void k_occlude_top_rows(const float *depth, int32_t width, int32_t height, int32_t stride, float *grid, int32_t gridW,
+                                   int32_t gridH, int32_t perspective);
 constexpr int32_t SETUP_SUM_BLOCKS = 2048;  // up to this many setup blocks, each sums its offset itself
 __global__ void k_scan_tiles_local(FrameDev f);
 __global__ void k_scan_tiles_add(FrameDev f);

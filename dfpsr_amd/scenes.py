@@ -99,6 +99,11 @@ class Scene:
     # Optional (M, 3) records of the caller's ProjectedPoints (cs, is, flat) per triangle corner:
     # the frame then gives the triangles with renderer_giveTask_triangle instead of the model.
     projected: Optional[np.ndarray] = None
+    # Optional occluder calls of the renderer around the instances, in order: tuples (kind,
+    # instance the call comes before, box minimum, box maximum, box transform) with kind one of
+    # "box" (renderer_occludeFromBox), "top_rows", "existing" (renderer_occludeFrom...) and
+    # "query" (renderer_isBoxVisible); the box fields are unused by top_rows and existing.
+    occluders: Optional[list] = None
 
     @property
     def triangle_count(self) -> int:

@@ -180,6 +180,43 @@ int dfpsr_hip_renderer_give_projected_triangles(dfpsr_hip_renderer *renderer, in
                                                 const float *texcoords, const dfpsr_hip_texture *diffuse,
                                                 const dfpsr_hip_texture *light, int32_t filter,
                                                 const dfpsr_hip_camera *camera);
+/* ---- Occlusion (api/rendererAPI.h:77-93, :131, rendererAPI.cpp:36-477) ----------------------
+ * Between renderer_begin and renderer_end a renderer keeps an occlusion grid of one cell per
+ * 16 x 16 pixels (rounded up), each holding the largest camera-space depth at which something
+ * in the cell can still be seen; it starts at +inf with the frame's first occluder call.
+ * With occluders, renderer_giveTask skips a model whose bounding box the grid hides
+ * (modelAPI.cpp:229-233), and renderer_end drops every draw command whose nearest corner is
+ * not nearer than the grid plus 0.001 in every cell under its bound (completeOcclusion).
+ * All calls fail with DFPSR_HIP_ERROR_STATE outside renderer_begin .. renderer_end. */
+
+/* renderer_occludeFromBox (rendererAPI.h:77, rendererAPI.cpp:270-295): the convex hull of the
+ * box's projected corners lowers the cells it covers whole to the box's farthest corner
+ * depth.  A box with any corner outside the cull frustum (x and y halved) occludes nothing.
+ * The reference's debugSilhouette lines belong to its debug drawing, which is not built. */
+int dfpsr_hip_renderer_occlude_from_box(dfpsr_hip_renderer *renderer, const float minimum[3], const float maximum[3],
+                                        const float transform[12], const dfpsr_hip_camera *camera);
+/* renderer_occludeFromTopRows (rendererAPI.h:83, rendererAPI.cpp:403-474): each cell takes the
+ * farthest depth of the first pixel row of its cell row in the depth canvas (as the canvas is
+ * now: nothing of this frame is drawn before renderer_end), if nearer.  Runs on the GPU over
+ * the device (or staged) canvas.  Fails with DFPSR_HIP_ERROR_STATE without a depth canvas. */
+int dfpsr_hip_renderer_occlude_from_top_rows(dfpsr_hip_renderer *renderer, const dfpsr_hip_camera *camera);
+/* renderer_occludeFromExistingTriangles (rendererAPI.h:131, rendererAPI.cpp:241-258): the draw
+ * commands of every task given so far with the Solid filter lower the cells their triangles
+ * cover whole.  Projects and sets up those tasks on the GPU; waits for it. */
+int dfpsr_hip_renderer_occlude_from_existing_triangles(dfpsr_hip_renderer *renderer);
+/* renderer_isBoxVisible (rendererAPI.h:93, rendererAPI.cpp:298-343, :536-541): *out = 0 when the
+ * box is outside the cull frustum or hidden by the grid, else 1.  As in the reference the
+ * grid outlives the frame: before the frame's first occluder call a query reads the grid an
+ * earlier frame left, or 0 in every cell if this renderer never had one. */
+int dfpsr_hip_renderer_is_box_visible(const dfpsr_hip_renderer *renderer, const float minimum[3], const float maximum[3],
+                                      const float transform[12], const dfpsr_hip_camera *camera, int32_t *out);
+/* renderer_hasOccluders (rendererAPI.h:85): *out = 1 after any occluder call of the frame. */
+int dfpsr_hip_renderer_has_occluders(const dfpsr_hip_renderer *renderer, int32_t *out);
+/* The frame's occlusion cells as they are now (no counterpart in the reference, which keeps
+ * its grid private): *width x *height cells, row by row, into out (NULL: the size only);
+ * 0 everywhere while the renderer never had a grid. */
+int dfpsr_hip_renderer_occlusion_grid(const dfpsr_hip_renderer *renderer, float *out, int32_t *width, int32_t *height);
+
 /* renderer_takesTriangles (api/rendererAPI.h:73). */
 int dfpsr_hip_renderer_takes_triangles(const dfpsr_hip_renderer *renderer, int32_t *out);
 /* renderer_end (api/rendererAPI.h:135, RendererImpl::endFrame rendererAPI.cpp:352-402):

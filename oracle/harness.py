@@ -23,6 +23,10 @@ PROJECTED_DTYPE = np.dtype({"names": ["cs", "is", "flat"], "formats": [(np.float
                             "offsets": [0, 12, 24], "itemsize": 40})
 
 
+# Occluder calls of a scene: (kind, instance it comes before, box minimum, maximum, transform).
+OCCLUDER_KINDS = {"box": 1, "top_rows": 2, "existing": 3, "query": 4}
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("width", C.c_int32), ("height", C.c_int32),
@@ -40,6 +44,7 @@ class SceneDesc(C.Structure):
         ("instance_count", C.c_int32), ("instance_transforms", C.c_void_p),
         ("threaded", C.c_int32),
         ("projected", C.c_void_p),
+        ("occluder_count", C.c_int32), ("occluders", C.c_void_p), ("visible_out", C.c_void_p),
     ]
 
 
@@ -94,6 +99,20 @@ def make_desc(scene, threaded: bool = True) -> Tuple[SceneDesc, list]:
         assert a.dtype == PROJECTED_DTYPE and a.shape == (scene.indices.shape[0], 3)
         keep.append(a)
         d.projected = a.ctypes.data_as(C.c_void_p)
+    ops = getattr(scene, "occluders", None)
+    if ops:
+        table = np.zeros((len(ops), 20), np.float32)
+        for k, op in enumerate(ops):
+            table[k, 0] = OCCLUDER_KINDS[op[0]]
+            table[k, 1] = op[1]
+            if op[0] in ("box", "query"):
+                table[k, 2:5], table[k, 5:8], table[k, 8:20] = op[2], op[3], op[4]
+        keep.append(table)
+        d.occluder_count = len(ops)
+        d.occluders = table.ctypes.data_as(C.c_void_p)
+        visible = np.full(max(1, sum(op[0] == "query" for op in ops)), -1, np.int32)
+        keep.append(visible)
+        d.visible_out = visible.ctypes.data_as(C.c_void_p)
     return d, keep
 
 
@@ -144,6 +163,8 @@ def load_oracle():
     lib.oracle_sample_quads.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p]
     lib.oracle_project.argtypes = [C.POINTER(SceneDesc), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.oracle_occlusion_grid.argtypes = [C.POINTER(SceneDesc), C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.oracle_occlusion_grid.restype = C.c_int32
     lib.oracle_commands.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.oracle_commands.restype = C.c_int32
     return lib
@@ -239,6 +260,49 @@ def ref_render(scene, threaded: bool = True):
     if rc != 0:
         raise RuntimeError("reference renderer failed")
     return color, depth
+
+
+def _visible_out(scene, keep):
+    ops = getattr(scene, "occluders", None)
+    if not ops:
+        return np.zeros(0, np.int32)
+    n = sum(op[0] == "query" for op in ops)
+    return keep[-1][:n].copy()
+
+
+def ref_render_queries(scene):
+    """The compiled reference's frame with the scene's occluder calls: (colour, depth, the
+    answers of its renderer_isBoxVisible calls)."""
+    lib = load_ref()
+    d, keep = make_desc(scene, True)
+    color, depth = _outputs(scene)
+    rc = lib.ref_render_scene(C.byref(d), color.ctypes.data_as(C.c_void_p), depth.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError("reference renderer failed")
+    return color, depth, _visible_out(scene, keep)
+
+
+def oracle_render_queries(scene, threads: int = 1):
+    """The oracle's frame with the scene's occluder calls: (colour, depth, box answers)."""
+    lib = load_oracle()
+    d, keep = make_desc(scene, threads > 1)
+    color, depth = _outputs(scene)
+    rc = lib.oracle_render_scene(C.byref(d), threads, color.ctypes.data_as(C.c_void_p), depth.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"oracle failed with {rc}")
+    return color, depth, _visible_out(scene, keep)
+
+
+def oracle_occlusion_grid(scene):
+    """(grid of ceil(H/16) x ceil(W/16) cells, whether the frame had occluders)."""
+    lib = load_oracle()
+    d, keep = make_desc(scene, True)
+    gw, gh = (scene.width + 15) // 16, (scene.height + 15) // 16
+    grid = np.zeros((gh, gw), np.float32)
+    w, h = C.c_int32(), C.c_int32()
+    occluded = lib.oracle_occlusion_grid(C.byref(d), grid.ctypes.data_as(C.c_void_p), C.byref(w), C.byref(h))
+    assert (w.value, h.value) == (gw, gh)
+    return grid, bool(occluded)
 
 
 def oracle_render(scene, threads: int = 1):

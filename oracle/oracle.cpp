@@ -989,6 +989,10 @@ struct Frame {
 	std::vector<Texture> textures;
 	std::vector<TriangleInput> inputs;
 	std::vector<Command> commands;
+	// Occlusion grid (RendererImpl::depthGrid, rendererAPI.cpp:140-188): 16 x 16 pixel cells.
+	std::vector<float> grid;
+	int32_t gridW = 0, gridH = 0;
+	bool occluded = false;
 };
 
 // executeTriangleDrawing (renderCore.cpp:203-217) with the shader dispatch of fillShape (fillerTemplates.h:387-441).
@@ -1063,6 +1067,216 @@ void giveTriangle(Frame &fr, const Projected &a, const Projected &b, const Proje
 
 inline V3 cornerOf(const float *p4, int comp) { (void)comp; return V3{p4[0], p4[1], p4[2]}; }
 
+// ---------------------------------------------------------------------------
+// Occlusion grid (api/rendererAPI.cpp:36-477).
+// ---------------------------------------------------------------------------
+const int32_t cellSize = 16;  // rendererAPI.cpp:36
+
+// prepareForOcclusion (rendererAPI.cpp:179-188)
+void prepareForOcclusion(Frame &fr) {
+	if (!fr.occluded) fr.grid.assign((size_t)fr.gridW * (size_t)fr.gridH, std::numeric_limits<float>::infinity());
+	fr.occluded = true;
+}
+
+// getOuterCellBound (rendererAPI.cpp:166-177)
+Rect outerCellBound(const Frame &fr, const Rect &pixel) {
+	int32_t minX = pixel.l / cellSize, maxX = pixel.right() / cellSize + 1;
+	int32_t minY = pixel.t / cellSize, maxY = pixel.bottom() / cellSize + 1;
+	minX = std::max(minX, 0); minY = std::max(minY, 0);
+	maxX = std::min(maxX, fr.gridW); maxY = std::min(maxY, fr.gridH);
+	Rect r; r.l = minX; r.t = minY; r.w = maxX - minX; r.h = maxY - minY;
+	return r;
+}
+
+// counterClockwise (rendererAPI.cpp:38-40)
+bool counterClockwise(const Projected &p, const Projected &q, const Projected &r) {
+	return (q.flat.y - p.flat.y) * (r.flat.x - q.flat.x) - (q.flat.x - p.flat.x) * (r.flat.y - q.flat.y) < 0;
+}
+
+// jarvisConvexHullAlgorithm (rendererAPI.cpp:44-72)
+void jarvisHull(const Projected *in, int32_t n, Projected *out, int32_t &outCount) {
+	if (n < 3) {
+		outCount = n;
+		for (int32_t i = 0; i < n; i++) out[i] = in[i];
+		return;
+	}
+	int32_t l = 0;
+	outCount = 0;
+	for (int32_t i = 1; i < n; i++)
+		if (in[i].flat.x < in[l].flat.x) l = i;
+	int32_t p = l;
+	do {
+		if (outCount >= n) return;
+		out[outCount++] = in[p];
+		int32_t q = (p + 1) % n;
+		for (int32_t i = 0; i < n; i++)
+			if (counterClockwise(in[p], in[i], in[q])) q = i;
+		p = q;
+	} while (p != l);
+}
+
+// getPixelBoundFromProjection (rendererAPI.cpp:88-94) with IRect::merge (math/IRect.h:68-74)
+Rect pixelBoundFromProjection(const Projected *pts, int32_t n) {
+	int32_t l = (int32_t)(pts[0].flat.x / 256), t = (int32_t)(pts[0].flat.y / 256), r = l + 1, b = t + 1;
+	for (int32_t i = 1; i < n; i++) {
+		int32_t x = (int32_t)(pts[i].flat.x / 256), y = (int32_t)(pts[i].flat.y / 256);
+		l = std::min(l, x); t = std::min(t, y); r = std::max(r, x + 1); b = std::max(b, y + 1);
+	}
+	return rectFromSides(l, t, r, b);
+}
+
+// pointInsideOfEdge / pointInsideOfHull / rectangleInsideOfHull (rendererAPI.cpp:97-129)
+bool pointInsideOfEdge(const L2 &a, const L2 &b, const L2 &point) {
+	L2 dir{b.y - a.y, a.x - b.x};
+	L2 rel{point.x - a.x, point.y - a.y};
+	return (dir.x * rel.x) + (dir.y * rel.y) <= 0;
+}
+bool pointInsideOfHull(const Projected *hull, int32_t n, const L2 &point) {
+	for (int32_t c = 0; c < n; c++) {
+		int32_t nc = c + 1;
+		if (nc == n) nc = 0;
+		if (!pointInsideOfEdge(hull[c].flat, hull[nc].flat, point)) return false;
+	}
+	return true;
+}
+bool rectangleInsideOfHull(const Projected *hull, int32_t n, const Rect &r) {
+	return pointInsideOfHull(hull, n, L2{r.l, r.t}) && pointInsideOfHull(hull, n, L2{r.right(), r.t}) &&
+	       pointInsideOfHull(hull, n, L2{r.l, r.bottom()}) && pointInsideOfHull(hull, n, L2{r.right(), r.bottom()});
+}
+
+// occludeFromSortedHull (rendererAPI.cpp:218-239)
+void occludeFromSortedHull(Frame &fr, const Projected *hull, int32_t n, const Rect &pixelBound) {
+	if (!(pixelBound.w > cellSize && pixelBound.h > cellSize)) return;
+	float distance = 0.0f;
+	for (int32_t c = 0; c < n; c++)
+		if (hull[c].cs.z > distance) distance = hull[c].cs.z;
+	Rect cells = outerCellBound(fr, pixelBound);
+	for (int32_t cy = cells.t; cy < cells.bottom(); cy++) {
+		for (int32_t cx = cells.l; cx < cells.right(); cx++) {
+			Rect sub;
+			sub.l = cx * cellSize * 256; sub.t = cy * cellSize * 256; sub.w = cellSize * 256; sub.h = cellSize * 256;
+			if (rectangleInsideOfHull(hull, n, sub)) {
+				float &cell = fr.grid[(size_t)cy * (size_t)fr.gridW + (size_t)cx];
+				if (distance < cell) cell = distance;
+			}
+		}
+	}
+}
+
+// The corners of a box (GENERATE_BOX_CORNERS, rendererAPI.cpp:259-267)
+void boxCornersOf(const V3 &mn, const V3 &mx, V3 *out) {
+	out[0] = V3{mn.x, mn.y, mn.z}; out[1] = V3{mn.x, mn.y, mx.z}; out[2] = V3{mn.x, mx.y, mn.z}; out[3] = V3{mn.x, mx.y, mx.z};
+	out[4] = V3{mx.x, mn.y, mn.z}; out[5] = V3{mx.x, mn.y, mx.z}; out[6] = V3{mx.x, mx.y, mn.z}; out[7] = V3{mx.x, mx.y, mx.z};
+}
+
+// occludeFromBox with projectHull (rendererAPI.cpp:76-90, :270-295)
+void occludeFromBox(Frame &fr, const V3 &mn, const V3 &mx, const Transform &modelToWorld) {
+	prepareForOcclusion(fr);
+	V3 local[8];
+	boxCornersOf(mn, mx, local);
+	Projected proj[8], hull[8];
+	for (int32_t p = 0; p < 8; p++) {
+		V3 cameraPoint = transformPointTransposedInverse(fr.camera.location, transformPoint(modelToWorld, local[p]));
+		V3 narrow{cameraPoint.x * 0.5f, cameraPoint.y * 0.5f, cameraPoint.z * 1.0f};
+		for (int32_t s = 0; s < fr.camera.cull.count; s++)
+			if (!insidePlane(fr.camera.cull.planes[s], narrow)) return;
+		proj[p] = cameraToScreen(fr.camera, cameraPoint);
+	}
+	int32_t count = 0;
+	jarvisHull(proj, 8, hull, count);
+	occludeFromSortedHull(fr, hull, count, pixelBoundFromProjection(hull, count));
+}
+
+// isBoxOccluded / isHullOccluded (rendererAPI.cpp:298-343).  Before the first occluder call a
+// fresh renderer has no grid image, which image_readPixel_clamp reads as 0 (imageAPI.h:276-282).
+bool isBoxOccluded(const Frame &fr, const V3 &mn, const V3 &mx, const Transform &modelToWorld) {
+	V3 local[8], cam[8];
+	Projected proj[8];
+	boxCornersOf(mn, mx, local);
+	for (int32_t p = 0; p < 8; p++) {
+		cam[p] = transformPointTransposedInverse(fr.camera.location, transformPoint(modelToWorld, local[p]));
+		proj[p] = cameraToScreen(fr.camera, cam[p]);
+	}
+	for (int32_t s = 0; s < fr.camera.cull.count; s++) {
+		bool allOutside = true;
+		for (int32_t p = 0; p < 8; p++)
+			if (insidePlane(fr.camera.cull.planes[s], cam[p])) { allOutside = false; break; }
+		if (allOutside) return true;
+	}
+	Rect pixelBound = pixelBoundFromProjection(proj, 8);
+	float closest = std::numeric_limits<float>::infinity();
+	for (int32_t c = 0; c < 8; c++)
+		if (proj[c].cs.z < closest) closest = proj[c].cs.z;
+	Rect cells = outerCellBound(fr, pixelBound);
+	for (int32_t cy = cells.t; cy < cells.bottom(); cy++)
+		for (int32_t cx = cells.l; cx < cells.right(); cx++) {
+			float cell = fr.occluded ? fr.grid[(size_t)cy * (size_t)fr.gridW + (size_t)cx] : 0.0f;
+			if (closest < cell) return false;
+		}
+	return true;
+}
+
+// occludeFromTopRows (rendererAPI.cpp:403-474): one scan along the first row of each cell row,
+// where each cell after the first starts one column early (right starts at cellSize - 1).
+void occludeFromTopRows(Frame &fr) {
+	prepareForOcclusion(fr);
+	const int32_t width = fr.canvas.width, height = fr.canvas.height;
+	for (int32_t y = 0, gy = 0; y < height; y += cellSize, gy++) {
+		const float *depthRow = fr.canvas.depth + (size_t)y * (size_t)width;
+		int32_t x = 0, right = cellSize - 1;
+		for (int32_t gx = 0; gx < fr.gridW; gx++) {
+			if (right >= width) right = width;
+			float farthest;
+			if (fr.camera.perspective) {
+				float smallestInverse = std::numeric_limits<float>::infinity();
+				while (x < right) {
+					if (depthRow[x] < smallestInverse) smallestInverse = depthRow[x];
+					x++;
+				}
+				farthest = 1.0f / smallestInverse;
+			} else {
+				farthest = 0.0f;
+				while (x < right) {
+					if (depthRow[x] > farthest) farthest = depthRow[x];
+					x++;
+				}
+			}
+			float &cell = fr.grid[(size_t)gy * (size_t)fr.gridW + (size_t)gx];
+			if (farthest < cell) cell = farthest;
+			right += cellSize;
+		}
+	}
+}
+
+// occludeFromExistingTriangles (rendererAPI.cpp:241-258): the Solid commands so far.
+void occludeFromExistingTriangles(Frame &fr) {
+	prepareForOcclusion(fr);
+	if (fr.filter != 0) return;  // one model per scene: every command has the scene's filter
+	for (const Command &c : fr.commands) occludeFromSortedHull(fr, c.tri.p, 3, c.tri.wholeBound);
+}
+
+// completeOcclusion (rendererAPI.cpp:190-215): commands whose nearest corner is not nearer than
+// 0.001 past any cell under their bound are removed before execution.
+void completeOcclusion(Frame &fr) {
+	if (!fr.occluded) return;
+	std::vector<Command> kept;
+	for (const Command &c : fr.commands) {
+		bool anyVisible = false;
+		Rect cells = outerCellBound(fr, c.tri.wholeBound);
+		for (int32_t cy = cells.t; cy < cells.bottom(); cy++) {
+			for (int32_t cx = cells.l; cx < cells.right(); cx++) {
+				float background = fr.grid[(size_t)cy * (size_t)fr.gridW + (size_t)cx];
+				float nearest = c.tri.p[0].cs.z;
+				if (c.tri.p[1].cs.z < nearest) nearest = c.tri.p[1].cs.z;
+				if (c.tri.p[2].cs.z < nearest) nearest = c.tri.p[2].cs.z;
+				if (nearest < background + 0.001) anyVisible = true;
+			}
+		}
+		if (anyVisible) kept.push_back(c);
+	}
+	fr.commands.swap(kept);
+}
+
 // model_render_threaded (modelAPI.cpp:214-281): box cull, project every point once,
 // then fan each polygon into (0,1,2) and (0,2,3).
 void giveModel(Frame &fr, const scene_desc *s, const Transform &modelToWorld, int32_t instance, int32_t &sourceCounter) {
@@ -1083,7 +1297,7 @@ void giveModel(Frame &fr, const scene_desc *s, const Transform &modelToWorld, in
 		for (int32_t poly = s->part_polygon_start[part]; poly < s->part_polygon_start[part + 1]; poly++)
 			polygonTriangles += (s->indices[poly * 4 + 3] >= 0) ? 2 : 1;
 	(void)instance;
-	if (boxSeen(fr.camera, mn, mx, modelToWorld) == 0) {
+	if (boxSeen(fr.camera, mn, mx, modelToWorld) == 0 || (fr.occluded && isBoxOccluded(fr, mn, mx, modelToWorld))) {
 		sourceCounter += polygonTriangles;
 		return;
 	}
@@ -1175,15 +1389,45 @@ void giveProjectedTriangles(Frame &fr, const scene_desc *s) {
 	}
 }
 
+// The scene's occluder calls that come before instance `before` (before == instance_count: after
+// the last one), in order.
+void applyOccluders(const scene_desc *s, Frame &fr, int32_t before, int32_t &query) {
+	for (int32_t k = 0; k < s->occluder_count; k++) {
+		const float *op = s->occluders + (size_t)k * 20;
+		if ((int32_t)op[1] != before) continue;
+		V3 mn{op[2], op[3], op[4]}, mx{op[5], op[6], op[7]};
+		Transform t = transformFrom(op + 8);
+		switch ((int32_t)op[0]) {
+		case 1: occludeFromBox(fr, mn, mx, t); break;
+		case 2: occludeFromTopRows(fr); break;
+		case 3: occludeFromExistingTriangles(fr); break;
+		case 4:
+			if (s->visible_out) s->visible_out[query] = isBoxOccluded(fr, mn, mx, t) ? 0 : 1;
+			query++;
+			break;
+		default: break;
+		}
+	}
+}
+
 void giveAll(const scene_desc *s, Frame &fr) {
 	fr.commands.clear();
 	fr.inputs.clear();
+	fr.grid.clear();
+	fr.occluded = false;
+	fr.gridW = (fr.canvas.width + (cellSize - 1)) / cellSize;
+	fr.gridH = (fr.canvas.height + (cellSize - 1)) / cellSize;
 	if (s->projected) {
 		giveProjectedTriangles(fr, s);
 		return;
 	}
-	int32_t source = 0;
-	for (int32_t i = 0; i < s->instance_count; i++) giveModel(fr, s, transformFrom(s->instance_transforms + i * 12), i, source);
+	int32_t source = 0, query = 0;
+	for (int32_t i = 0; i < s->instance_count; i++) {
+		applyOccluders(s, fr, i, query);
+		giveModel(fr, s, transformFrom(s->instance_transforms + i * 12), i, source);
+	}
+	applyOccluders(s, fr, s->instance_count, query);
+	completeOcclusion(fr);
 }
 
 // CommandQueue::execute (renderCore.cpp:449-480): 12 slices, boundaries on even rows except
@@ -1466,11 +1710,26 @@ void oracle_project(const scene_desc *s, const float *transform, int32_t n, cons
 	}
 }
 
+int32_t oracle_occlusion_grid(const scene_desc *s, float *outGrid, int32_t *outWidth, int32_t *outHeight) {
+	Frame fr;
+	std::vector<uint32_t> color;
+	std::vector<float> depth;
+	buildFrame(s, fr, color, depth);
+	loadCanvas(s, fr);
+	giveAll(s, fr);
+	*outWidth = fr.gridW;
+	*outHeight = fr.gridH;
+	size_t n = (size_t)fr.gridW * (size_t)fr.gridH;
+	for (size_t i = 0; i < n; i++) outGrid[i] = fr.occluded ? fr.grid[i] : 0.0f;
+	return fr.occluded ? 1 : 0;
+}
+
 int32_t oracle_commands(const scene_desc *s, int32_t capacity, int32_t *outTriangle, int64_t *outFlat, int32_t *outBound) {
 	Frame fr;
 	std::vector<uint32_t> color;
 	std::vector<float> depth;
 	buildFrame(s, fr, color, depth);
+	loadCanvas(s, fr);
 	giveAll(s, fr);
 	int32_t n = (int32_t)fr.commands.size();
 	if (n > capacity) return n;

@@ -60,6 +60,10 @@ int32_t oracle_sample_quads(int32_t w, int32_t h, int32_t res, const uint32_t *l
 void oracle_project(const scene_desc *scene, const float *transform, int32_t n, const float *points,
                     float *out_cs, float *out_is, int64_t *out_flat);
 
+/* The occlusion grid a frame ends with (renderer_end's completeOcclusion reads it):
+ * ceil(W/16) x ceil(H/16) cells into out_grid; returns 1 if the frame had occluder calls. */
+int32_t oracle_occlusion_grid(const scene_desc *scene, float *out_grid, int32_t *out_width, int32_t *out_height);
+
 /* Draw-command list of a frame, as renderer_end would execute it: per command the
  * source triangle (instance-major, then polygon fan order), the three projected
  * corners (flat sub-pixel x,y) and the pixel bound.  Returns the command count

@@ -137,8 +137,42 @@ static void giveProjectedTriangles(const scene_desc *s, BuiltScene &b, Renderer 
 	}
 }
 
+// The scene's occluder calls that come before instance `before`, through the renderer API.
+static void applyOccluders(const scene_desc *s, BuiltScene &b, Renderer &renderer, int32_t before, int32_t &query) {
+	for (int32_t k = 0; k < s->occluder_count; k++) {
+		const float *op = s->occluders + (size_t)k * 20;
+		if ((int32_t)op[1] != before) continue;
+		FVector3D mn(op[2], op[3], op[4]), mx(op[5], op[6], op[7]);
+		Transform3D t = makeTransform(op + 8);
+		switch ((int32_t)op[0]) {
+		case 1: renderer_occludeFromBox(renderer, mn, mx, t, b.camera); break;
+		case 2: renderer_occludeFromTopRows(renderer, b.camera); break;
+		case 3: renderer_occludeFromExistingTriangles(renderer); break;
+		case 4: {
+			bool visible = renderer_isBoxVisible(renderer, mn, mx, t, b.camera);
+			if (s->visible_out) s->visible_out[query] = visible ? 1 : 0;
+			query++;
+			break;
+		}
+		default: break;
+		}
+	}
+}
+
 static void renderFrame(const scene_desc *s, BuiltScene &b) {
-	if (s->projected) {
+	if (s->occluder_count > 0) {
+		Renderer renderer = renderer_create();
+		ImageRgbaU8 color = b.color;
+		ImageF32 depth = b.depth;
+		renderer_begin(renderer, color, depth);
+		int32_t query = 0;
+		for (int32_t i = 0; i < s->instance_count; i++) {
+			applyOccluders(s, b, renderer, i, query);
+			renderer_giveTask(renderer, b.model, makeTransform(s->instance_transforms + i * 12), b.camera);
+		}
+		applyOccluders(s, b, renderer, s->instance_count, query);
+		renderer_end(renderer);
+	} else if (s->projected) {
 		Renderer renderer = renderer_create();
 		ImageRgbaU8 color = b.color;
 		ImageF32 depth = b.depth;

@@ -60,6 +60,15 @@ typedef struct {
 	 * is xy as float, flat xy as int64).  Colours, texture coordinates and each part's textures
 	 * come from the model's arrays; points and instances are not used.  NULL: the model path. */
 	const void *projected;
+	/* Optional occlusion calls of the renderer (rendererAPI.h:77-93, :131), in order, each as 20
+	 * floats: [0] kind (1 renderer_occludeFromBox, 2 renderer_occludeFromTopRows,
+	 * 3 renderer_occludeFromExistingTriangles, 4 renderer_isBoxVisible), [1] the instance the
+	 * call comes before (instance_count: after the last), [2..4] box minimum, [5..7] box
+	 * maximum, [8..19] box transform (kinds 1 and 4).  The answers of the kind-4 calls go to
+	 * visible_out, in order (1 = visible).  Scenes with occluder calls use the renderer. */
+	int32_t occluder_count;
+	const float *occluders;
+	int32_t *visible_out;
 } scene_desc;
 
 #ifdef __cplusplus

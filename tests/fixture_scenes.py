@@ -267,6 +267,85 @@ def projected_scenes():
     return out
 
 
+def _translated(x, y, z):
+    t = S.identity_transform().copy()
+    t[0:3] = (x, y, z)
+    return t
+
+
+def occlusion_scenes():
+    """Scenes with the renderer's occluder calls (rendererAPI.h:77-93, :131) around several
+    instances of one model: a box occluder in front of a row of instances, the top rows of a
+    preloaded depth canvas (perspective and orthogonal), the triangles already given, and a
+    mix with box queries before and after each call.  Canvas sizes are not multiples of the
+    16-pixel cell."""
+    out = []
+    ident = S.identity_transform()
+    # A wall box in front of a grid of small soups; queries left, centre and behind.
+    sc = S.cube_soup(width=150, height=100, triangles=260, seed=61, spread=0.12, name="occ_box")
+    sc.cam_location = S.look_transform((0.0, 0.0, -7.0), 0.0, 0.0)
+    sc.instances = np.stack([_translated(x, y, z) for z in (2.0, 0.0) for y in (-1.2, 1.2) for x in (-3.0, 0.0, 3.0)])
+    wall = ((-2.2, -1.6, -3.0), (2.2, 1.6, -2.6), ident)
+    sc.occluders = [("query", 0, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), _translated(0.0, 0.0, 2.0)),
+                    ("box", 0) + wall,
+                    ("query", 0, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), _translated(0.0, 0.0, 2.0)),
+                    ("query", 3, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), _translated(-4.5, 0.0, 1.0)),
+                    ("query", 6, (-0.2, -0.2, -0.2), (0.2, 0.2, 0.2), _translated(0.5, 0.3, 4.0))]
+    out.append(sc)
+    # Top rows of a preloaded ground: inverse depth falling toward the top, zero (sky) above row 30.
+    sc = S.cube_soup(width=170, height=90, triangles=300, seed=62, spread=0.15, name="occ_top_rows")
+    sc.cam_location = S.look_transform((0.0, 0.0, -6.0), 0.0, 0.0)
+    sc.instances = np.stack([_translated(x, y, z) for z in (-2.0, 1.0, 4.0) for x in (-2.5, 0.0, 2.5) for y in (-0.8,)])
+    rows = np.arange(sc.height, dtype=np.float32)
+    ground = np.where(rows[:, None] >= 30, (rows[:, None] - 20.0) / 300.0, 0.0) + np.zeros((1, sc.width))
+    ground[:, ::7] *= 1.05
+    sc.initial_depth = ground.astype(np.float32)
+    sc.occluders = [("top_rows", 0),
+                    ("query", 0, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), _translated(0.0, -1.0, 6.0)),
+                    ("query", 0, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), _translated(0.0, -1.0, -3.5))]
+    out.append(sc)
+    # Orthogonal: the top rows keep the largest linear depth.
+    sc = S.cube_soup(width=130, height=97, triangles=300, seed=63, spread=0.15, name="occ_top_rows_orthogonal")
+    sc.perspective = False
+    sc.width_slope = 4.0
+    sc.cam_location = S.look_transform((0.0, 0.0, -6.0), 0.0, 0.0)
+    sc.instances = np.stack([_translated(x, y, z) for z in (-3.0, 2.0) for x in (-2.0, 1.5) for y in (-1.0, 1.0)])
+    depth = np.full((sc.height, sc.width), 1.0e6, np.float32)
+    depth[40:, :] = 5.0 + np.arange(sc.width, dtype=np.float32)[None, :] * 0.01
+    sc.initial_depth = depth
+    sc.occluders = [("top_rows", 0), ("query", 4, (-0.3, -0.3, -0.3), (0.3, 0.3, 0.3), _translated(0.0, -1.0, 3.0))]
+    out.append(sc)
+    # The first instance's large triangles, close to the camera, hide later instances behind.
+    sc = S.cube_soup(width=160, height=120, triangles=40, seed=70, spread=1.2, name="occ_existing")
+    sc.cam_location = S.look_transform((0.0, 0.0, -4.0), 0.0, 0.0)
+    sc.instances = np.stack([_translated(0.0, 0.0, -2.8), _translated(0.3, 0.2, 3.0), _translated(-0.5, 0.0, 5.0),
+                             _translated(4.0, 0.0, 6.0)])
+    sc.occluders = [("existing", 1), ("query", 1, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), _translated(0.0, 0.0, 4.0)),
+                    ("existing", 3)]
+    out.append(sc)
+    # Everything together, textured, with queries between the calls.
+    n = 200
+    sc = S.cube_soup(width=144, height=108, triangles=n, seed=65, spread=0.5, name="occ_mixed")
+    sc.cam_location = S.look_transform((0.3, 0.5, -6.0), 0.05, 0.08)
+    sc.textures = [S.noise_checker_texture(64, 66, 7, check=8)]
+    sc.parts = [S.Part(0, n, diffuse=0)]
+    sc.texcoords[:, :3, :2] = S.Rng(67).uniform(n * 6, -1, 2).reshape(n, 3, 2)
+    sc.instances = np.stack([_translated(0.0, 0.0, -2.0), _translated(-2.0, 0.5, 2.0), _translated(1.5, -0.5, 3.0),
+                             _translated(0.0, 0.0, 6.0), _translated(3.5, 1.5, 1.0)])
+    ground = np.zeros((sc.height, sc.width), np.float32)
+    ground[60:, :] = 0.125
+    sc.initial_depth = ground
+    sc.occluders = [("box", 0, (-1.0, -2.0, -0.5), (1.0, 0.0, -0.2), _translated(0.0, 0.5, 0.0)),
+                    ("query", 1, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), _translated(0.0, 0.0, 6.0)),
+                    ("existing", 1),
+                    ("query", 2, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), _translated(0.0, 0.0, 6.0)),
+                    ("top_rows", 3),
+                    ("query", 5, (-0.2, -0.2, -0.2), (0.2, 0.2, 0.2), _translated(0.0, -1.5, 8.0)),
+                    ("query", 5, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), _translated(3.5, 1.5, 1.0))]
+    out.append(sc)
+    return out
+
+
 def id_color(scene: S.Scene) -> S.Scene:
     """Same geometry with one constant colour per polygon and no textures: selects the
     "Single color" shader (RgbaMultiply.h:170-171), so the colour buffer is a triangle-id map
@@ -292,6 +371,7 @@ def fixture_scenes():
     out += [id_color(s) for s in clip_scenes()]
     out += shader_branch_scenes()
     out += projected_scenes()
+    out += occlusion_scenes()
     return out
 
 

@@ -8,7 +8,8 @@ built by `make -f oracle/Makefile.ref`):
 Fixtures written (all data, no reference source):
   known_answers.json    known-answer vectors taken from the reference's own unit tests
                         (test/tests/TextureTest.cpp, test/tests/SimdTest.cpp), extracted as data;
-  scenes.npz            colour + depth of every scene in tests/fixture_scenes.py;
+  scenes.npz            colour + depth of every scene in tests/fixture_scenes.py, and the answers of
+                        the renderer_isBoxVisible calls of scenes with occluder calls;
   config0.npz           colour + depth of the 640x480 cube-soup frame;
   config1.json          SHA-256 of the 1920x1080 terrain frame (colour, depth, id-colour) + crops;
   config1_crops.npz     64x64 crops of that frame;
@@ -156,8 +157,12 @@ def main():
     data = {}
     names = []
     for sc in fixture_scenes.fixture_scenes():
-        c, d = harness.ref_render(sc, True)
         names.append(sc.name)
+        if sc.occluders:
+            c, d, visible = harness.ref_render_queries(sc)
+            data[sc.name + "/visible"] = visible
+        else:
+            c, d = harness.ref_render(sc, True)
         data[sc.name + "/color"] = c
         data[sc.name + "/depth"] = d
     np.savez_compressed(os.path.join(HERE, "scenes.npz"), names=np.array(names), vendor=np.array(vendor), **data)

@@ -370,3 +370,66 @@ def test_points_replaced_by_fewer_are_refused_by_every_draw(api):
     with pytest.raises(api.DfpsrError) as e:
         diff.forward(views.model, views.views, tex)
     assert e.value.code == -1
+
+
+@pytest.mark.parametrize("scene", fixture_scenes.occlusion_scenes(), ids=lambda s: s.name)
+def test_occlusion_queries_commands_and_grid(api, scene):
+    """The renderer's occluder calls: box-query answers, the draw commands left after
+    completeOcclusion and the final grid equal the oracle's, and the canvases meet the bar."""
+    print(f"[gpu] occlusion {scene.name}", flush=True)
+    r = api.renderer_create()
+    g = api.GpuScene(scene)
+    color = np.zeros((scene.height, scene.width), np.uint32)
+    depth = scene.initial_depth.copy() if scene.initial_depth is not None else np.zeros((scene.height, scene.width), np.float32)
+    api.renderer_begin(r, api.host_image(color), api.host_image(depth))
+    assert not api.renderer_hasOccluders(r)
+    g.give(r)
+    assert api.renderer_hasOccluders(r)
+    api.renderer_end(r)
+    oc, od, visible = H.oracle_render_queries(scene)
+    assert [int(v) for v in g.visible] == visible.tolist()
+    assert int(r.stats()[0]) == len(H.oracle_commands(scene)[0])
+    grid, _ = H.oracle_occlusion_grid(scene)
+    assert np.array_equal(api.renderer_occlusionGrid(r).view(np.uint32), grid.view(np.uint32))
+    assert_parity(scene, (color, depth), (oc, od))
+
+
+def test_occlusion_grid_outlives_the_frame(api):
+    """A box query before the frame's first occluder call reads the grid the previous frame
+    left (the reference keeps its grid image); a renderer that never had a grid reads 0."""
+    sc = fixture_scenes.occlusion_scenes()[0]
+    g = api.GpuScene(sc)
+    box = ((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    r = api.renderer_create()
+    color = np.zeros((sc.height, sc.width), np.uint32)
+    api.renderer_begin(r, api.host_image(color), api.host_image(None))
+    answers_fresh = [api.renderer_isBoxVisible(r, *box, _translate(x), g.camera) for x in (-6.0, 0.0, 6.0)]
+    g.give(r)
+    answers_end = [api.renderer_isBoxVisible(r, *box, _translate(x), g.camera) for x in (-6.0, 0.0, 6.0)]
+    api.renderer_end(r)
+    api.renderer_begin(r, api.host_image(color), api.host_image(None))
+    answers_next = [api.renderer_isBoxVisible(r, *box, _translate(x), g.camera) for x in (-6.0, 0.0, 6.0)]
+    assert not api.renderer_hasOccluders(r)
+    api.renderer_end(r)
+    assert answers_fresh == [False, False, False]
+    assert answers_next == answers_end and True in answers_end and False in answers_end
+
+
+def _translate(x):
+    t = S.identity_transform().copy()
+    t[0], t[2] = x, 2.0
+    return t
+
+
+def test_occlusion_calls_need_begin_and_a_depth_canvas(api):
+    r = api.renderer_create()
+    cam = api.camera_create(True, S.identity_transform(), 32, 32, 1.0)
+    with pytest.raises(api.DfpsrError) as e:
+        api.renderer_occludeFromExistingTriangles(r)
+    assert e.value.code == -2
+    color = np.zeros((32, 32), np.uint32)
+    api.renderer_begin(r, api.host_image(color), api.host_image(None))
+    with pytest.raises(api.DfpsrError) as e:
+        api.renderer_occludeFromTopRows(r, cam)
+    assert e.value.code == -2 and "depth buffer" in str(e.value)
+    api.renderer_end(r)

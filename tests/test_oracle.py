@@ -222,7 +222,11 @@ def compare_color(got, want, same_vendor):
 
 @pytest.mark.parametrize("scene", fixture_scenes.fixture_scenes(), ids=lambda s: s.name)
 def test_scene_golden(scene, scene_goldens):
-    color, depth = H.oracle_render(scene, threads=1)
+    if scene.occluders:
+        color, depth, visible = H.oracle_render_queries(scene)
+        assert visible.tolist() == scene_goldens[scene.name + "/visible"].tolist()
+    else:
+        color, depth = H.oracle_render(scene, threads=1)
     want_c = scene_goldens[scene.name + "/color"]
     want_d = scene_goldens[scene.name + "/depth"]
     assert np.array_equal(depth.view(np.uint32), want_d.view(np.uint32))
@@ -397,3 +401,38 @@ def test_projected_scenes_move_coverage():
         a = H.oracle_render(sc)
         b = H.oracle_render(exact)
         assert not np.array_equal(a[0], b[0]), sc.name
+
+
+def test_occlusion_scenes_exercise_the_grid():
+    """Each occluder fixture drops draw commands (completeOcclusion) or whole models, the box
+    wall changes the image, and the box queries answer both ways over the set."""
+    import copy
+    answers = []
+    for sc in fixture_scenes.occlusion_scenes():
+        plain = copy.copy(sc)
+        plain.occluders = None
+        with_occluders = len(H.oracle_commands(sc)[0])
+        without = len(H.oracle_commands(plain)[0])
+        grid, occluded = H.oracle_occlusion_grid(sc)
+        assert occluded and np.isfinite(grid).any(), sc.name
+        assert with_occluders <= without, sc.name
+        answers += H.oracle_render_queries(sc)[2].tolist()
+        if sc.name in ("occ_box", "occ_existing", "occ_top_rows_orthogonal"):
+            assert with_occluders < without * 0.8, sc.name
+    assert 0 in answers and 1 in answers
+
+
+def test_occlusion_grid_cells_follow_the_rules():
+    """Mathematical checks of the grid the occluders leave: a box occluder's cells hold its
+    farthest corner depth or +inf; top-row cells of a constant inverse depth d hold exactly
+    1/d; existing triangles never write below the nearest queued corner."""
+    import copy
+    sc = fixture_scenes.occlusion_scenes()[0]
+    grid, _ = H.oracle_occlusion_grid(sc)
+    values = set(grid[np.isfinite(grid)].tolist())
+    assert len(values) == 1
+    flat = copy.copy(fixture_scenes.occlusion_scenes()[1])
+    flat.initial_depth = np.full((flat.height, flat.width), 0.25, np.float32)
+    flat.occluders = [("top_rows", 0)]
+    grid, _ = H.oracle_occlusion_grid(flat)
+    assert np.all(grid == np.float32(4.0))
