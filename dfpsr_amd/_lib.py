@@ -71,6 +71,11 @@ _SIGNATURES = {
     "dfpsr_hip_model_set_points": ([_P, _I, _P], C.c_int),
     "dfpsr_hip_model_add_part": ([_P, _I, _P, _P, _P, _P, _P], C.c_int),
     "dfpsr_hip_model_bound": ([_P, _P, _P], C.c_int),
+    "dfpsr_hip_model_import_dmf1": ([C.POINTER(_P), C.c_char_p, C.c_int64, _I, _I, _P, _P], C.c_int),
+    "dfpsr_hip_model_get_points": ([_P, _P, _P], C.c_int),
+    "dfpsr_hip_model_part_count": ([_P, _P], C.c_int),
+    "dfpsr_hip_model_get_part": ([_P, _I, _P, _P, _P, _P, _P, _P], C.c_int),
+    "dfpsr_hip_model_part_name": ([_P, _I, _P, _I, _P], C.c_int),
     "dfpsr_hip_model_render_depth": ([_P, _P, _P, _P], C.c_int),
     "dfpsr_hip_model_render_depth_noalloc": ([_P, _P, _P, _P], C.c_int),
     "dfpsr_hip_model_destroy": ([_P], None),

@@ -97,6 +97,56 @@ def model_addPart(model: Model, indices, colors, texcoords, diffuse: Optional[Te
     model._keep += [t for t in (diffuse, light) if t is not None]
 
 
+def model_importDMF1(content: bytes, detail_level: int = 2, pool: Optional[dict] = None) -> Model:
+    """importFromContent_DMF1 (modelAPI.h:319): `pool` maps texture names to Textures (the
+    caller's resource pool; names compared ignoring ASCII case)."""
+    pool = pool or {}
+    names = (C.c_char_p * max(1, len(pool)))(*[n.encode() for n in pool])
+    handles = (C.c_void_p * max(1, len(pool)))(*[t.handle.value if hasattr(t.handle, "value") else t.handle for t in pool.values()])
+    h = C.c_void_p()
+    check(lib().dfpsr_hip_model_import_dmf1(C.byref(h), content, len(content), detail_level, len(pool),
+                                            C.cast(names, C.c_void_p), C.cast(handles, C.c_void_p)))
+    m = Model(h)
+    m._keep += list(pool.values())
+    return m
+
+
+def model_getPoints(model: Model) -> np.ndarray:
+    n = C.c_int32()
+    check(lib().dfpsr_hip_model_get_points(model.handle, C.byref(n), None))
+    out = np.zeros((n.value, 3), np.float32)
+    check(lib().dfpsr_hip_model_get_points(model.handle, C.byref(n), out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def model_getNumberOfParts(model: Model) -> int:
+    n = C.c_int32()
+    check(lib().dfpsr_hip_model_part_count(model.handle, C.byref(n)))
+    return n.value
+
+
+def model_getPart(model: Model, part: int) -> dict:
+    """One part: indices (N, 4), colors (N, 4, 4), texcoords (N, 4, 4) and the raw handles of
+    its diffuse and light textures (None when absent)."""
+    n = C.c_int32()
+    check(lib().dfpsr_hip_model_get_part(model.handle, part, C.byref(n), None, None, None, None, None))
+    idx = np.zeros((n.value, 4), np.int32)
+    col = np.zeros((n.value, 4, 4), np.float32)
+    tex = np.zeros((n.value, 4, 4), np.float32)
+    d, l = C.c_void_p(), C.c_void_p()
+    check(lib().dfpsr_hip_model_get_part(model.handle, part, C.byref(n), idx.ctypes.data_as(C.c_void_p),
+                                         col.ctypes.data_as(C.c_void_p), tex.ctypes.data_as(C.c_void_p), C.byref(d), C.byref(l)))
+    return {"indices": idx, "colors": col, "texcoords": tex, "diffuse": d.value, "light": l.value}
+
+
+def model_getPartName(model: Model, part: int) -> str:
+    n = C.c_int32()
+    check(lib().dfpsr_hip_model_part_name(model.handle, part, None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(lib().dfpsr_hip_model_part_name(model.handle, part, buf, n.value + 1, C.byref(n)))
+    return buf.value.decode("utf-8", "replace")
+
+
 def model_getBoundingBox(model: Model):
     mn = np.zeros(3, np.float32)
     mx = np.zeros(3, np.float32)

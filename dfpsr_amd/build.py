@@ -16,10 +16,10 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdfpsr_hip.so")
 SOURCES = ["api.hip", "frame.hip", "kernels.hip", "depth.hip"]
-HEADERS = ["geometry.h", "occlusion.h", "shading.h", "pipeline.h", "frame.h"]
+HEADERS = ["geometry.h", "occlusion.h", "shading.h", "pipeline.h", "frame.h", "dmf1.h"]
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++14", "-fPIC", "-shared",
     "-ffp-contract=off", "-fno-fast-math", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
@@ -33,9 +33,32 @@ def hipcc() -> str:
     return "hipcc"
 
 
+# Host-only sources of the library, plain C++14 built with g++ and linked into the library.
+HOST_SOURCES = ["dmf1.cpp"]
+HOST_FLAGS = ["-std=c++14", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra"]
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
+def build_host_objects(verbose: bool = False) -> list:
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objects = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        cmd = ["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"g++ failed to build {src}")
+        objects.append(obj)
+    return objects
+
+
 def build(verbose: bool = False) -> str:
+    objects = build_host_objects(verbose)
     cmd = [hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include")] + \
-        [os.path.join(CSRC, s) for s in SOURCES] + ["-o", LIB + ".tmp"]
+        [os.path.join(CSRC, s) for s in SOURCES] + ["-x", "none"] + objects + ["-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd))
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -6,6 +6,7 @@
 #include "../../include/dfpsr_hip.h"
 #include "frame.h"
 #include "occlusion.h"
+#include "dmf1.h"
 
 #include <algorithm>
 #include <atomic>
@@ -140,6 +141,8 @@ struct dfpsr_hip_model {
 	std::vector<float> colors, texcoords;
 	std::vector<int32_t> fanPolygon, polygonPart;
 	std::vector<const TextureDesc *> partDiffuse, partLight;
+	std::vector<const dfpsr_hip_texture *> partDiffuseHandle, partLightHandle;
+	std::vector<std::string> partNames;
 	DevBuf dPoints, dIndices, dColors, dTexcoords, dFanPolygon, dPolygonPart;
 	std::atomic<bool> dirty{true};
 	// Guards the upload of changed arrays: several threads may draw one model at once (the
@@ -235,7 +238,87 @@ extern "C" int dfpsr_hip_model_add_part(dfpsr_hip_model *m, int32_t count, const
 	}
 	m->partDiffuse.push_back(diffuse ? &diffuse->desc : nullptr);
 	m->partLight.push_back(light ? &light->desc : nullptr);
+	m->partDiffuseHandle.push_back(diffuse);
+	m->partLightHandle.push_back(light);
+	m->partNames.emplace_back();
 	m->dirty = true;
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_model_import_dmf1(dfpsr_hip_model **out, const char *content, int64_t length, int32_t detailLevel,
+                                           int32_t textureCount, const char *const *textureNames,
+                                           const dfpsr_hip_texture *const *textures) {
+	if (!out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_import_dmf1: null output");
+	*out = nullptr;
+	if (length < 0 || (length > 0 && !content) || textureCount < 0 || (textureCount > 0 && (!textureNames || !textures)))
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_import_dmf1: bad argument");
+	dmf1::Imported imported;
+	if (!dmf1::importModel(content, (size_t)length, detailLevel, imported))
+		return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_import_dmf1: the content does not start with \"DMF1\"");
+	// fetchTextureRgba from the caller's pool: the first entry whose name matches, ignoring
+	// ASCII case; an empty name or a name the pool lacks leaves the part without that texture.
+	auto fetch = [&](const std::string &name) -> const dfpsr_hip_texture * {
+		if (name.empty()) return nullptr;
+		for (int32_t i = 0; i < textureCount; i++)
+			if (textureNames[i] && dmf1::sameTextureName(name, textureNames[i])) return textures[i];
+		return nullptr;
+	};
+	std::unique_ptr<dfpsr_hip_model> m(new (std::nothrow) dfpsr_hip_model());
+	if (!m) return fail(DFPSR_HIP_ERROR_MEMORY, "model_import_dmf1: out of host memory");
+	int rc = dfpsr_hip_model_set_points(m.get(), (int32_t)(imported.points.size() / 3), imported.points.data());
+	if (rc) return rc;
+	for (const dmf1::ImportedPart &part : imported.parts) {
+		rc = dfpsr_hip_model_add_part(m.get(), (int32_t)(part.indices.size() / 4), part.indices.data(), part.colors.data(),
+		                              part.texcoords.data(), fetch(part.diffuseName), fetch(part.lightName));
+		if (rc) return rc;
+		m->partNames.back() = part.name;
+	}
+	*out = m.release();
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_model_get_points(const dfpsr_hip_model *m, int32_t *count, float *xyz) {
+	if (!m || !count) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_get_points: null argument");
+	if (m->pointFloats != 3) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_get_points: not a model of positions");
+	*count = m->pointCount();
+	if (xyz) std::memcpy(xyz, m->points.data(), m->points.size() * sizeof(float));
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_model_part_count(const dfpsr_hip_model *m, int32_t *out) {
+	if (!m || !out) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_part_count: null argument");
+	*out = (int32_t)m->partDiffuse.size();
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_model_get_part(const dfpsr_hip_model *m, int32_t part, int32_t *polygons, int32_t *indices, float *colors,
+                                        float *texcoords, const dfpsr_hip_texture **diffuse, const dfpsr_hip_texture **light) {
+	if (!m || !polygons) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_get_part: null argument");
+	if (part < 0 || part >= (int32_t)m->partDiffuse.size()) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_get_part: part index out of range");
+	// Parts own consecutive polygons (model_add_part appends them).
+	int32_t first = 0;
+	while (first < (int32_t)m->polygonPart.size() && m->polygonPart[(size_t)first] != part) first++;
+	int32_t count = 0;
+	while (first + count < (int32_t)m->polygonPart.size() && m->polygonPart[(size_t)(first + count)] == part) count++;
+	*polygons = count;
+	if (indices) std::memcpy(indices, &m->indices[(size_t)first * 4], (size_t)count * 4 * sizeof(int32_t));
+	if (colors) std::memcpy(colors, &m->colors[(size_t)first * 16], (size_t)count * 16 * sizeof(float));
+	if (texcoords) std::memcpy(texcoords, &m->texcoords[(size_t)first * 16], (size_t)count * 16 * sizeof(float));
+	if (diffuse) *diffuse = m->partDiffuseHandle[(size_t)part];
+	if (light) *light = m->partLightHandle[(size_t)part];
+	return DFPSR_HIP_OK;
+}
+
+extern "C" int dfpsr_hip_model_part_name(const dfpsr_hip_model *m, int32_t part, char *out, int32_t capacity, int32_t *length) {
+	if (!m) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_part_name: null model");
+	if (part < 0 || part >= (int32_t)m->partNames.size()) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_part_name: part index out of range");
+	const std::string &name = m->partNames[(size_t)part];
+	if (length) *length = (int32_t)name.size();
+	if (out && capacity > 0) {
+		const size_t n = std::min(name.size(), (size_t)capacity - 1);
+		std::memcpy(out, name.data(), n);
+		out[n] = '\0';
+	}
 	return DFPSR_HIP_OK;
 }
 

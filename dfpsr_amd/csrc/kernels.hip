@@ -953,6 +953,34 @@ __device__ int32_t nextRound(const int32_t *src, int32_t n, int32_t done, int32_
 	return take;
 }
 
+// The three places where the in-order tile walk differs between the RGBA8 shaders (MODE 0)
+// and the differentiable float shader (MODE 1), chosen by overload on the quad's pixel type.
+__device__ __forceinline__ void loadQuadColor(QuadPixels<0> &q, int s, bool in, const CanvasDev &cv, const FrameDev &f, int32_t x, int32_t y) {
+	q.color[s] = (in && cv.color && f.alphaAny) ? cv.color[(size_t)y * cv.colorStride + x] : 0u;
+}
+__device__ __forceinline__ void loadQuadColor(QuadPixels<1> &, int, bool, const CanvasDev &, const FrameDev &, int32_t, int32_t) {}
+
+__device__ __forceinline__ void drawQuadCommand(const FrameDev &f, const Command &cmd, int32_t c, int32_t ci, const TaskDev &task,
+                                                const QuadEval &e, QuadPixels<0> &q, bool colorWrite, bool depthRead, bool depthWrite,
+                                                bool alphaFilter) {
+	(void)ci;
+	(void)task;
+	drawCommand(f, cmd, c, e, q, colorWrite, depthRead, depthWrite, alphaFilter);
+}
+__device__ __forceinline__ void drawQuadCommand(const FrameDev &f, const Command &cmd, int32_t c, int32_t ci, const TaskDev &task,
+                                                const QuadEval &e, QuadPixels<1> &q, bool, bool depthRead, bool depthWrite, bool) {
+	drawCommandDiff(f, cmd, c - f.canvasCommandBase[ci], task, e, q, depthRead, depthWrite);
+}
+
+__device__ __forceinline__ void storeQuadColor(const QuadPixels<0> &q, int s, const CanvasDev &cv, int32_t x, int32_t y) {
+	cv.color[(size_t)y * cv.colorStride + x] = q.color[s];
+}
+__device__ __forceinline__ void storeQuadColor(const QuadPixels<1> &q, int s, const CanvasDev &cv, int32_t x, int32_t y) {
+	size_t p = (size_t)y * cv.diffStride + x;
+	*(float4 *)(cv.diffColor + p * 4) = make_float4(q.r[s], q.g[s], q.b[s], q.a[s]);
+	cv.visibility[p] = q.vis[s];
+}
+
 template <int TW, int TH, int MODE>
 __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowInterval *rows, int32_t *roundInfo) {
 	constexpr int32_t NL = (TW / 2) * (TH / 2);
@@ -973,7 +1001,7 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 		int32_t x = px + (s & 1), y = py + (s >> 1);
 		bool in = x < cv.width && y < cv.height;
 		q.depth[s] = (in && cv.depth) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
-		if constexpr (MODE == 0) q.color[s] = (in && cv.color && f.alphaAny) ? cv.color[(size_t)y * cv.colorStride + x] : 0u;
+		loadQuadColor(q, s, in, cv, f, x, y);
 	}
 
 	int32_t last = -1;
@@ -997,8 +1025,7 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 			__syncthreads();
 			QuadEval e;
 			if (py >= alT && py < alB && evalQuad(cmd, rows[py - tileY0], rows[py + 1 - tileY0], px, py, cv.height, e)) {
-				if constexpr (MODE == 0) drawCommand(f, cmd, c, e, q, colorWrite, depthRead, depthWrite, alphaFilter);
-				else drawCommandDiff(f, cmd, c - f.canvasCommandBase[ci], task, e, q, depthRead, depthWrite);
+				drawQuadCommand(f, cmd, c, ci, task, e, q, colorWrite, depthRead, depthWrite, alphaFilter);
 			}
 		}
 		done += take;
@@ -1008,15 +1035,7 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 	for (int s = 0; s < 4; s++) {
 		int32_t x = px + (s & 1), y = py + (s >> 1);
 		if (x >= cv.width || y >= cv.height) continue;
-		if ((q.colorDirty >> s) & 1u) {
-			if constexpr (MODE == 0) {
-				cv.color[(size_t)y * cv.colorStride + x] = q.color[s];
-			} else {
-				size_t p = (size_t)y * cv.diffStride + x;
-				*(float4 *)(cv.diffColor + p * 4) = make_float4(q.r[s], q.g[s], q.b[s], q.a[s]);
-				cv.visibility[p] = q.vis[s];
-			}
-		}
+		if ((q.colorDirty >> s) & 1u) storeQuadColor(q, s, cv, x, y);
 		if ((q.depthDirty >> s) & 1u) cv.depth[(size_t)y * cv.depthStride + x] = q.depth[s];
 	}
 }

@@ -106,6 +106,33 @@ int dfpsr_hip_model_set_points(dfpsr_hip_model *model, int32_t count, const floa
  * reference, rendererAPI.h:104-107). */
 int dfpsr_hip_model_add_part(dfpsr_hip_model *model, int32_t count, const int32_t *indices, const float *colors,
                              const float *texcoords, const dfpsr_hip_texture *diffuse, const dfpsr_hip_texture *light);
+/* importFromContent_DMF1(content, pool, detail_level) (api/modelAPI.h:319,
+ * implementation/render/model/format/dmf1.cpp): a new model from the text of a DMF1 file
+ * (length bytes, ASCII or UTF-8).  Parts outside [MinDetailLevel, MaxDetailLevel] are left out;
+ * equal points (closer than 0.00001) are merged; each triangle becomes a polygon.  Shader
+ * M_Diffuse_1Tex takes Texture[0] as the diffuse map and M_Diffuse_2Tex also Texture[1] as
+ * the light map, fetched from the caller's pool: texture_names[i] -> textures[i], names
+ * compared ignoring ASCII case, first match; a name the pool lacks leaves the part without
+ * that texture, as a texture the reference's pool fails to load does.  The model is Solid
+ * (the reference's import does not apply FilterType).  Fails with DFPSR_HIP_ERROR_ARGUMENT
+ * when the text does not start with "DMF1". */
+int dfpsr_hip_model_import_dmf1(dfpsr_hip_model **out, const char *content, int64_t length, int32_t detail_level,
+                                int32_t texture_count, const char *const *texture_names,
+                                const dfpsr_hip_texture *const *textures);
+/* model_getNumberOfPoints + model_getPoint (api/modelAPI.h:106-109): *count points into xyz
+ * (count x 3 floats; xyz may be NULL to ask for the count). */
+int dfpsr_hip_model_get_points(const dfpsr_hip_model *model, int32_t *count, float *xyz);
+/* model_getNumberOfParts (api/modelAPI.h:96). */
+int dfpsr_hip_model_part_count(const dfpsr_hip_model *model, int32_t *out);
+/* model_getNumberOfPolygons, model_getVertexPointIndex, model_getVertexColor, model_getTexCoord,
+ * model_getDiffuseMap, model_getLightMap (api/modelAPI.h:143-239) of one part: *polygons, then
+ * (each output may be NULL) polygons x 4 indices (-1 for a triangle's fourth corner),
+ * polygons x 16 colour and texture-coordinate floats, and the part's textures. */
+int dfpsr_hip_model_get_part(const dfpsr_hip_model *model, int32_t part, int32_t *polygons, int32_t *indices, float *colors,
+                             float *texcoords, const dfpsr_hip_texture **diffuse, const dfpsr_hip_texture **light);
+/* model_getPartName (api/modelAPI.h:102): the name's length in *length, and the name,
+ * NUL-terminated and cut to capacity - 1 bytes, in out (may be NULL). */
+int dfpsr_hip_model_part_name(const dfpsr_hip_model *model, int32_t part, char *out, int32_t capacity, int32_t *length);
 /* model_getBoundingBox (api/modelAPI.h:138). */
 int dfpsr_hip_model_bound(const dfpsr_hip_model *model, float minimum[3], float maximum[3]);
 /* model_renderDepth (api/modelAPI.h:291, modelAPI.cpp:202-206; renderCore.cpp:344-443): draws

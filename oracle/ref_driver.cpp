@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <string>
 #include <vector>
 
 using namespace dsr;
@@ -210,7 +211,81 @@ static void readCanvas(const scene_desc *s, BuiltScene &b, uint32_t *outColor, f
 	}
 }
 
+// A resource pool that loads nothing: every non-empty name gets a 1x1 texture whose one texel
+// holds the name's number (1, 2, ... in order of first request), so that the imported model's
+// part textures can be traced back to the names the file asked for.
+struct RecordingPool : public ResourcePool {
+	List<String> names;
+	List<TextureRgbaU8> textures;
+	const ImageRgbaU8 fetchImageRgba(const ReadableString &) override { return ImageRgbaU8(); }
+	const TextureRgbaU8 fetchTextureRgba(const ReadableString &name, int32_t) override {
+		if (string_length(name) == 0) return TextureRgbaU8();
+		for (int32_t i = 0; i < names.length(); i++) {
+			if (string_caseInsensitiveMatch(names[i], name)) return textures[i];
+		}
+		names.push(name);
+		TextureRgbaU8 t = texture_create_RgbaU8(1, 1, 1);
+		texture_writePixel(t, 0, 0, 0, (uint32_t)names.length());
+		textures.push(t);
+		return t;
+	}
+};
+
+static int32_t textureNumber(const TextureRgbaU8 &t) { return texture_exists(t) ? (int32_t)texture_readPixel(t, 0u, 0u, 0u) : 0; }
+
 extern "C" {
+
+// importFromContent_DMF1 of an ASCII text at one detail level.  Writes the points, per part
+// (polygon count, diffuse texture number, light texture number), each polygon's three corners
+// (point index, RGBA, u1 v1 u2 v2) and the part names and requested texture names, each
+// ending in a line feed.  Returns the polygon count, or -1 when a capacity is too small.
+int ref_import_dmf1(const char *text, int32_t detail, int32_t pointCapacity, float *points, int32_t *pointCount, int32_t partCapacity,
+                    int32_t *partInfo, int32_t *partCount, int32_t polygonCapacity, int32_t *indices, float *colors, float *texcoords,
+                    int32_t namesCapacity, char *names) {
+	try {
+		RecordingPool pool;
+		Model model = importFromContent_DMF1(String(text), pool, detail);
+		*pointCount = model_getNumberOfPoints(model);
+		*partCount = model_getNumberOfParts(model);
+		if (*pointCount > pointCapacity || *partCount > partCapacity) return -1;
+		for (int32_t p = 0; p < *pointCount; p++) {
+			FVector3D v = model_getPoint(model, p);
+			points[p * 3] = v.x; points[p * 3 + 1] = v.y; points[p * 3 + 2] = v.z;
+		}
+		int32_t polygon = 0;
+		std::string nameText;
+		for (int32_t part = 0; part < *partCount; part++) {
+			int32_t count = model_getNumberOfPolygons(model, part);
+			partInfo[part * 3] = count;
+			partInfo[part * 3 + 1] = textureNumber(model_getDiffuseMap(model, part));
+			partInfo[part * 3 + 2] = textureNumber(model_getLightMap(model, part));
+			String partName = model_getPartName(model, part);
+			for (int32_t c = 0; c < string_length(partName); c++) nameText.push_back((char)partName[c]);
+			nameText.push_back('\n');
+			for (int32_t i = 0; i < count; i++, polygon++) {
+				if (polygon >= polygonCapacity) return -1;
+				for (int32_t v = 0; v < 3; v++) {
+					indices[polygon * 3 + v] = model_getVertexPointIndex(model, part, i, v);
+					FVector4D col = model_getVertexColor(model, part, i, v);
+					FVector4D tc = model_getTexCoord(model, part, i, v);
+					float *cd = colors + (polygon * 3 + v) * 4;
+					float *td = texcoords + (polygon * 3 + v) * 4;
+					cd[0] = col.x; cd[1] = col.y; cd[2] = col.z; cd[3] = col.w;
+					td[0] = tc.x; td[1] = tc.y; td[2] = tc.z; td[3] = tc.w;
+				}
+			}
+		}
+		for (int32_t i = 0; i < pool.names.length(); i++) {
+			for (int32_t c = 0; c < string_length(pool.names[i]); c++) nameText.push_back((char)pool.names[i][c]);
+			nameText.push_back('\n');
+		}
+		if ((int32_t)nameText.size() + 1 > namesCapacity) return -1;
+		memcpy(names, nameText.c_str(), nameText.size() + 1);
+		return polygon;
+	} catch (...) {
+		return -2;
+	}
+}
 
 int ref_render_scene(const scene_desc *s, uint32_t *outColor, float *outDepth) {
 	try {
