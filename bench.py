@@ -255,6 +255,8 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=-1, help="1/0: replay the frame as a hipGraph (default: fastest)")
     ap.add_argument("--persistent", type=int, default=0, help="persistent raster workgroups per CU (0 = one per tile)")
     ap.add_argument("--raster-path", type=int, default=0, help="0 visibility raster where allowed, 1 in-order")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the untimed tuning sweep (with --tile-config and --graph, for profiling one tuning)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES between two device buffers three times (calibrates HBM counters)")
     args = ap.parse_args()
@@ -300,8 +302,8 @@ def main() -> int:
 
     # Untimed sweep of the renderer's tunings; the timed run uses the fastest (or the flags').
     sweep = {}
-    for tile_config in (0, 1, 2):
-        for graph in (False, True):
+    for tile_config in ((0, 1, 2) if not args.no_sweep else (max(args.tile_config, 0),)):
+        for graph in ((False, True) if not args.no_sweep else (bool(max(args.graph, 0)),)):
             r = make_renderer(tile_config, args.persistent, graph, args.raster_path)
             for _ in range(3):
                 step(r)

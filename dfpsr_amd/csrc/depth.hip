@@ -62,6 +62,9 @@ __device__ void drawDepthTriangle(const Projected &A, const Projected &B, const 
 // group.  The grid holds fanCount << groupShift threads or more, in workgroups of 256.
 __global__ void __launch_bounds__(256) k_depth_draw(const Projected *projected, const int32_t *indices, const int32_t *fanPolygon,
                                                     int32_t fanCount, int32_t groupShift, CameraParams cam, DepthCanvas cv) {
+	// One clipped polygon per lane group (at most 32 groups of 8 lanes in a workgroup), in LDS
+	// rather than in each lane's private scratch memory.
+	__shared__ ClipPolygon groupPolys[256 / 8];
 	const int32_t thread = blockIdx.x * blockDim.x + threadIdx.x;
 	const int32_t g = thread >> groupShift;
 	const int32_t lanes = 1 << groupShift;
@@ -78,9 +81,14 @@ __global__ void __launch_bounds__(256) k_depth_draw(const Projected *projected, 
 		drawDepthTriangle(A, B, C, perspective, cv, lane, lanes);
 		return;
 	}
-	ClipPolygon poly9;
-	poly9.init(A.cs, B.cs, C.cs);
-	for (int32_t s = 0; s < cam.clip.count; s++) poly9.clip(cam.clip.planes[s]);
+	// The group's first lane clips; the group's lanes (all in one wavefront, whose LDS accesses
+	// complete in order) then read the polygon.
+	ClipPolygon &poly9 = groupPolys[threadIdx.x >> groupShift];
+	if (lane == 0) {
+		poly9.init(A.cs, B.cs, C.cs);
+		for (int32_t s = 0; s < cam.clip.count; s++) poly9.clip(cam.clip.planes[s]);
+	}
+	__builtin_amdgcn_wave_barrier();
 	for (int32_t i = 0; i < poly9.count - 2; i++) {
 		const Projected a = cameraToScreen(cam, poly9.v[0].cs);
 		const Projected b = cameraToScreen(cam, poly9.v[1 + i].cs);

@@ -137,6 +137,15 @@ private:
 	// them (256 views of 256x256 with lists of several hundred commands: a forward pass of
 	// 2.7 ms with bins against 1.7 ms packed).
 	int32_t binCapacity_ = 64;
+	// Single-pass setup: command slots per setup block (every block may write that many commands
+	// from blockIdx * blockCapacity_ on).  Grows to the most commands a block has made; frames
+	// whose slots would pass SINGLE_SLOT_LIMIT, and differentiable frames (whose visibility ids
+	// count commands densely), take the two-pass setup.
+	int32_t blockCapacity_ = SCAN_BLOCK;
+	static constexpr int64_t SINGLE_SLOT_LIMIT = (int64_t)1 << 21;
+	bool singlePass(const RenderOptions &o, int32_t fanBlocks) const {
+		return o.mode == 0 && (int64_t)fanBlocks * blockCapacity_ <= SINGLE_SLOT_LIMIT;
+	}
 	static constexpr int64_t BIN_ENTRY_LIMIT = (int64_t)1 << 25;
 	// All per-frame tables in one upload, skipped when identical to the previous frame's.
 	std::vector<uint8_t> tables_, uploadedTables_;

@@ -23,7 +23,7 @@ Frame::~Frame() {
 
 int Frame::init() {
 	HIP_TRY(hipStreamCreate(&stream));  // blocking: ordered after work on the legacy default stream
-	HIP_TRY(hipHostMalloc((void **)&hostCounters_, 8 * sizeof(int32_t), hipHostMallocDefault));
+	HIP_TRY(hipHostMalloc((void **)&hostCounters_, HOST_COUNTERS * sizeof(int32_t), hipHostMallocDefault));
 	HIP_TRY(hipHostGetDevicePointer((void **)&hostCountersDevice_, hostCounters_, 0));
 	for (hipEvent_t &e : events_) HIP_TRY(hipEventCreate(&e));
 	int device = 0;
@@ -150,10 +150,14 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	if (beginThreads < COUNTERS) beginThreads = COUNTERS;
 	hipLaunchKernelGGL(k_frame_begin, dim3(blocksOf(beginThreads, 256)), dim3(256), 0, stream, f);
 	if (events) HIP_TRY(hipEventRecord(events_[1], stream));
-	hipLaunchKernelGGL(k_setup_count, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
-	const int32_t sumBlocks = fanBlocks <= SETUP_SUM_BLOCKS ? 1 : 0;
-	if (!sumBlocks) hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, fanBlocks, f.counters + 0);
-	hipLaunchKernelGGL(k_setup_emit, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f, sumBlocks);
+	if (f.blockCapacity > 0) {
+		hipLaunchKernelGGL(k_setup_single, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
+	} else {
+		hipLaunchKernelGGL(k_setup_count, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
+		const int32_t sumBlocks = fanBlocks <= SETUP_SUM_BLOCKS ? 1 : 0;
+		if (!sumBlocks) hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SCAN_BLOCK), 0, stream, f.blockSums, fanBlocks, f.counters + 0);
+		hipLaunchKernelGGL(k_setup_emit, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f, sumBlocks);
+	}
 	if (events) HIP_TRY(hipEventRecord(events_[2], stream));
 	if (f.binCapacity == 0) {
 	hipLaunchKernelGGL(k_scan_tiles_local, dim3(tileBlocks), dim3(SCAN_BLOCK), 0, stream, f);
@@ -275,6 +279,9 @@ int Frame::run(const RenderOptions &o) {
 		return fail(-4, "device allocation failed");
 
 	for (int pass = 1; pass <= 3; pass++) {
+		const bool single = singlePass(o, fanBlocks);
+		if (single && commandCapacity_ < fanBlocks * blockCapacity_) commandCapacity_ = fanBlocks * blockCapacity_;
+		if (pairCapacity_ < 4 * commandCapacity_) pairCapacity_ = 4 * commandCapacity_;
 		const bool direct = o.binning == 0 && (int64_t)tileTotal * binCapacity_ <= BIN_ENTRY_LIMIT;
 		const size_t listEntries = direct ? (size_t)tileTotal * binCapacity_ : (size_t)pairCapacity_;
 		if (dCommands_.ensure((size_t)commandCapacity_ * sizeof(Command)) != hipSuccess ||
@@ -327,6 +334,7 @@ int Frame::run(const RenderOptions &o) {
 		f.ftexLog2w = ftexLog2w_;
 		f.ftexLog2h = ftexLog2h_;
 		f.multipliers = multipliers_;
+		f.blockCapacity = single ? blockCapacity_ : 0;
 		f.occlusionGrid = occlusionGrid_;
 		f.gridW = gridW_;
 		f.gridH = gridH_;
@@ -401,7 +409,15 @@ int Frame::run(const RenderOptions &o) {
 			lastRunValid_ = true;
 			return 0;
 		}
-		if (overflow & 1) commandCapacity_ = hostCounters_[0] + 1024;
+		if (overflow & 1) {
+			if (single) {
+				// A setup block made more commands than its slots: every block gets room for that many.
+				const int32_t most = hostCounters_[8];
+				blockCapacity_ = std::max(blockCapacity_, ((most + 63) / 64) * 64);
+			} else {
+				commandCapacity_ = hostCounters_[0] + 1024;
+			}
+		}
 		if (overflow & 2) {
 			// Tile pairs no longer fit: the packed lists grow to the frame's pair count, the
 			// direct bins to the longest tile list with a quarter to spare.

@@ -191,9 +191,12 @@ __device__ __forceinline__ bool survivesOcclusion(const FrameDev &f, const Proje
 	                            P[1].cs.z, P[2].cs.z);
 }
 
-// corners: the triangle's corners when the caller has loaded them already.
+// corners: the triangle's corners when the caller has loaded them already.  clipPoly: the
+// calling wavefront's clipped polygon in LDS.  Triangles that need clipping are rare; the
+// lanes holding one take the wavefront's polygon in turn, so that the polygon (nine
+// sub-vertices indexed at run time) lives in LDS instead of private scratch memory.
 template <int MODE>
-__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset, BinTally &tally,
+__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset, BinTally &tally, ClipPolygon *clipPoly,
                                  BinJob *jobs = nullptr, int32_t jobBase = 0, const TriangleCorners *corners = nullptr) {
 	constexpr bool EMIT = MODE == SETUP_EMIT;
 	const TaskDev &t = f.tasks[ti];
@@ -233,7 +236,12 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 			count = 1;
 		}
 	} else {
-		ClipPolygon poly9;
+		// The lanes here are exactly those that clip: one at a time, in lane order.
+		const uint64_t clipping = __ballot(1);
+		const int32_t me = (int32_t)(threadIdx.x & 63);
+		for (uint64_t pending = clipping; pending != 0; pending &= pending - 1) {
+		if (me != __builtin_ctzll(pending)) continue;
+		ClipPolygon &poly9 = *clipPoly;
 		poly9.init(P[0].cs, P[1].cs, P[2].cs);
 		for (int32_t s = 0; s < cam.clip.count; s++) poly9.clip(cam.clip.planes[s]);
 		for (int32_t i = 0; i < poly9.count - 2; i++) {
@@ -257,6 +265,7 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 				}
 				count++;
 			}
+		}
 		}
 	}
 	return count;
@@ -300,10 +309,29 @@ __device__ int32_t blockExclusiveScan(int32_t value, int32_t &exclusive) {
 	return workgroupPrefix<SCAN_BLOCK>(value, exclusive, waveTotals);
 }
 
-__global__ void k_setup_count(FrameDev f) {
+// The task of fan triangle g, and whether every active lane of the wavefront has the same one.
+// A wavefront's triangles nearly always come from one task; the task index is then made
+// wave-uniform (readfirstlane) so that the task's fields -- camera planes, array bases -- are
+// read with scalar loads instead of one vector load per lane.
+__device__ __forceinline__ int32_t taskOf(const FrameDev &f, int32_t g, bool &uniform) {
+	const int32_t ti = findSegment(f.taskFanEnd, f.taskCount, g);
+	const int32_t first = __builtin_amdgcn_readfirstlane(ti);
+	uniform = __all(ti == first);
+	return ti;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_setup_count(FrameDev f) {
+	__shared__ ClipPolygon clipPolys[SCAN_BLOCK / 64];
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
 	BinTally unused{0, 0};
-	int32_t n = (g < f.fanTotal) ? setupTriangle<SETUP_COUNT>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), 0, unused) : 0;
+	int32_t n = 0;
+	if (g < f.fanTotal) {
+		bool uniform;
+		const int32_t ti = taskOf(f, g, uniform);
+		ClipPolygon *poly = &clipPolys[threadIdx.x >> 6];
+		if (uniform) n = setupTriangle<SETUP_COUNT>(f, g, __builtin_amdgcn_readfirstlane(ti), 0, unused, poly);
+		else n = setupTriangle<SETUP_COUNT>(f, g, ti, 0, unused, poly);
+	}
 	if (g < f.fanTotal) f.fanCount[g] = n;
 	int32_t ex;
 	int32_t total = blockExclusiveScan(n, ex);
@@ -385,37 +413,69 @@ __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_
 
 // sumBlocks = 1: each block adds up the totals of the blocks before it (and the last block writes
 // the command total) instead of reading offsets scanned by k_scan_block_sums.
-__global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
+// Writes the draw commands of one block of SCAN_BLOCK fan triangles.  Two-pass form (SINGLE =
+// false): k_setup_count has stored each triangle's command count, and the commands take dense
+// indices from the prefix over all blocks.  Single-pass form (SINGLE = true): the block counts
+// its own commands and writes them from index blockIdx.x * blockCapacity on, so no block waits
+// for another and no count kernel runs before; indices keep submission order but leave gaps
+// (the raster only orders commands by index).  A block with more commands than blockCapacity
+// writes nothing and reports its count, and the frame runs again with room for it.
+template <bool SINGLE>
+__device__ __forceinline__ void setupEmit(const FrameDev &f, int32_t sumBlocks) {
+	__shared__ ClipPolygon clipPolys[SCAN_BLOCK / 64];
 	__shared__ int32_t blockTally[2];
 	__shared__ int32_t recordBaseLds;
 	__shared__ BinJob jobs[BIN_JOBS];
 	__shared__ int32_t tableTile[TILE_TABLE];
 	__shared__ int32_t tableCount[TILE_TABLE];
 	int32_t g = blockIdx.x * SCAN_BLOCK + threadIdx.x;
-	int32_t n = (g < f.fanTotal) ? f.fanCount[g] : 0;
 	for (int32_t e = threadIdx.x; e < TILE_TABLE; e += SCAN_BLOCK) {  // empty before the prefix sums' barriers
 		tableTile[e] = -1;
 		tableCount[e] = 0;
 	}
-	// The corners are loaded before the block's prefix sums, so their latency overlaps them.
 	const int32_t ti = g < f.fanTotal ? findSegment(f.taskFanEnd, f.taskCount, g) : 0;
 	TriangleCorners corners;
-	if (n > 0) corners = loadCorners(f, g, ti);
+	int32_t n;
+	if (SINGLE) {
+		n = 0;
+		if (g < f.fanTotal) {
+			BinTally unused{0, 0};
+			n = setupTriangle<SETUP_COUNT>(f, g, ti, 0, unused, &clipPolys[threadIdx.x >> 6]);
+		}
+	} else {
+		n = (g < f.fanTotal) ? f.fanCount[g] : 0;
+		// The corners are loaded before the block's prefix sums, so their latency overlaps them.
+		if (n > 0) corners = loadCorners(f, g, ti);
+	}
 	int32_t ex;
 	const int32_t blockCommands = blockExclusiveScan(n, ex);
 	if (threadIdx.x < 2) blockTally[threadIdx.x] = 0;
 	int32_t jobBase;
-	if (sumBlocks) {
-		int32_t partial = 0, unusedPrefix;
-		for (int32_t i = threadIdx.x; i < (int32_t)blockIdx.x; i += SCAN_BLOCK) partial += f.blockSums[i];
-		jobBase = blockExclusiveScan(partial, unusedPrefix);
-		if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) f.counters[0] = jobBase + blockCommands;
+	bool room;
+	if (SINGLE) {
+		jobBase = (int32_t)blockIdx.x * f.blockCapacity;
+		room = blockCommands <= f.blockCapacity;
+		if (threadIdx.x == 0 && blockCommands > 0) {
+			atomicAdd(&f.counters[0], blockCommands);
+			if (!room) {
+				atomicOr(&f.counters[2], 1);
+				atomicMax(&f.counters[8], blockCommands);
+			}
+		}
 	} else {
-		jobBase = f.blockSums[blockIdx.x];
+		if (sumBlocks) {
+			int32_t partial = 0, unusedPrefix;
+			for (int32_t i = threadIdx.x; i < (int32_t)blockIdx.x; i += SCAN_BLOCK) partial += f.blockSums[i];
+			jobBase = blockExclusiveScan(partial, unusedPrefix);
+			if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) f.counters[0] = jobBase + blockCommands;
+		} else {
+			jobBase = f.blockSums[blockIdx.x];
+		}
+		room = jobBase + blockCommands <= f.commandCapacity;
 	}
 	// The block's commands are binned together (below) when they fit the table and the command
 	// arrays (a block past the capacity leaves its jobs unwritten).
-	const bool together = blockCommands <= BIN_JOBS && jobBase + blockCommands <= f.commandCapacity;
+	const bool together = blockCommands <= BIN_JOBS && room;
 	BinTally tally{0, 0};
 	if (g < f.fanTotal) {
 		const int32_t offset = jobBase + ex;
@@ -423,8 +483,9 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 		// triangle of each task bounds its canvas's first command: one atomic per task.
 		if (g == f.tasks[ti].fanBase) atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
 		if (n > 0) {
-			if (offset + n > f.commandCapacity) atomicOr(&f.counters[2], 1);
-			else setupTriangle<SETUP_EMIT>(f, g, ti, offset, tally, together ? jobs : nullptr, jobBase, &corners);
+			if (SINGLE) corners = loadCorners(f, g, ti);  // the count above read them; these loads hit the caches
+			if (!SINGLE && offset + n > f.commandCapacity) atomicOr(&f.counters[2], 1);
+			else if (room) setupTriangle<SETUP_EMIT>(f, g, ti, offset, tally, &clipPolys[threadIdx.x >> 6], together ? jobs : nullptr, jobBase, &corners);
 		}
 	}
 	if (together) {
@@ -503,6 +564,9 @@ __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks) {
 	if (threadIdx.x == 0 && blockTally[0]) atomicAdd(&f.counters[1], blockTally[0]);
 	if (threadIdx.x == 1 && blockTally[1]) atomicAdd(&f.counters[3], blockTally[1]);
 }
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_setup_emit(FrameDev f, int32_t sumBlocks) { setupEmit<false>(f, sumBlocks); }
+__global__ void __launch_bounds__(SCAN_BLOCK) k_setup_single(FrameDev f) { setupEmit<true>(f, 0); }
 
 // ---------------------------------------------------------------------------------------
 // Binning into screen tiles.  List slots come from atomics on the tile counts, so a list is
@@ -1047,7 +1111,7 @@ __device__ void rasterTile(const FrameDev &f, int32_t tile, int32_t *list, RowIn
 // The raster kernels start after every counter is final: workgroup 0 copies them to the
 // frame's pinned host memory, which the host reads after the stream completes.
 __device__ __forceinline__ void reportCounters(const FrameDev &f) {
-	if (blockIdx.x == 0 && threadIdx.x < 8) f.hostCounters[threadIdx.x] = f.counters[threadIdx.x];
+	if (blockIdx.x == 0 && threadIdx.x < HOST_COUNTERS) f.hostCounters[threadIdx.x] = f.counters[threadIdx.x];
 }
 
 // Runs body(tile) over the tiles with commands -- every tile when the raster fills the
@@ -1770,11 +1834,12 @@ __global__ void k_frame_begin(FrameDev f) {
 // renderer_occludeFromExistingTriangles (rendererAPI.cpp:241-258): the Solid draw commands of
 // every task given so far lower the grid under their hulls.  One thread per fan triangle,
 // after k_frame_begin has projected the points.
-__global__ void k_occlude_existing(FrameDev f) {
+__global__ void __launch_bounds__(256) k_occlude_existing(FrameDev f) {
+	__shared__ ClipPolygon clipPolys[256 / 64];
 	const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
 	if (g >= f.fanTotal) return;
 	BinTally unused{0, 0};
-	(void)setupTriangle<SETUP_OCCLUDE>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), 0, unused);
+	(void)setupTriangle<SETUP_OCCLUDE>(f, g, findSegment(f.taskFanEnd, f.taskCount, g), 0, unused, &clipPolys[threadIdx.x >> 6]);
 }
 
 // renderer_occludeFromTopRows (rendererAPI.cpp:403-474), one thread per cell: the first row of

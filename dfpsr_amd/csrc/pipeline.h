@@ -87,7 +87,9 @@ struct FrameDev {
 	int32_t commandCapacity;
 	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flags, [3] tiles with commands,
 	                              // [4] raster head, [5] backward head, [6] longest tile list when a
-	                              // bin overflowed, [7] pair records, [16..23] per-XCD raster heads
+	                              // bin overflowed, [7] pair records, [8] most commands of one setup
+	                              // block when a single-pass block overflowed, [16..23] per-XCD raster heads
+	int32_t blockCapacity;        // single-pass setup: command slots per setup block (0: two passes)
 	int32_t *tileCount;           // tileTotal + 1
 	// Tile lists.  With binCapacity > 0 tile t owns tileList[t * binCapacity, ...) and the setup
 	// kernel appends to it directly; otherwise the lists are packed at tileOffset (an exclusive
@@ -145,6 +147,7 @@ __global__ void k_setup_count(FrameDev f);
 __global__ void k_scan_block_sums(int32_t *blockSums, int32_t blocks, int32_t *total);
 __global__ void k_scan_block_sums64(long long *blockSums, int32_t blocks, int32_t *totalLow, int32_t *totalHigh);
 __global__ void k_setup_emit(FrameDev f, int32_t sumBlocks);
+__global__ void k_setup_single(FrameDev f);
 __global__ void k_occlude_existing(FrameDev f);
 __global__ void k_occlude_top_rows(const float *depth, int32_t width, int32_t height, int32_t stride, float *grid, int32_t gridW,
                                    int32_t gridH, int32_t perspective);
@@ -174,6 +177,7 @@ __global__ void k_depth_draw(const Projected *projected, const int32_t *indices,
 
 constexpr int32_t SCAN_BLOCK = 256;
 constexpr int32_t COUNTERS = 32;
+constexpr int32_t HOST_COUNTERS = 16;  // counters[0..15] are copied to pinned host memory
 
 // Workgroups are dispatched to gfx950's eight XCDs round-robin (workgroup b runs on XCD
 // b % 8, each XCD with its own L2).  The ordered tile queue is cut into segments of
