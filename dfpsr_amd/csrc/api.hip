@@ -471,13 +471,63 @@ static int scratchAddress(Projected **out) {
 	return DFPSR_HIP_OK;
 }
 
+// Tile shape of the depth-only frames: 32 x 16 pixels.
+static constexpr int32_t DEPTH_TILE_CONFIG = 1;
+
+// model_renderDepth through the tiled pipeline: one frame of one task (the model, its
+// transform and the camera) on the depth canvas alone, drawn by the depth-only tile kernel.
+// The frame, its buffers and its stream belong to the calling thread: threads drawing one model
+// into their own canvases never share working memory, and a call waits for its frame before
+// the same thread's next call reuses it.
+static int renderDepthTiled(dfpsr_hip_model *m, const float transform[12], const dfpsr_hip_image *depth, const CameraParams &cam) {
+	struct DepthFrame {
+		Frame frame;
+		DevBuf stage;  // a host canvas's device copy
+		bool ready = false;
+	};
+	static thread_local DepthFrame own;
+	if (!own.ready) {
+		int rc = own.frame.init();
+		if (rc) return rc;
+		own.ready = true;
+	}
+	const size_t canvasBytes = (size_t)depth->stride_bytes * depth->height;
+	float *pixels = (float *)depth->pixels;
+	if (!depth->on_device) {
+		if (own.stage.ensure(canvasBytes) != hipSuccess) return fail(DFPSR_HIP_ERROR_MEMORY, "model_render_depth: device allocation failed");
+		pixels = own.stage.as<float>();
+		HIP_TRY(hipMemcpy(pixels, depth->pixels, canvasBytes, hipMemcpyHostToDevice));
+	}
+	Frame &fr = own.frame;
+	fr.reset(DEPTH_TILE_CONFIG);
+	CanvasHost c;
+	c.color = nullptr;
+	c.depth = pixels;
+	c.width = depth->width;
+	c.height = depth->height;
+	c.colorStride = 0;
+	c.depthStride = depth->stride_bytes / 4;
+	const int32_t slot = fr.addCanvas(c);
+	ModelView v = m->view();
+	v.filter = 0;  // every polygon is drawn whatever the model's filter (Model.cpp:164-210)
+	int rc = fr.addTask(v, transform, cam, slot, POINTS_MODEL);
+	if (rc) return rc;
+	RenderOptions o;
+	o.tileConfig = DEPTH_TILE_CONFIG;
+	o.mode = 2;
+	rc = fr.render(o);
+	if (rc) return rc;
+	if (!depth->on_device) HIP_TRY(hipMemcpy(depth->pixels, pixels, canvasBytes, hipMemcpyDeviceToHost));
+	return DFPSR_HIP_OK;
+}
+
 // model_renderDepth (modelAPI.cpp:202-206): box cull, project the points, draw every fan
 // triangle's depth.  Synchronous like the reference: the canvas is final on return.
-// noAlloc = false: the projected points and a staged host canvas live in the calling
-// thread's growable buffers.  noAlloc = true: the projected points go to the shared scratch
-// array (at most NOALLOC_MAX_POINTS), the canvas must be device memory and the model
-// resident, and nothing is allocated; the lock keeps two calls in one process from using
-// the array at once.
+// noAlloc = false: the tiled pipeline (renderDepthTiled).  noAlloc = true: the projected
+// points go to the shared scratch array (at most NOALLOC_MAX_POINTS), the canvas must be
+// device memory and the model resident, nothing is allocated, and each fan triangle is drawn
+// by its own lanes with a compare-and-swap per pixel; the lock keeps two calls in one
+// process from using the array at once.
 static int renderDepth(const dfpsr_hip_model *cm, const float transform[12], const dfpsr_hip_image *depth,
                        const dfpsr_hip_camera *camera, const char *who, bool noAlloc) {
 	if (!cm || !transform) return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": null argument");
@@ -499,42 +549,26 @@ static int renderDepth(const dfpsr_hip_model *cm, const float transform[12], con
 	if (boxSeen(cam, cm->minBound, cm->maxBound, modelToWorld) == 0) return DFPSR_HIP_OK;
 	if (pointCount == 0 || fanCount == 0) return DFPSR_HIP_OK;
 	dfpsr_hip_model *m = const_cast<dfpsr_hip_model *>(cm);
-	if (noAlloc) {
-		if (m->dirty)
-			return fail(DFPSR_HIP_ERROR_STATE, std::string(who) + ": the model has changes that are not on the device yet "
-			                                                      "(any allocating draw of it uploads them)");
-	} else {
+	if (!noAlloc) {
 		rc = m->upload();
 		if (rc) return rc;
+		return renderDepthTiled(m, transform, depth, cam);
 	}
+	if (m->dirty)
+		return fail(DFPSR_HIP_ERROR_STATE, std::string(who) + ": the model has changes that are not on the device yet "
+		                                                      "(any allocating draw of it uploads them)");
 	const int32_t groupShift = depthGroupShift();
 	const int64_t drawThreads = (int64_t)fanCount << groupShift;
 	if (drawThreads > INT32_MAX - 255) return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": too many polygons for one launch");
-	const size_t canvasBytes = (size_t)depth->stride_bytes * depth->height;
-	std::unique_lock<std::mutex> hold(g_scratchLock, std::defer_lock);
+	std::lock_guard<std::mutex> hold(g_scratchLock);
 	Projected *projected = nullptr;
-	float *stagedCanvas = nullptr;
-	if (noAlloc) {
-		hold.lock();
-		rc = scratchAddress(&projected);
-		if (rc) return rc;
-	} else {
-		// Working memory belongs to the calling thread, not to the model: threads drawing one
-		// model into their own canvases never share it, and a call has finished with it (the
-		// call is synchronous) before the same thread's next call can grow it.
-		static thread_local DevBuf ownProjected, stage;
-		if (ownProjected.ensure((size_t)pointCount * sizeof(Projected)) != hipSuccess ||
-		    (!depth->on_device && stage.ensure(canvasBytes) != hipSuccess))
-			return fail(DFPSR_HIP_ERROR_MEMORY, std::string(who) + ": device allocation failed");
-		projected = ownProjected.as<Projected>();
-		stagedCanvas = stage.as<float>();
-	}
+	rc = scratchAddress(&projected);
+	if (rc) return rc;
 	DepthCanvas cv;
-	cv.depth = depth->on_device ? (float *)depth->pixels : stagedCanvas;
+	cv.depth = (float *)depth->pixels;
 	cv.width = depth->width;
 	cv.height = depth->height;
 	cv.stride = depth->stride_bytes / 4;
-	if (!depth->on_device) HIP_TRY(hipMemcpy(cv.depth, depth->pixels, canvasBytes, hipMemcpyHostToDevice));
 	// Every projected point the draw reads ([0, pointCount)) is written first: nothing in the
 	// shared array needs clearing between calls.
 	hipLaunchKernelGGL(k_depth_project, dim3((pointCount + 255) / 256), dim3(256), 0, 0, m->dPoints.as<float>(), pointCount,
@@ -543,7 +577,6 @@ static int renderDepth(const dfpsr_hip_model *cm, const float transform[12], con
 	hipLaunchKernelGGL(k_depth_draw, dim3((uint32_t)((drawThreads + 255) / 256)), dim3(256), 0, 0, projected,
 	                   m->dIndices.as<int32_t>(), m->dFanPolygon.as<int32_t>(), fanCount, groupShift, cam, cv);
 	HIP_TRY(hipGetLastError());
-	if (!depth->on_device) HIP_TRY(hipMemcpy(depth->pixels, cv.depth, canvasBytes, hipMemcpyDeviceToHost));
 	HIP_TRY(hipDeviceSynchronize());
 	return DFPSR_HIP_OK;
 }

@@ -66,7 +66,8 @@ struct RenderOptions {
 	bool clear = false;              // fill the canvases before drawing (image_fill)
 	uint32_t clearColor = 0;
 	float clearDepth = 0.0f;
-	int32_t mode = 0;                // 0: the reference's RGBA8 shaders; 1: the differentiable float shader
+	int32_t mode = 0;                // 0: the reference's RGBA8 shaders; 1: the differentiable float shader;
+	                                 // 2: depth only (model_renderDepth), one task, depth canvases only
 	int32_t rasterPath = 0;          // 0: visibility raster where the frame allows it; 1: always in order
 	int32_t binning = 0;             // 0: direct per-tile bins while they fit; 1: counted, scanned and filled lists
 };
@@ -144,7 +145,7 @@ private:
 	int32_t blockCapacity_ = SCAN_BLOCK;
 	static constexpr int64_t SINGLE_SLOT_LIMIT = (int64_t)1 << 21;
 	bool singlePass(const RenderOptions &o, int32_t fanBlocks) const {
-		return o.mode == 0 && (int64_t)fanBlocks * blockCapacity_ <= SINGLE_SLOT_LIMIT;
+		return o.mode != 1 && (int64_t)fanBlocks * blockCapacity_ <= SINGLE_SLOT_LIMIT;
 	}
 	static constexpr int64_t BIN_ENTRY_LIMIT = (int64_t)1 << 25;
 	// All per-frame tables in one upload, skipped when identical to the previous frame's.

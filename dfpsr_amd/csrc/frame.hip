@@ -174,7 +174,13 @@ int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks,
 	}
 	const int32_t persistent = o.persistentPerCU > 0 ? 1 : 0;
 	const int32_t grid = persistent ? deviceCUs_ * o.persistentPerCU : xcdGrid(tileTotal);
-	if (o.mode == 1) {
+	if (o.mode == 2) {
+		switch (tileConfig_) {
+		case 1: hipLaunchKernelGGL((k_raster_depth<32, 16>), dim3(xcdGrid(tileTotal)), dim3(32 * 16 / 4), 0, stream, f); break;
+		case 2: hipLaunchKernelGGL((k_raster_depth<32, 32>), dim3(xcdGrid(tileTotal)), dim3(32 * 32 / 4), 0, stream, f); break;
+		default: hipLaunchKernelGGL((k_raster_depth<16, 16>), dim3(xcdGrid(tileTotal)), dim3(16 * 16 / 4), 0, stream, f); break;
+		}
+	} else if (o.mode == 1) {
 		const bool deferred = deferredRaster(o);
 		switch (tileConfig_) {
 		case 1:
@@ -285,7 +291,7 @@ int Frame::run(const RenderOptions &o) {
 		const bool direct = o.binning == 0 && (int64_t)tileTotal * binCapacity_ <= BIN_ENTRY_LIMIT;
 		const size_t listEntries = direct ? (size_t)tileTotal * binCapacity_ : (size_t)pairCapacity_;
 		if (dCommands_.ensure((size_t)commandCapacity_ * sizeof(Command)) != hipSuccess ||
-		    dShaders_.ensure((size_t)commandCapacity_ * sizeof(ShaderInputs)) != hipSuccess ||
+		    (o.mode == 0 && dShaders_.ensure((size_t)commandCapacity_ * sizeof(ShaderInputs)) != hipSuccess) ||
 		    dTriangleRows_.ensure((size_t)commandCapacity_ * sizeof(TriangleRows)) != hipSuccess ||
 		    dTileList_.ensure(listEntries * 4) != hipSuccess ||
 		    (!direct && dPairRecords_.ensure((size_t)pairCapacity_ * sizeof(PairRecord)) != hipSuccess))

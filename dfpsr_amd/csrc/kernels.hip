@@ -1592,6 +1592,155 @@ template __global__ void k_raster_diff<16, 16>(FrameDev, int32_t);
 template __global__ void k_raster_diff<32, 16>(FrameDev, int32_t);
 template __global__ void k_raster_diff<32, 32>(FrameDev, int32_t);
 
+// ---------------------------------------------------------------------------------------
+// Depth-only tiles (model_renderDepth: executeTriangleDrawingDepth, renderCore.cpp:344-387).
+//
+// Each row of a draw command starts at the depth of its first pixel from the weight plane and
+// adds the plane's x step once per pixel; a pixel takes the value when it is strictly greater
+// than the depth held (perspective, 1/W) or strictly smaller (orthogonal, W).  The tile's
+// depths live in LDS as 64-bit keys: the depth's order in the high half (reversed for
+// orthogonal cameras, +0 and -0 equal), and below it the command's priority -- the held value
+// first, then the tile's commands in submission order -- and the sign of a zero depth.  An
+// atomic max over the keys of every covered pixel of every command leaves, per pixel, the
+// first command to reach the best depth: the value the in-order walk keeps, including which of
+// +0 and -0 when they tie.  NaN is never written and a held NaN is never replaced.  One thread
+// per (command, row): the row is exact (rasterizeTriangle), and its depth chain runs from the
+// row's first pixel, so columns left of the tile are stepped over without being written.
+// ---------------------------------------------------------------------------------------
+constexpr uint32_t DEPTH_HELD_PRIORITY = 0x7FFFFFFFu;
+
+__device__ __forceinline__ uint32_t depthOnlyOrder(float d, bool affine) {
+	const uint32_t o = depthOrder(d);
+	return affine ? ~o : o;
+}
+
+__device__ __forceinline__ unsigned long long depthOnlyKey(float d, bool affine, uint32_t priority) {
+	return ((unsigned long long)depthOnlyOrder(d, affine) << 32) | (unsigned long long)((priority << 1) | (__float_as_uint(d) >> 31));
+}
+
+// The depth a key holds.
+__device__ __forceinline__ float depthOfKey(unsigned long long key, bool affine) {
+	uint32_t o = (uint32_t)(key >> 32);
+	if (affine) o = ~o;
+	const uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+	return __uint_as_float(u == 0u ? (((uint32_t)key & 1u) << 31) : u);
+}
+
+template <int TW, int TH> struct DepthLds {
+	static constexpr int32_t NT = TW * TH / 4;
+	static constexpr int32_t CHUNK = 64;   // commands staged at once
+	static constexpr int32_t LIST = 256;   // commands sorted per round
+	int32_t list[LIST];
+	unsigned long long keys[TW * TH];
+	TriangleRows tr[CHUNK];
+	CommandTail tail[CHUNK];
+	int32_t rowFirst[CHUNK];
+	int32_t rowStart[CHUNK + 1];
+	int32_t roundInfo[4];
+	int32_t slot;
+};
+
+template <int TW, int TH>
+__device__ void depthTile(const FrameDev &f, int32_t tile, DepthLds<TW, TH> &s) {
+	using S = DepthLds<TW, TH>;
+	constexpr int32_t NT = S::NT, NP = TW * TH, CHUNK = S::CHUNK;
+	const int32_t n = f.tileCount[tile];
+	const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
+	const CanvasDev cv = f.canvases[ci];
+	const int32_t local = tile - cv.tileBase;
+	const int32_t tileX0 = (local % cv.tilesX) * TW;
+	const int32_t tileY0 = (local / cv.tilesX) * TH;
+	const int32_t t = threadIdx.x;
+	const int32_t *src = tileListOf(f, tile);
+	// One projection kind per depth-only frame (one model, one camera).
+	const bool affine = f.tasks[0].camera.perspective == 0;
+
+	for (int32_t p = t; p < NP; p += NT) {
+		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
+		const float d = (x < cv.width && y < cv.height) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
+		s.keys[p] = d != d ? ~0ull : depthOnlyKey(d, affine, DEPTH_HELD_PRIORITY);
+	}
+
+	int32_t last = -1;
+	int32_t done = 0;
+	while (done < n) {
+		const int32_t take = nextRound<NT, S::LIST>(src, n, done, last, s.list, s.roundInfo);
+		for (int32_t c0 = 0; c0 < take; c0 += CHUNK) {
+			const int32_t cn = take - c0 < CHUNK ? take - c0 : CHUNK;
+			stageToLds<NT>(s.tr, (const uint8_t *)f.triangleRows, (int32_t)sizeof(TriangleRows), 0, s.list, c0, cn);
+			stageToLds<NT>(s.tail, (const uint8_t *)f.commands, (int32_t)sizeof(Command), COMMAND_TAIL_OFFSET, s.list, c0, cn);
+			stageWait();
+			__syncthreads();
+			// Each command's rows inside the tile and inside the canvas, and their prefix.
+			if (t < 64) {
+				int32_t first = 0, count = 0;
+				if (t < cn) {
+					const int32_t bottom = tmin(tmin(s.tr[t].alB, cv.height) - tileY0, TH);
+					first = tmin(tmax(s.tr[t].alT - tileY0, 0), TH);
+					count = tmax(bottom - first, 0);
+				}
+				int32_t x = count;
+#pragma unroll
+				for (int32_t offset = 1; offset < 64; offset <<= 1) {
+					const int32_t y = __shfl_up(x, offset, 64);
+					if (t >= offset) x += y;
+				}
+				if (t < cn) {
+					s.rowFirst[t] = first;
+					s.rowStart[t] = x - count;
+				}
+				if (t == cn - 1) s.rowStart[cn] = x;
+			}
+			__syncthreads();
+			const int32_t rowItems = s.rowStart[cn];
+			for (int32_t it = t; it < rowItems; it += NT) {
+				const int32_t j = segmentOf(s.rowStart, cn, it);
+				const int32_t y = tileY0 + s.rowFirst[j] + (it - s.rowStart[j]);
+				const RowInterval row = triangleRow(s.tr[j], y);
+				const int32_t x1 = tmin(row.right, tileX0 + TW);
+				if (row.left >= x1 || row.right <= tileX0) continue;
+				const Projection &pl = s.tail[j].plane;
+				const float dx = pl.dx.x;
+				float value = weightAtPixel(pl, row.left, y).x;
+				int32_t x = row.left;
+#pragma clang loop unroll(disable)
+				for (; x < tileX0; x++) value += dx;
+				const uint32_t priority = DEPTH_HELD_PRIORITY - 1u - (uint32_t)(done + c0 + j);
+				unsigned long long *key = &s.keys[(y - tileY0) * TW + (x - tileX0)];
+#pragma clang loop unroll(disable)
+				for (; x < x1; x++) {
+					if (value == value) atomicMax(key, depthOnlyKey(value, affine, priority));
+					value += dx;
+					key++;
+				}
+			}
+			__syncthreads();
+		}
+		done += take;
+		if (take == 0) break;
+	}
+
+	for (int32_t p = t; p < NP; p += NT) {
+		const unsigned long long key = s.keys[p];
+		if (key == ~0ull || (uint32_t)key >> 1 == DEPTH_HELD_PRIORITY) continue;
+		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
+		if (x >= cv.width || y >= cv.height) continue;
+		cv.depth[(size_t)y * cv.depthStride + x] = depthOfKey(key, affine);
+	}
+}
+
+template <int TW, int TH>
+__global__ void __launch_bounds__(TW * TH / 4) k_raster_depth(FrameDev f) {
+	__shared__ DepthLds<TW, TH> s;
+	reportCounters(f);
+	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvas untouched
+	forEachQueuedTile(f, 0, &s.slot, [&](int32_t tile) { depthTile<TW, TH>(f, tile, s); });
+}
+
+template __global__ void k_raster_depth<16, 16>(FrameDev);
+template __global__ void k_raster_depth<32, 16>(FrameDev);
+template __global__ void k_raster_depth<32, 32>(FrameDev);
+
 // Orders one wave's LDS writes before its later reads of what other lanes of the wave wrote.
 __device__ __forceinline__ void waveSync() {
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

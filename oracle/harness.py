@@ -132,8 +132,16 @@ def ref_available() -> bool:
     return os.path.exists(REF_SO)
 
 
+_ref_exit_registered = False
+
+
 def load_ref():
+    global _ref_exit_registered
     lib = _load(REF_SO, "ref")
+    if not _ref_exit_registered and hasattr(lib, "ref_terminate"):
+        import atexit
+        atexit.register(lib.ref_terminate)
+        _ref_exit_registered = True
     lib.ref_render_scene.argtypes = [C.POINTER(SceneDesc), C.c_void_p, C.c_void_p]
     lib.ref_time_scene.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     lib.ref_render_depth_scene.argtypes = [C.POINTER(SceneDesc), C.c_void_p]

@@ -235,6 +235,10 @@ static int32_t textureNumber(const TextureRgbaU8 &t) { return texture_exists(t) 
 
 extern "C" {
 
+// The reference's own end-of-program step (heap.cpp heap_terminatingApplication): after it the
+// heap's static pool is torn down quietly.  Called once, when the loading process exits.
+void ref_terminate() { heap_terminatingApplication(); }
+
 // importFromContent_DMF1 of an ASCII text at one detail level.  Writes the points, per part
 // (polygon count, diffuse texture number, light texture number), each polygon's three corners
 // (point index, RGBA, u1 v1 u2 v2) and the part names and requested texture names, each

@@ -402,7 +402,37 @@ def depth_scenes():
         sc.has_color, sc.has_depth = False, True
         sc.name = f"depth_{sc.name}"
         out.append(sc)
+    out.append(zero_plane_depth_scene())
+    out.append(dense_depth_scene())
     return out
+
+
+def zero_plane_depth_scene():
+    """Orthogonal view of overlapping triangles in the camera's own z = 0 plane (corners at +0
+    or -0), drawn over a canvas preloaded with -0, +0, NaN and other special values: every
+    drawn depth is zero, so the held -0 and +0 tie with it and must be kept (a strictly smaller
+    value replaces)."""
+    n, w, h = 60, 48, 32
+    rng = np.random.default_rng(4321)
+    sc = S.cube_soup(width=w, height=h, triangles=n, seed=4321, spread=0.6, cam_distance=0.0, width_slope=1.5,
+                     name="depth_zero_plane")
+    pts = sc.points.reshape(n, 3, 3).copy()
+    negative = rng.integers(0, 2, size=n)[:, None] == 1
+    pts[:, :, 2] = np.where(negative, np.float32(-0.0), np.float32(0.0))
+    sc.points = pts.reshape(-1, 3).astype(np.float32)
+    sc.perspective = False
+    sc.has_color, sc.has_depth = False, True
+    sc.initial_depth = np.full((h, w), np.inf, np.float32)
+    return special_depth_canvas(sc, 4321)
+
+
+def dense_depth_scene():
+    """Thousands of overlapping triangles on a small canvas: tiles hold several hundred draw
+    commands, more than one sorted round of a tile's list."""
+    sc = S.cube_soup(width=96, height=64, triangles=3000, seed=97, spread=0.5, cam_distance=1.6, width_slope=1.0,
+                     name="depth_dense")
+    sc.has_color, sc.has_depth = False, True
+    return sc
 
 
 def special_depth_canvas(scene, seed):
