@@ -111,6 +111,7 @@ _SIGNATURES = {
     "dfpsr_hip_diff_stats": ([_P, _P], C.c_int),
     "dfpsr_hip_diff_set_timing": ([_P, _I], C.c_int),
     "dfpsr_hip_diff_timing": ([_P, _P], C.c_int),
+    "dfpsr_hip_probe_edge_side": ([_P, _P, _P, _P, _I, _P], C.c_int),
 }
 
 

@@ -1114,3 +1114,29 @@ extern "C" int dfpsr_hip_diff_timing(const dfpsr_hip_diff *d, float out[5]) {
 	for (int i = 0; i < 5; i++) out[i] = d->frame.stageMs[i];
 	return DFPSR_HIP_OK;
 }
+
+// ---------------------------------------------------------------------------------------
+// Test hook
+// ---------------------------------------------------------------------------------------
+
+extern "C" int dfpsr_hip_probe_edge_side(const int64_t *n, const int64_t *d, const int32_t *L, const int32_t *R, int32_t count,
+                                         int32_t *out) {
+	if (count < 0 || (count > 0 && (!n || !d || !L || !R || !out))) return fail(DFPSR_HIP_ERROR_ARGUMENT, "probe_edge_side: bad argument");
+	for (int32_t i = 0; i < count; i++)
+		if (d[i] == 0) return fail(DFPSR_HIP_ERROR_ARGUMENT, "probe_edge_side: zero divisor");
+	if (count == 0) return DFPSR_HIP_OK;
+	DevBuf buf;
+	const size_t c = (size_t)count;
+	if (buf.ensure(c * 28) != hipSuccess) return fail(DFPSR_HIP_ERROR_MEMORY, "probe_edge_side: device allocation failed");
+	uint8_t *base = (uint8_t *)buf.ptr;
+	int64_t *dn = (int64_t *)base, *dd = (int64_t *)(base + c * 8);
+	int32_t *dl = (int32_t *)(base + c * 16), *dr = (int32_t *)(base + c * 20), *dout = (int32_t *)(base + c * 24);
+	HIP_TRY(hipMemcpy(dn, n, c * 8, hipMemcpyHostToDevice));
+	HIP_TRY(hipMemcpy(dd, d, c * 8, hipMemcpyHostToDevice));
+	HIP_TRY(hipMemcpy(dl, L, c * 4, hipMemcpyHostToDevice));
+	HIP_TRY(hipMemcpy(dr, R, c * 4, hipMemcpyHostToDevice));
+	hipLaunchKernelGGL(k_probe_edge_side, dim3((count + 255) / 256), dim3(256), 0, 0, dn, dd, dl, dr, count, dout);
+	HIP_TRY(hipGetLastError());
+	HIP_TRY(hipMemcpy(out, dout, c * 4, hipMemcpyDeviceToHost));
+	return DFPSR_HIP_OK;
+}

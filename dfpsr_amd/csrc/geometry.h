@@ -485,9 +485,41 @@ DH TriangleRows triangleRows(const l2 &A, const l2 &B, const l2 &C, const Rect &
 	return tr;
 }
 
+// The float nearest to an integer below 2^46 in magnitude: both 23-bit halves are exact in
+// float, so their sum is rounded once.
+__device__ __forceinline__ float nearestFloat46(int64_t v) {
+	return (float)(int32_t)(v >> 23) * 8388608.0f + (float)(int32_t)(v & 0x7FFFFF);
+}
+
+// clamp((int32_t)(trunc(n / d) + 1), L, R) for d != 0, as the rows take an edge's side.
+// Operands below 2^46 with L and R within +-65536 take a float estimate of the quotient: its
+// relative error stays below 2^-21, so an estimate at least 4 above R (or 6 below L) decides
+// the clamp, and one nearer is within one of trunc(n / d), which one exact remainder step
+// (trunc semantics: the remainder has the dividend's sign) corrects.  Estimates of 2^30 and
+// more, and larger operands, take the exact division, which also keeps the int32 conversion
+// of huge quotients as it is.
+__device__ __forceinline__ int32_t edgeSide(int64_t n, int64_t d, int32_t L, int32_t R) {
+	const int64_t big = (int64_t)1 << 46;
+	if (n < big && n > -big && d < big && d > -big && L >= -65536 && R <= 65536) {
+		const float est = nearestFloat46(n) * __builtin_amdgcn_rcpf(nearestFloat46(d));
+		if (est < 1073741824.0f && est > -1073741824.0f) {
+			if (est >= (float)R + 4.0f) return R;
+			if (est <= (float)L - 6.0f) return tmin(L, R);
+			int64_t q = (int64_t)(int32_t)est;
+			const int64_t r = n - q * d;
+			const int64_t dir = ((n < 0) == (d < 0)) ? 1 : -1;  // sign of the true quotient
+			const int64_t ad = d < 0 ? -d : d;
+			if (r != 0 && ((r < 0) != (n < 0))) q -= dir;       // one step too far from zero
+			else if ((r < 0 ? -r : r) >= ad) q += dir;           // one step short
+			return tmin(tmax(L, (int32_t)(q + 1)), R);
+		}
+	}
+	return tmin(tmax(L, (int32_t)(divTrunc64(n, d) + 1)), R);
+}
+
 // Row y of the triangle (rasterizeTriangle, ITriangle2D.cpp:150-169, row y - alT of the
 // aligned bound); empty outside the aligned bound.
-DH RowInterval triangleRow(const TriangleRows &tr, int32_t y) {
+__device__ __forceinline__ RowInterval triangleRow(const TriangleRows &tr, int32_t y) {
 	RowInterval row{0, 0};
 	if (y < tr.alT || y >= tr.alB) return row;
 	const int32_t L = tr.clipL, R = tr.clipR;
@@ -500,12 +532,10 @@ DH RowInterval triangleRow(const TriangleRows &tr, int32_t y) {
 		const int32_t kind = (tr.flags >> (2 * k)) & 3;
 		if (kind == 1) {
 			int64_t limit = e.c - e.b * r;
-			int32_t side = tmin(tmax(L, (int32_t)(divTrunc64(limit + 1, e.a) + 1)), R);
-			row.left = tmax(row.left, side);
+			row.left = tmax(row.left, edgeSide(limit + 1, e.a, L, R));
 		} else if (kind == 2) {
 			int64_t limit = e.c - e.b * r;
-			int32_t side = tmin(tmax(L, (int32_t)(divTrunc64(limit, e.a) + 1)), R);
-			row.right = tmin(row.right, side);
+			row.right = tmin(row.right, edgeSide(limit, e.a, L, R));
 		} else if (kind == 3) {
 			int64_t value = e.c + e.b * r;
 			if (value > e.a) { row.left = R; row.right = L; }

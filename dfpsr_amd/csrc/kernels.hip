@@ -2020,6 +2020,13 @@ __global__ void k_occlude_top_rows(const float *depth, int32_t width, int32_t he
 	if (farthest < grid[cell]) grid[cell] = farthest;
 }
 
+// The side a row takes from one edge (edgeSide) for count (n, d, L, R) inputs: the check of
+// the row intervals' quotient against exact integer arithmetic.
+__global__ void k_probe_edge_side(const int64_t *n, const int64_t *d, const int32_t *L, const int32_t *R, int32_t count, int32_t *out) {
+	const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < count) out[i] = edgeSide(n[i], d[i], L[i], R[i]);
+}
+
 __global__ void k_fill(int32_t *p, int32_t n, int32_t value) {
 	int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) p[i] = value;

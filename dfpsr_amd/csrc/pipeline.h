@@ -160,6 +160,7 @@ template <int TW, int TH, int P> __global__ void k_raster_deferred(FrameDev f);
 template <int TW, int TH> __global__ void k_raster_diff(FrameDev f, int32_t persistent);
 template <int TW, int TH, int P> __global__ void k_raster_deferred_diff(FrameDev f);
 template <int TW, int TH> __global__ void k_raster_depth(FrameDev f);
+__global__ void k_probe_edge_side(const int64_t *n, const int64_t *d, const int32_t *L, const int32_t *R, int32_t count, int32_t *out);
 __global__ void k_backward(FrameDev f, int32_t *head);
 __global__ void k_fill(int32_t *p, int32_t n, int32_t value);
 __global__ void k_clear_canvases(FrameDev f, uint32_t color, float depth);

@@ -367,6 +367,14 @@ int dfpsr_hip_diff_stats(const dfpsr_hip_diff *diff, int64_t out[4]);
 int dfpsr_hip_diff_set_timing(dfpsr_hip_diff *diff, int32_t enable);
 int dfpsr_hip_diff_timing(const dfpsr_hip_diff *diff, float out[5]);
 
+/* ---- Test hook --------------------------------------------------------------------------
+ * No counterpart in the reference.  For each i, the column the device's row intervals take
+ * from one edge: clamp((int32_t)(trunc(n[i] / d[i]) + 1), L[i], R[i]), the truncating int64
+ * division of ITriangle2D.cpp:86-148 followed by the row's clamp to its clip columns.  Lets
+ * the tests check the device arithmetic against exact integers.  d[i] must not be 0. */
+int dfpsr_hip_probe_edge_side(const int64_t *n, const int64_t *d, const int32_t *L, const int32_t *R, int32_t count,
+                              int32_t *out);
+
 #ifdef __cplusplus
 }
 #endif
