@@ -1508,13 +1508,13 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 					float wa[4], wb[4], wc[4];
 					for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
 					// All four lanes are shaded (every texel index is masked into its level), so the
-					// sixteen texel loads of the quad can be issued together; only won lanes are kept.
-					float r[4], g[4], b[4], a[4];
-					shadeQuad(s.si[j], textures, wa, wb, wc, 0xFu, r, g, b, a);
+					// texel loads of a lane pair can be issued together; only won lanes are kept.
+					uint32_t packed[4];
+					shadeQuadPacked(s.si[j], textures, wa, wb, wc, packed);
 					for (int l = 0; l < 4; l++) {
 						if (!((mask >> l) & 1u)) continue;
 						const int32_t sl = (int32_t)((slots >> (2 * l)) & 3u);
-						s.color[(qy + (sl >> 1)) * TW + qx + (sl & 1)] = packSaturated(r[l], g[l], b[l], a[l]);
+						s.color[(qy + (sl >> 1)) * TW + qx + (sl & 1)] = packed[l];
 					}
 				}
 			}

@@ -303,6 +303,68 @@ DH void shadeQuad(const ShaderInputs &s, const TextureDesc *textures, const floa
 	}
 }
 
+// shadeQuad for all four lanes followed by packSaturated, two lanes at a time so that fewer
+// values are live at once (the texel loads of a lane pair are issued together).  Each lane's
+// arithmetic is shadeQuad's; the mip level still comes from lanes 0, 1 and 2.
+__device__ __forceinline__ void shadeQuadPacked(const ShaderInputs &s, const TextureDesc *textures, const float *wa, const float *wb,
+                                                const float *wc, uint32_t *out) {
+	const bool textureOnly = s.colorless && ((s.diffuse >= 0) != (s.light >= 0));
+	uint32_t mip = 0;
+	if (s.diffuse >= 0) {
+		float u[3], v[3];
+		for (int l = 0; l < 3; l++) { u[l] = interp(s.u1, wa[l], wb[l], wc[l]); v[l] = interp(s.v1, wa[l], wb[l], wc[l]); }
+		mip = mipLevelIndex(textures[s.diffuse], u[0], u[1], u[2], v[0], v[1], v[2]);
+	}
+	for (int h = 0; h < 4; h += 2) {
+		float r[2], g[2], b[2], a[2];
+		if (!textureOnly) {
+			for (int l = 0; l < 2; l++) {
+				const float la = wa[h + l], lb = wb[h + l], lc = wc[h + l];
+				if (s.fading) {
+					r[l] = interp(s.red, la, lb, lc);
+					g[l] = interp(s.green, la, lb, lc);
+					b[l] = interp(s.blue, la, lb, lc);
+					a[l] = interp(s.alpha, la, lb, lc);
+				} else {
+					r[l] = s.red.x; g[l] = s.green.x; b[l] = s.blue.x; a[l] = s.alpha.x;
+				}
+			}
+		}
+		if (s.diffuse >= 0) {
+			const TextureDesc &t = textures[s.diffuse];
+			BilinearSample bs[2];
+			for (int l = 0; l < 2; l++)
+				bs[l] = bilinearSample(t, interp(s.u1, wa[h + l], wb[h + l], wc[h + l]), interp(s.v1, wa[h + l], wb[h + l], wc[h + l]), mip, false);
+			uint32_t px[8];
+			for (int l = 0; l < 2; l++)
+				for (int k = 0; k < 4; k++) px[l * 4 + k] = loadTexel(t.texels, bs[l].offset[k]);
+			for (int l = 0; l < 2; l++) {
+				float sr, sg, sb, sa;
+				unpackColor(mixSample(bs[l], px[l * 4], px[l * 4 + 1], px[l * 4 + 2], px[l * 4 + 3]), sr, sg, sb, sa);
+				if (textureOnly) {
+					r[l] = sr; g[l] = sg; b[l] = sb; a[l] = sa;
+				} else {
+					r[l] = r[l] * sr; g[l] = g[l] * sg; b[l] = b[l] * sb; a[l] = a[l] * sa;
+				}
+			}
+		}
+		if (s.light >= 0) {
+			const TextureDesc &t = textures[s.light];
+			for (int l = 0; l < 2; l++) {
+				float sr, sg, sb, sa;
+				unpackColor(sampleBilinear(t, interp(s.u2, wa[h + l], wb[h + l], wc[h + l]), interp(s.v2, wa[h + l], wb[h + l], wc[h + l]), 0u, true),
+				            sr, sg, sb, sa);
+				if (textureOnly) {
+					r[l] = sr; g[l] = sg; b[l] = sb; a[l] = sa;
+				} else {
+					r[l] = r[l] * sr; g[l] = g[l] * sg; b[l] = b[l] * sb; a[l] = a[l] * sa;
+				}
+			}
+		}
+		for (int l = 0; l < 2; l++) out[h + l] = packSaturated(r[l], g[l], b[l], a[l]);
+	}
+}
+
 // Lane l of shadeQuad: the same operations for one pixel of the quad.  The mip level still
 // comes from the texture coordinates of lanes 0, 1 and 2 (texture_getMipLevelIndex).
 DH float pickLane(const float *v, int l) { return l == 0 ? v[0] : (l == 1 ? v[1] : (l == 2 ? v[2] : v[3])); }
