@@ -527,19 +527,17 @@ __device__ __forceinline__ RowInterval triangleRow(const TriangleRows &tr, int32
 	const int64_t r = y - tr.alT;
 	row.left = L;
 	row.right = R;
+	// Every edge takes the same steps whatever its kind (lanes of one wavefront hold edges of
+	// different kinds); the kind picks which result is kept.
 	for (int k = 0; k < 3; k++) {
 		const EdgeRows &e = tr.edges[k];
 		const int32_t kind = (tr.flags >> (2 * k)) & 3;
-		if (kind == 1) {
-			int64_t limit = e.c - e.b * r;
-			row.left = tmax(row.left, edgeSide(limit + 1, e.a, L, R));
-		} else if (kind == 2) {
-			int64_t limit = e.c - e.b * r;
-			row.right = tmin(row.right, edgeSide(limit, e.a, L, R));
-		} else if (kind == 3) {
-			int64_t value = e.c + e.b * r;
-			if (value > e.a) { row.left = R; row.right = L; }
-		}
+		const int64_t along = e.b * r;
+		const bool sloped = kind == 1 || kind == 2;
+		const int32_t side = edgeSide(e.c - along + (kind == 1 ? 1 : 0), sloped ? e.a : 1, L, R);
+		if (kind == 1) row.left = tmax(row.left, side);
+		if (kind == 2) row.right = tmin(row.right, side);
+		if (kind == 3 && e.c + along > e.a) { row.left = R; row.right = L; }
 	}
 	return row;
 }
