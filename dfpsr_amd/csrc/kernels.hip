@@ -734,15 +734,25 @@ __device__ __forceinline__ bool evalQuad(const Projection &cmd, RowInterval uppe
 	const f3 dx2 = scale3(dx, 2.0f);
 	f3 wU = weightAtPixel(cmd, outerBlockStart, py);
 	f3 wL = add3(wU, cmd.dy);
-	e.clipSides = true;
-	bool splitSpan = innerBlockEnd > innerBlockStart;
-	if (splitSpan && px >= innerBlockStart && px < innerBlockEnd) {
-		// Unclipped inner span: lanes start at the span and step per lane.
-		stepPair(wU, wL, dx2, (innerBlockStart - outerBlockStart) >> 1);
-		e.clipSides = false;
-		e.l0[0] = wU.x; e.l0[1] = wU.x + dx.x; e.l0[2] = wL.x; e.l0[3] = wL.x + dx.x;
-		e.l1[0] = wU.y; e.l1[1] = wU.y + dx.y; e.l1[2] = wL.y; e.l1[3] = wL.y + dx.y;
-		e.l2[0] = wU.z; e.l2[1] = wU.z + dx.z; e.l2[2] = wL.z; e.l2[3] = wL.z + dx.z;
+	const bool splitSpan = innerBlockEnd > innerBlockStart;
+	const bool inner = splitSpan && px >= innerBlockStart && px < innerBlockEnd;  // unclipped inner span
+	const bool right = splitSpan && px >= innerBlockEnd;                         // clipped, right of it
+	// The lanes of a wavefront take the same steps in the same places whatever part of the
+	// row pair their quads are in: the pair chain to the quad (left part) or to the inner
+	// span's start; past the inner span, its whole length at once and the chain to the quad.
+	stepPair(wU, wL, dx2, (((inner || right) ? innerBlockStart : px) - outerBlockStart) >> 1);
+	if (right) {
+		float quadCount = (float)((innerBlockEnd - innerBlockStart) / 2);
+		wU = add3(wU, scale3(dx2, quadCount));
+		wL = add3(wL, scale3(dx2, quadCount));
+		stepPair(wU, wL, dx2, (px - innerBlockEnd) >> 1);
+	}
+	e.clipSides = !inner;
+	e.l0[0] = wU.x; e.l0[1] = wU.x + dx.x; e.l0[2] = wL.x; e.l0[3] = wL.x + dx.x;
+	e.l1[0] = wU.y; e.l1[1] = wU.y + dx.y; e.l1[2] = wL.y; e.l1[3] = wL.y + dx.y;
+	e.l2[0] = wU.z; e.l2[1] = wU.z + dx.z; e.l2[2] = wL.z; e.l2[3] = wL.z + dx.z;
+	if (inner) {
+		// Inside the span the four lanes step on their own, a quad at a time.
 		int32_t x = innerBlockStart;
 #pragma clang loop unroll(disable)
 		for (; x + 4 <= px; x += 4) {
@@ -753,19 +763,6 @@ __device__ __forceinline__ bool evalQuad(const Projection &cmd, RowInterval uppe
 		for (; x < px; x += 2) {
 			for (int l = 0; l < 4; l++) { e.l0[l] = e.l0[l] + dx2.x; e.l1[l] = e.l1[l] + dx2.y; e.l2[l] = e.l2[l] + dx2.z; }
 		}
-	} else {
-		if (!splitSpan || px < innerBlockStart) {
-			stepPair(wU, wL, dx2, (px - outerBlockStart) >> 1);
-		} else {
-			stepPair(wU, wL, dx2, (innerBlockStart - outerBlockStart) >> 1);
-			float quadCount = (float)((innerBlockEnd - innerBlockStart) / 2);
-			wU = add3(wU, scale3(dx2, quadCount));
-			wL = add3(wL, scale3(dx2, quadCount));
-			stepPair(wU, wL, dx2, (px - innerBlockEnd) >> 1);
-		}
-		e.l0[0] = wU.x; e.l0[1] = wU.x + dx.x; e.l0[2] = wL.x; e.l0[3] = wL.x + dx.x;
-		e.l1[0] = wU.y; e.l1[1] = wU.y + dx.y; e.l1[2] = wL.y; e.l1[3] = wL.y + dx.y;
-		e.l2[0] = wU.z; e.l2[1] = wU.z + dx.z; e.l2[2] = wL.z; e.l2[3] = wL.z + dx.z;
 	}
 	e.slot[0] = hasTop ? 0 : 2;
 	e.slot[1] = hasTop ? 1 : 3;
