@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdfpsr_hip.so")
+# DFPSR_HIP_LIBRARY names another build of the library (the host-sanitized one, for instance).
+LIB_PATH = os.environ.get("DFPSR_HIP_LIBRARY") or os.path.join(HERE, "libdfpsr_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "dfpsr_hip.h")
 
 OK = 0

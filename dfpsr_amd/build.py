@@ -39,12 +39,21 @@ HOST_FLAGS = ["-std=c++14", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math
 OBJ_DIR = os.path.join(HERE, "_build")
 
 
-def build_host_objects(verbose: bool = False) -> list:
+# Host-side sanitizers (AddressSanitizer, UndefinedBehaviorSanitizer) for the library's CPU
+# code: argument checks, model and texture bookkeeping, the DMF1 parser, the occlusion grid's
+# host part.  Device code is not instrumented.  The plain host sources are built with clang here
+# (not g++), so that the process needs only clang's sanitizer runtime.
+SANITIZE_FLAGS = ["-fsanitize=address", "-fsanitize=undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g"]
+SANITIZED_LIB = os.path.join(HERE, "_build", "libdfpsr_hip_san.so")
+CLANGXX = "/opt/rocm/lib/llvm/bin/clang++"
+
+
+def build_host_objects(verbose: bool = False, sanitize: bool = False) -> list:
     os.makedirs(OBJ_DIR, exist_ok=True)
     objects = []
     for src in HOST_SOURCES:
-        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
-        cmd = ["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ("_san.o" if sanitize else ".o"))
+        cmd = ([CLANGXX] + HOST_FLAGS + SANITIZE_FLAGS if sanitize else ["g++"] + HOST_FLAGS) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
         res = subprocess.run(cmd, capture_output=True, text=True)
@@ -53,6 +62,23 @@ def build_host_objects(verbose: bool = False) -> list:
             raise RuntimeError(f"g++ failed to build {src}")
         objects.append(obj)
     return objects
+
+
+def build_sanitized(verbose: bool = False) -> str:
+    """The library with its host code sanitized, into dfpsr_amd/_build/ (tests/run_sanitized.py)."""
+    objects = build_host_objects(verbose, sanitize=True)
+    host_only = []
+    for flag in SANITIZE_FLAGS:
+        host_only += ["-Xarch_host", flag]
+    cmd = [hipcc()] + HIPCC_FLAGS + host_only + ["-I" + os.path.join(ROOT, "include")] + \
+        [os.path.join(CSRC, s) for s in SOURCES] + ["-x", "none"] + objects + ["-o", SANITIZED_LIB]
+    if verbose:
+        print(" ".join(cmd))
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed to build the sanitized library")
+    return SANITIZED_LIB
 
 
 def build(verbose: bool = False) -> str:
@@ -91,4 +117,7 @@ def build_facade_program(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    if "--sanitize" in sys.argv[1:]:
+        print(build_sanitized(verbose=True))
+    else:
+        print(build(verbose=True))

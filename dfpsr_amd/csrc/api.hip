@@ -281,7 +281,7 @@ extern "C" int dfpsr_hip_model_get_points(const dfpsr_hip_model *m, int32_t *cou
 	if (!m || !count) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_get_points: null argument");
 	if (m->pointFloats != 3) return fail(DFPSR_HIP_ERROR_ARGUMENT, "model_get_points: not a model of positions");
 	*count = m->pointCount();
-	if (xyz) std::memcpy(xyz, m->points.data(), m->points.size() * sizeof(float));
+	if (xyz && !m->points.empty()) std::memcpy(xyz, m->points.data(), m->points.size() * sizeof(float));
 	return DFPSR_HIP_OK;
 }
 
@@ -301,9 +301,11 @@ extern "C" int dfpsr_hip_model_get_part(const dfpsr_hip_model *m, int32_t part, 
 	int32_t count = 0;
 	while (first + count < (int32_t)m->polygonPart.size() && m->polygonPart[(size_t)(first + count)] == part) count++;
 	*polygons = count;
-	if (indices) std::memcpy(indices, &m->indices[(size_t)first * 4], (size_t)count * 4 * sizeof(int32_t));
-	if (colors) std::memcpy(colors, &m->colors[(size_t)first * 16], (size_t)count * 16 * sizeof(float));
-	if (texcoords) std::memcpy(texcoords, &m->texcoords[(size_t)first * 16], (size_t)count * 16 * sizeof(float));
+	if (count > 0) {  // an empty part has no polygon to point into
+		if (indices) std::memcpy(indices, m->indices.data() + (size_t)first * 4, (size_t)count * 4 * sizeof(int32_t));
+		if (colors) std::memcpy(colors, m->colors.data() + (size_t)first * 16, (size_t)count * 16 * sizeof(float));
+		if (texcoords) std::memcpy(texcoords, m->texcoords.data() + (size_t)first * 16, (size_t)count * 16 * sizeof(float));
+	}
 	if (diffuse) *diffuse = m->partDiffuseHandle[(size_t)part];
 	if (light) *light = m->partLightHandle[(size_t)part];
 	return DFPSR_HIP_OK;

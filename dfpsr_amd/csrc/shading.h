@@ -50,11 +50,13 @@ __host__ inline TextureDesc textureLayout(uint32_t log2w, uint32_t log2h, uint32
 // texture_getPixelOffsetToLayer (textureAPI.h:79-85)
 DH uint32_t layerOffset(const TextureDesc &t, uint32_t mip) { return t.startOffset & (t.maxLevelMask >> (mip << 1)); }
 
-// texture_getPixelOffset without optimization flags (textureAPI.h:104-170)
+// texture_getPixelOffset without optimization flags (textureAPI.h:104-170).  Past log2w the
+// stride wraps around as a uint32 and the shift takes it modulo 32, as the reference's scalar
+// shift (x86 shl) and v_lshlrev_b32 both do.
 DH uint32_t pixelOffset(const TextureDesc &t, uint32_t x, uint32_t y, uint32_t mip) {
 	uint32_t mx = (t.maxWidthAndMask >> mip) | t.minWidthOrMask;
 	uint32_t my = (t.maxHeightAndMask >> mip) | t.minHeightOrMask;
-	return layerOffset(t, mip) + (((y & my) << (t.log2w - mip)) | (x & mx));
+	return layerOffset(t, mip) + (((y & my) << ((t.log2w - mip) & 31u)) | (x & mx));
 }
 
 // _mm_cvttps_epi32 of one lane: truncation, 0x80000000 when out of range or NaN.

@@ -14,7 +14,8 @@ from typing import Optional, Tuple
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+# DFPSR_ORACLE_LIBRARY names another build of the oracle (the sanitized one, for instance).
+ORACLE_SO = os.environ.get("DFPSR_ORACLE_LIBRARY") or os.path.join(HERE, "_build", "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libdfpsr_ref.so")
 
 

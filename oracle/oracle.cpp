@@ -409,11 +409,13 @@ inline uint32_t pixelOffsetToLayer(const Texture &t, uint32_t mip) {  // texture
 inline uint32_t levelWidth(const Texture &t, uint32_t mip) { return 1u << (t.log2w - mip); }
 inline uint32_t levelHeight(const Texture &t, uint32_t mip) { return 1u << (t.log2h - mip); }
 
-// texture_getPixelOffset without the optimization flags (textureAPI.h:104-170)
+// texture_getPixelOffset without the optimization flags (textureAPI.h:104-170).  For a mip
+// level past log2w the stride log2w - mip wraps around as a uint32 and the reference's scalar
+// shift takes it modulo 32 (x86 shl); the shift here says so, which keeps it defined.
 inline uint32_t pixelOffset(const Texture &t, uint32_t x, uint32_t y, uint32_t mip) {
 	uint32_t mx = (t.maxWidthAndMask >> mip) | t.minWidthOrMask;
 	uint32_t my = (t.maxHeightAndMask >> mip) | t.minHeightOrMask;
-	return pixelOffsetToLayer(t, mip) + (((y & my) << (t.log2w - mip)) | (x & mx));
+	return pixelOffsetToLayer(t, mip) + (((y & my) << ((t.log2w - mip) & 31u)) | (x & mx));
 }
 
 // texture_create_RgbaU8 + the Texture constructor (textureAPI.cpp:65-78, Texture.h:66-101),
