@@ -143,7 +143,8 @@ struct dfpsr_hip_model {
 	std::vector<const TextureDesc *> partDiffuse, partLight;
 	std::vector<const dfpsr_hip_texture *> partDiffuseHandle, partLightHandle;
 	std::vector<std::string> partNames;
-	DevBuf dPoints, dIndices, dColors, dTexcoords, dFanPolygon, dPolygonPart;
+	std::vector<FanRecord> fans;  // rebuilt from indices and fanPolygon at each upload
+	DevBuf dPoints, dIndices, dColors, dTexcoords, dFans, dPolygonPart;
 	std::atomic<bool> dirty{true};
 	// Guards the upload of changed arrays: several threads may draw one model at once (the
 	// reference's model_renderDepth is const and documented for background threads,
@@ -156,7 +157,7 @@ struct dfpsr_hip_model {
 		v.indices = dIndices.as<int32_t>();
 		v.colors = dColors.as<float>();
 		v.texcoords = dTexcoords.as<float>();
-		v.fanPolygon = dFanPolygon.as<int32_t>();
+		v.fans = dFans.as<FanRecord>();
 		v.polygonPart = dPolygonPart.as<int32_t>();
 		v.pointCount = pointCount();
 		v.fanCount = (int32_t)fanPolygon.size();
@@ -171,10 +172,16 @@ struct dfpsr_hip_model {
 	int upload() {
 		std::lock_guard<std::mutex> hold(uploadLock);
 		if (!dirty) return DFPSR_HIP_OK;
+		fans.resize(fanPolygon.size());
+		for (size_t g = 0; g < fanPolygon.size(); g++) {
+			const int32_t fp = fanPolygon[g], poly = fp >> 1, k = fp & 1;
+			const int32_t *ix = &indices[(size_t)poly * 4];
+			fans[g] = FanRecord{{ix[0], ix[1 + k], ix[2 + k]}, fp};
+		}
 		struct { DevBuf *buf; const void *src; size_t bytes; } items[] = {
 			{&dPoints, points.data(), points.size() * 4}, {&dIndices, indices.data(), indices.size() * 4},
 			{&dColors, colors.data(), colors.size() * 4}, {&dTexcoords, texcoords.data(), texcoords.size() * 4},
-			{&dFanPolygon, fanPolygon.data(), fanPolygon.size() * 4}, {&dPolygonPart, polygonPart.data(), polygonPart.size() * 4}};
+			{&dFans, fans.data(), fans.size() * sizeof(FanRecord)}, {&dPolygonPart, polygonPart.data(), polygonPart.size() * 4}};
 		for (auto &it : items) {
 			if (it.buf->ensure(it.bytes) != hipSuccess) return fail(DFPSR_HIP_ERROR_MEMORY, "model upload: device allocation failed");
 			if (it.bytes) HIP_TRY(hipMemcpy(it.buf->ptr, it.src, it.bytes, hipMemcpyHostToDevice));
@@ -577,7 +584,7 @@ static int renderDepth(const dfpsr_hip_model *cm, const float transform[12], con
 	                   modelToWorld, cam, projected);
 	HIP_TRY(hipGetLastError());
 	hipLaunchKernelGGL(k_depth_draw, dim3((uint32_t)((drawThreads + 255) / 256)), dim3(256), 0, 0, projected,
-	                   m->dIndices.as<int32_t>(), m->dFanPolygon.as<int32_t>(), fanCount, groupShift, cam, cv);
+	                   m->dFans.as<FanRecord>(), fanCount, groupShift, cam, cv);
 	HIP_TRY(hipGetLastError());
 	HIP_TRY(hipDeviceSynchronize());
 	return DFPSR_HIP_OK;

@@ -60,8 +60,8 @@ __device__ void drawDepthTriangle(const Projected &A, const Projected &B, const 
 
 // renderTriangleFromDataDepth (renderCore.cpp:405-443) for fan triangle (poly, k) of each lane
 // group.  The grid holds fanCount << groupShift threads or more, in workgroups of 256.
-__global__ void __launch_bounds__(256) k_depth_draw(const Projected *projected, const int32_t *indices, const int32_t *fanPolygon,
-                                                    int32_t fanCount, int32_t groupShift, CameraParams cam, DepthCanvas cv) {
+__global__ void __launch_bounds__(256) k_depth_draw(const Projected *projected, const FanRecord *fans, int32_t fanCount,
+                                                    int32_t groupShift, CameraParams cam, DepthCanvas cv) {
 	// One clipped polygon per lane group (at most 32 groups of 8 lanes in a workgroup), in LDS
 	// rather than in each lane's private scratch memory.
 	__shared__ ClipPolygon groupPolys[256 / 8];
@@ -70,11 +70,10 @@ __global__ void __launch_bounds__(256) k_depth_draw(const Projected *projected, 
 	const int32_t lanes = 1 << groupShift;
 	const int32_t lane = thread & (lanes - 1);
 	if (g >= fanCount) return;
-	const int32_t fp = fanPolygon[g];
-	const int32_t poly = fp >> 1, k = fp & 1;
-	const Projected A = projected[indices[poly * 4]];
-	const Projected B = projected[indices[poly * 4 + 1 + k]];
-	const Projected C = projected[indices[poly * 4 + 2 + k]];
+	const FanRecord fan = fans[g];
+	const Projected A = projected[fan.corner[0]];
+	const Projected B = projected[fan.corner[1]];
+	const Projected C = projected[fan.corner[2]];
 	const bool perspective = cam.perspective != 0;
 	if (triangleVisibility(A.cs, B.cs, C.cs, cam.cull) == VIS_HIDDEN) return;
 	if (triangleVisibility(A.cs, B.cs, C.cs, cam.clip) == VIS_FULL) {

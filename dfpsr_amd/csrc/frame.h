@@ -50,7 +50,7 @@ struct ModelView {
 	const int32_t *indices;
 	const float *colors;
 	const float *texcoords;
-	const int32_t *fanPolygon;
+	const FanRecord *fans;
 	const int32_t *polygonPart;
 	int32_t pointCount, fanCount, partCount, filter;
 	int32_t maxIndex;  // the largest point index of any polygon (-1 without polygons)

@@ -111,7 +111,7 @@ int Frame::addTask(const ModelView &m, const float *transform, const CameraParam
 	t.indices = m.indices;
 	t.colors = m.colors;
 	t.texcoords = m.texcoords;
-	t.fanPolygon = m.fanPolygon;
+	t.fans = m.fans;
 	t.polygonPart = m.polygonPart;
 	t.partTexture = nullptr;  // patched when the tables are packed
 	t.pointCount = m.pointCount;

@@ -153,12 +153,11 @@ struct TriangleCorners {
 
 __device__ __forceinline__ TriangleCorners loadCorners(const FrameDev &f, int32_t g, int32_t ti) {
 	const TaskDev &t = f.tasks[ti];
-	const int32_t fp = t.fanPolygon[g - t.fanBase];
+	const FanRecord fan = t.fans[g - t.fanBase];
 	TriangleCorners tc;
-	tc.poly = fp >> 1;
-	tc.k = fp & 1;
-	const int32_t corner[3] = {0, 1 + tc.k, 2 + tc.k};
-	for (int c = 0; c < 3; c++) tc.P[c] = f.projected[t.pointBase + t.indices[tc.poly * 4 + corner[c]]];
+	tc.poly = fan.polygonFan >> 1;
+	tc.k = fan.polygonFan & 1;
+	for (int c = 0; c < 3; c++) tc.P[c] = f.projected[t.pointBase + fan.corner[c]];
 	return tc;
 }
 
@@ -195,9 +194,13 @@ __device__ __forceinline__ bool survivesOcclusion(const FrameDev &f, const Proje
 // calling wavefront's clipped polygon in LDS.  Triangles that need clipping are rare; the
 // lanes holding one take the wavefront's polygon in turn, so that the polygon (nine
 // sub-vertices indexed at run time) lives in LDS instead of private scratch memory.
+// single: the caller has counted this triangle already and found one draw command, inside the
+// clip frustum and front-facing (the tests are not repeated).  With SETUP_COUNT, *single is set
+// when that is the case.
 template <int MODE>
 __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset, BinTally &tally, ClipPolygon *clipPoly,
-                                 BinJob *jobs = nullptr, int32_t jobBase = 0, const TriangleCorners *corners = nullptr) {
+                                 BinJob *jobs = nullptr, int32_t jobBase = 0, const TriangleCorners *corners = nullptr,
+                                 bool *single = nullptr) {
 	constexpr bool EMIT = MODE == SETUP_EMIT;
 	const TaskDev &t = f.tasks[ti];
 	const CanvasDev &cv = f.canvases[t.canvas];
@@ -208,22 +211,26 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 	const int32_t corner[3] = {0, 1 + k, 2 + k};
 	const Projected *P = tc.P;
 	const CameraParams &cam = t.camera;
-	if (triangleVisibility(P[0].cs, P[1].cs, P[2].cs, cam.cull) == VIS_HIDDEN) return 0;
-	if (t.filter != 0) {
-		const float *col = t.colors + poly * 16;
-		if (almostZero(col[corner[0] * 4 + 3]) && almostZero(col[corner[1] * 4 + 3]) && almostZero(col[corner[2] * 4 + 3])) return 0;
+	const bool known = EMIT && single != nullptr && *single;
+	if (!known) {
+		if (triangleVisibility(P[0].cs, P[1].cs, P[2].cs, cam.cull) == VIS_HIDDEN) return 0;
+		if (t.filter != 0) {
+			const float *col = t.colors + poly * 16;
+			if (almostZero(col[corner[0] * 4 + 3]) && almostZero(col[corner[1] * 4 + 3]) && almostZero(col[corner[2] * 4 + 3])) return 0;
+		}
 	}
 	int32_t count = 0;
 	const bool perspective = cam.perspective != 0;
 	ShaderInputs si;
 	if (EMIT && f.mode == 0) si = triangleShader(t, poly, k);
-	if (triangleVisibility(P[0].cs, P[1].cs, P[2].cs, cam.clip) == VIS_FULL) {
-		if (frontFacing(P[0].flat, P[1].flat, P[2].flat)) {
+	if (known || triangleVisibility(P[0].cs, P[1].cs, P[2].cs, cam.clip) == VIS_FULL) {
+		if (known || frontFacing(P[0].flat, P[1].flat, P[2].flat)) {
 			if (MODE == SETUP_OCCLUDE) {
 				if (t.filter == 0) occludeFromCommand(f, P);
 				return 0;
 			}
-			if (!survivesOcclusion(f, P)) return 0;
+			if (!known && !survivesOcclusion(f, P)) return 0;
+			if (MODE == SETUP_COUNT && single != nullptr) *single = true;
 			if (EMIT) {
 				Command c;
 				const f2 is[3] = {P[0].is, P[1].is, P[2].is};
@@ -436,11 +443,12 @@ __device__ __forceinline__ void setupEmit(const FrameDev &f, int32_t sumBlocks) 
 	const int32_t ti = g < f.fanTotal ? findSegment(f.taskFanEnd, f.taskCount, g) : 0;
 	TriangleCorners corners;
 	int32_t n;
+	bool single = false;  // counted as one unclipped, front-facing command
 	if (SINGLE) {
 		n = 0;
 		if (g < f.fanTotal) {
 			BinTally unused{0, 0};
-			n = setupTriangle<SETUP_COUNT>(f, g, ti, 0, unused, &clipPolys[threadIdx.x >> 6]);
+			n = setupTriangle<SETUP_COUNT>(f, g, ti, 0, unused, &clipPolys[threadIdx.x >> 6], nullptr, 0, nullptr, &single);
 		}
 	} else {
 		n = (g < f.fanTotal) ? f.fanCount[g] : 0;
@@ -485,7 +493,8 @@ __device__ __forceinline__ void setupEmit(const FrameDev &f, int32_t sumBlocks) 
 		if (n > 0) {
 			if (SINGLE) corners = loadCorners(f, g, ti);  // the count above read them; these loads hit the caches
 			if (!SINGLE && offset + n > f.commandCapacity) atomicOr(&f.counters[2], 1);
-			else if (room) setupTriangle<SETUP_EMIT>(f, g, ti, offset, tally, &clipPolys[threadIdx.x >> 6], together ? jobs : nullptr, jobBase, &corners);
+			else if (room) setupTriangle<SETUP_EMIT>(f, g, ti, offset, tally, &clipPolys[threadIdx.x >> 6], together ? jobs : nullptr, jobBase, &corners,
+			                                         SINGLE ? &single : nullptr);
 		}
 	}
 	if (together) {
@@ -912,16 +921,16 @@ struct DiffInputs {
 };
 
 __device__ __forceinline__ DiffInputs diffInputs(const TaskDev &task, int32_t fan) {
-	int32_t fp = task.fanPolygon[fan];
-	int32_t poly = fp >> 1, k = fp & 1;
+	const FanRecord rec = task.fans[fan];
+	int32_t poly = rec.polygonFan >> 1, k = rec.polygonFan & 1;
 	const float *tc = task.texcoords + poly * 16;
 	const int32_t ca = 0, cb = (1 + k) * 4, cc = (2 + k) * 4;
 	DiffInputs d;
 	d.u = make3(tc[ca], tc[cb], tc[cc]);
 	d.v = make3(tc[ca + 1], tc[cb + 1], tc[cc + 1]);
-	d.point[0] = task.indices[poly * 4];
-	d.point[1] = task.indices[poly * 4 + 1 + k];
-	d.point[2] = task.indices[poly * 4 + 2 + k];
+	d.point[0] = rec.corner[0];
+	d.point[1] = rec.corner[1];
+	d.point[2] = rec.corner[2];
 	return d;
 }
 

@@ -21,12 +21,19 @@ constexpr int32_t TILE_SORT_CAP = 1024;
 // records of 40 bytes (renderer_giveTask_triangle as the reference takes it).
 enum { POINTS_MODEL = 0, POINTS_CAMERA_SPACE = 1, POINTS_PROJECTED = 2 };
 
+// A fan triangle of a model as the setup reads it: the point indices of its corners A, B, C
+// (polygon corners 0, 1 + k, 2 + k) and polygon * 2 + k, in one 16-byte load.
+struct __attribute__((aligned(16))) FanRecord {
+	int32_t corner[3];
+	int32_t polygonFan;
+};
+
 struct TaskDev {
 	const float *points;       // pointCount x 3
 	const int32_t *indices;    // polygons x 4
 	const float *colors;       // polygons x 16
 	const float *texcoords;    // polygons x 16
-	const int32_t *fanPolygon; // fan triangle -> polygon * 2 + k
+	const FanRecord *fans;     // per fan triangle
 	const int32_t *polygonPart;
 	const int32_t *partTexture;  // frame texture slot per part: diffuse, light (-1 = none)
 	int32_t pointCount, fanCount;
@@ -174,7 +181,7 @@ struct DepthCanvas {
 // Projects a model's points; then 2^groupShift lanes (3 <= groupShift <= 6) per fan triangle,
 // in workgroups of 256 threads.
 __global__ void k_depth_project(const float *points, int32_t n, Transform modelToWorld, CameraParams cam, Projected *out);
-__global__ void k_depth_draw(const Projected *projected, const int32_t *indices, const int32_t *fanPolygon, int32_t fanCount,
+__global__ void k_depth_draw(const Projected *projected, const FanRecord *fans, int32_t fanCount,
                              int32_t groupShift, CameraParams cam, DepthCanvas cv);
 
 constexpr int32_t SCAN_BLOCK = 256;
