@@ -47,6 +47,7 @@ struct BinTally { int32_t pairs, tiles; };
 struct BinJob { int32_t index, firstTile, width, tilesX, pairStart; };
 constexpr int32_t BIN_JOBS = 512;  // commands a setup block bins together (more: each thread bins its own)
 constexpr int32_t TILE_TABLE_LOG2 = 10, TILE_TABLE = 1 << TILE_TABLE_LOG2, TILE_TABLE_PROBES = 8;
+static_assert(TILE_TABLE % SCAN_BLOCK == 0, "the tile table's entries split evenly over a setup block");
 
 // Enters pair slot[k] of tile[k] into the tile's list (direct bins).
 __device__ __forceinline__ void binPair(const FrameDev &f, int32_t tile, int32_t slot, int32_t index, BinTally &tally) {
@@ -545,8 +546,19 @@ __device__ __forceinline__ void setupEmit(const FrameDev &f, int32_t sumBlocks) 
 				}
 			}
 			__syncthreads();
-			for (int32_t e = threadIdx.x; e < TILE_TABLE; e += SCAN_BLOCK)
-				if (tableTile[e] >= 0) tableCount[e] = atomicAdd(&f.tileCount[tableTile[e]], tableCount[e]);
+			{
+				// Every entry's global atomic is issued before any of the returned bases is used.
+				constexpr int32_t PER = TILE_TABLE / SCAN_BLOCK;
+				int32_t entryTile[PER], entryBase[PER];
+#pragma unroll
+				for (int q = 0; q < PER; q++) entryTile[q] = tableTile[(int32_t)threadIdx.x + q * SCAN_BLOCK];
+#pragma unroll
+				for (int q = 0; q < PER; q++)
+					entryBase[q] = entryTile[q] >= 0 ? atomicAdd(&f.tileCount[entryTile[q]], tableCount[(int32_t)threadIdx.x + q * SCAN_BLOCK]) : 0;
+#pragma unroll
+				for (int q = 0; q < PER; q++)
+					if (entryTile[q] >= 0) tableCount[(int32_t)threadIdx.x + q * SCAN_BLOCK] = entryBase[q];
+			}
 			__syncthreads();
 #pragma unroll
 			for (int u = 0; u < UNROLL; u++) {
