@@ -806,6 +806,43 @@ __device__ __forceinline__ bool evalQuad(const Command &cmd, RowInterval upper, 
 	return evalQuad(Projection{cmd.start, cmd.dx, cmd.dy}, upper, lower, px, py, canvasHeight, e);
 }
 
+// The integer part of evalQuad: whether the command touches the quad at (px, py) and, per lane,
+// the pixel it writes and whether the row's sides keep it -- without the weight chains.
+struct QuadLanes {
+	int slot[4];
+	bool clip[4];
+	bool clipSides;
+};
+
+__device__ __forceinline__ bool quadLanes(RowInterval upper, RowInterval lower, int32_t px, int32_t py, int32_t canvasHeight,
+                                          QuadLanes &q) {
+	const int32_t outerStart = tmin(upper.left, lower.left);
+	const int32_t outerEnd = tmax(upper.right, lower.right);
+	const int32_t innerStart = tmax(upper.left, lower.left);
+	const int32_t innerEnd = tmin(upper.right, lower.right);
+	const int32_t outerBlockStart = (int32_t)((uint32_t)outerStart & ~1u);
+	const int32_t outerBlockEnd = (int32_t)(((uint32_t)outerEnd + 1u) & ~1u);
+	const int32_t innerBlockStart = (int32_t)(((uint32_t)innerStart + 1u) & ~1u);
+	const int32_t innerBlockEnd = (int32_t)((uint32_t)innerEnd & ~1u);
+	if (py + 1 >= canvasHeight) lower.right = lower.left;
+	const bool hasTop = upper.right > upper.left;
+	const bool hasBottom = lower.right > lower.left;
+	if (!(hasTop || hasBottom)) return false;
+	if (px < outerBlockStart || px >= outerBlockEnd) return false;
+	const bool inner = innerBlockEnd > innerBlockStart && px >= innerBlockStart && px < innerBlockEnd;
+	q.clipSides = !inner;
+	q.slot[0] = hasTop ? 0 : 2;
+	q.slot[1] = hasTop ? 1 : 3;
+	q.slot[2] = hasBottom ? 2 : 0;
+	q.slot[3] = hasBottom ? 3 : 1;
+	const int32_t x2 = px + 1;
+	q.clip[0] = px >= upper.left && px < upper.right;
+	q.clip[1] = x2 >= upper.left && x2 < upper.right;
+	q.clip[2] = px >= lower.left && px < lower.right;
+	q.clip[3] = x2 >= lower.left && x2 < lower.right;
+	return true;
+}
+
 // Vertex weights of a lane (fillRowSuper, fillerTemplates.h:222-238): perspective divides the
 // depth-divided weights by 1/W.
 __device__ __forceinline__ void laneWeights(const QuadEval &e, int l, bool affine, float &wa, float &wb, float &wc) {
@@ -1304,6 +1341,7 @@ template <int TW, int TH> struct DeferLds {
 	uint32_t color[TW * TH];                     // the tile's canvas pixels as they stand
 	float depth[TW * TH];
 	uint32_t dirty[TW * TH / 32];                // pixels written, one bit each
+	uint32_t chunkWon[TW * TH / 32];             // pixels the current chunk's commands won
 	int32_t scan[S::NT / 64];                    // wavefront totals of the prefix sums
 	TextureDesc textures[LDS_TEXTURES];
 	int32_t roundInfo[4];
@@ -1362,6 +1400,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	constexpr int32_t PAIRS = DeferLds<TW, TH>::PAIRS;
 	constexpr int32_t ITEM_TABLE = DeferLds<TW, TH>::ITEM_TABLE;
 	constexpr int32_t PAIRS_PER_THREAD = PAIRS / NT > 0 ? PAIRS / NT : 1;
+	static_assert(PAIRS * (TW / 2) <= 32 * NT, "a thread's items fit the 32 four-bit masks of pass 2a");
 	const int32_t n = f.tileCount[tile];
 	const int32_t ci = findSegment(f.canvasTileEnd, f.canvasCount, tile);
 	const CanvasDev cv = f.canvases[ci];
@@ -1431,6 +1470,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				const float d = s.depth[p];
 				s.keys[p] = !depthRead ? 0ull : (d != d ? ~0ull : (((unsigned long long)depthOrder(d) << 32) | 0xFFFFFFFFull));
 			}
+			for (int32_t w = t; w < NP / 32; w += NT) s.chunkWon[w] = 0u;
 			// Quads of each (command, row pair) inside the tile: the outer span of the row pair
 			// (fillShapeSuper), cut to the tile's columns.
 			int32_t pairCount[PAIRS_PER_THREAD];
@@ -1487,53 +1527,106 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				}
 			}
 			__syncthreads();
-			// Pass 2: the same items, evaluated again (keeping pass 1's evaluations would cost more
-			// registers than the evaluation); each writes the lanes whose pixels its command won,
-			// with the quad's weights and mip level as in the in-order walk.  A pixel has one
-			// winner, so exactly one item writes it.
-			for (int32_t it = t; it < items; it += NT) {
-				const int32_t pr = it < ITEM_TABLE ? (int32_t)s.pairOfItem[it] : segmentOf(s.pairOffset, PAIRS, it);
-				const int32_t j = pr / RP, rp = pr % RP;
-				const int32_t qx = s.pairStart[pr] + 2 * (it - s.pairOffset[pr]), qy = 2 * rp;
-				QuadEval e;
-				if (!evalQuad(s.tail[j].plane, rowInterval(s.rows[j * TH + qy]), rowInterval(s.rows[j * TH + qy + 1]), tileX0 + qx,
-				              tileY0 + qy, cv.height, e))
-					continue;
-				uint32_t mask = 0;
-				for (int l = 0; l < 4; l++) {
-					if (e.clipSides && !e.clip[l]) continue;
-					const int32_t sl = e.slot[l];
-					if (keyWinner(s.keys[(qy + (sl >> 1)) * TW + qx + (sl & 1)], depthRead) == j) mask |= 1u << l;
+			if (MODE == 0) {
+				// Pass 2a: the lanes each item's command won, read from the keys before pass 2b
+				// overwrites them; four bits per item of this thread (at most 32 items).
+				uint64_t wonLow = 0, wonHigh = 0;
+				for (int32_t it = t, k = 0; it < items; it += NT, k++) {
+					const int32_t pr = it < ITEM_TABLE ? (int32_t)s.pairOfItem[it] : segmentOf(s.pairOffset, PAIRS, it);
+					const int32_t j = pr / RP, rp = pr % RP;
+					const int32_t qx = s.pairStart[pr] + 2 * (it - s.pairOffset[pr]), qy = 2 * rp;
+					QuadLanes q;
+					uint64_t mask = 0;
+					if (quadLanes(rowInterval(s.rows[j * TH + qy]), rowInterval(s.rows[j * TH + qy + 1]), tileX0 + qx, tileY0 + qy, cv.height, q)) {
+						for (int l = 0; l < 4; l++) {
+							if (q.clipSides && !q.clip[l]) continue;
+							const int32_t sl = q.slot[l];
+							if (keyWinner(s.keys[(qy + (sl >> 1)) * TW + qx + (sl & 1)], depthRead) == j) mask |= 1ull << l;
+						}
+					}
+					if (k < 16) wonLow |= mask << (4 * k);
+					else wonHigh |= mask << (4 * (k - 16));
 				}
-				if (!mask) continue;
-				// Depths first, so that only the weights and the won lanes' pixels outlive the quad
-				// evaluation while the colours are made.
-				uint32_t slots = 0;
-				for (int l = 0; l < 4; l++) {
-					if (!((mask >> l) & 1u)) continue;
-					const int32_t sl = e.slot[l];
-					const int32_t p = (qy + (sl >> 1)) * TW + qx + (sl & 1);
-					s.depth[p] = e.l0[l];
-					if (!f.clearInRaster) atomicOr(&s.dirty[p >> 5], 1u << (p & 31));
-					slots |= (uint32_t)sl << (2 * l);
+				__syncthreads();
+				// Pass 2b: each item that won a lane evaluates its quad again (keeping pass 1's
+				// evaluations would cost more registers than the evaluation), writes the won lanes'
+				// depths, and leaves for every won pixel its weights (in the pixel's key, no longer
+				// read) and its command and the quad's mip level (in its colour, about to be
+				// replaced), so that pass 2c shades each pixel once with all threads busy.
+				for (int32_t it = t, k = 0; it < items; it += NT, k++) {
+					const uint32_t mask = (uint32_t)((k < 16 ? wonLow >> (4 * k) : wonHigh >> (4 * (k - 16))) & 0xFu);
+					if (!mask) continue;
+					const int32_t pr = it < ITEM_TABLE ? (int32_t)s.pairOfItem[it] : segmentOf(s.pairOffset, PAIRS, it);
+					const int32_t j = pr / RP, rp = pr % RP;
+					const int32_t qx = s.pairStart[pr] + 2 * (it - s.pairOffset[pr]), qy = 2 * rp;
+					QuadEval e;
+					if (!evalQuad(s.tail[j].plane, rowInterval(s.rows[j * TH + qy]), rowInterval(s.rows[j * TH + qy + 1]), tileX0 + qx,
+					              tileY0 + qy, cv.height, e))
+						continue;
+					int32_t pixel[4];
+					for (int l = 0; l < 4; l++) {
+						const int32_t sl = e.slot[l];
+						pixel[l] = (qy + (sl >> 1)) * TW + qx + (sl & 1);
+						if (!((mask >> l) & 1u)) continue;
+						s.depth[pixel[l]] = e.l0[l];
+						if (!f.clearInRaster) atomicOr(&s.dirty[pixel[l] >> 5], 1u << (pixel[l] & 31));
+					}
+					if (colorWrite) {
+						const bool affine = (s.tail[j].flags & 1) != 0;
+						float wa[4], wb[4], wc[4];
+						for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
+						const uint32_t mip = quadMip(s.si[j], textures, wa, wb, wc);
+						for (int l = 0; l < 4; l++) {
+							if (!((mask >> l) & 1u)) continue;
+							s.keys[pixel[l]] = ((unsigned long long)__float_as_uint(wc[l]) << 32) | __float_as_uint(wb[l]);
+							s.color[pixel[l]] = (uint32_t)j | (mip << 8);
+							atomicOr(&s.chunkWon[pixel[l] >> 5], 1u << (pixel[l] & 31));
+						}
+					}
 				}
-				if (MODE == 1) {
-					const int32_t id = (s.list[c0 + j] - f.canvasCommandBase[ci]) * 4;
-					shadeDiffLanes(f, s.tail[j], e, mask, slots, id, cv, tileX0 + qx, tileY0 + qy);
-				}
+				__syncthreads();
+				// Pass 2c: every pixel won in this chunk, shaded once.
 				if (colorWrite) {
-					const bool affine = (s.tail[j].flags & 1) != 0;
-					float wa[4], wb[4], wc[4];
-					for (int l = 0; l < 4; l++) laneWeights(e, l, affine, wa[l], wb[l], wc[l]);
-					// All four lanes are shaded (every texel index is masked into its level), so the
-					// texel loads of a lane pair can be issued together; only won lanes are kept.
-					uint32_t packed[4];
-					shadeQuadPacked(s.si[j], textures, wa, wb, wc, packed);
+					for (int32_t p = t; p < NP; p += NT) {
+						if (!((s.chunkWon[p >> 5] >> (p & 31)) & 1u)) continue;
+						const uint32_t meta = s.color[p];
+						const unsigned long long weights = s.keys[p];
+						const float wb = __uint_as_float((uint32_t)weights), wc = __uint_as_float((uint32_t)(weights >> 32));
+						s.color[p] = shadePixel(s.si[meta & 0xFFu], textures, 1.0f - (wb + wc), wb, wc, meta >> 8);
+					}
+				}
+			} else {
+				// Pass 2 of the differentiable shader: the same items, evaluated again; each writes
+				// the lanes whose pixels its command won, with the quad's weights as in the in-order
+				// walk.  A pixel has one winner, so exactly one item writes it.
+				for (int32_t it = t; it < items; it += NT) {
+					const int32_t pr = it < ITEM_TABLE ? (int32_t)s.pairOfItem[it] : segmentOf(s.pairOffset, PAIRS, it);
+					const int32_t j = pr / RP, rp = pr % RP;
+					const int32_t qx = s.pairStart[pr] + 2 * (it - s.pairOffset[pr]), qy = 2 * rp;
+					QuadEval e;
+					if (!evalQuad(s.tail[j].plane, rowInterval(s.rows[j * TH + qy]), rowInterval(s.rows[j * TH + qy + 1]), tileX0 + qx,
+					              tileY0 + qy, cv.height, e))
+						continue;
+					uint32_t mask = 0;
+					for (int l = 0; l < 4; l++) {
+						if (e.clipSides && !e.clip[l]) continue;
+						const int32_t sl = e.slot[l];
+						if (keyWinner(s.keys[(qy + (sl >> 1)) * TW + qx + (sl & 1)], depthRead) == j) mask |= 1u << l;
+					}
+					if (!mask) continue;
+					// Depths first, so that only the weights and the won lanes' pixels outlive the quad
+					// evaluation while the colours are made.
+					uint32_t slots = 0;
 					for (int l = 0; l < 4; l++) {
 						if (!((mask >> l) & 1u)) continue;
-						const int32_t sl = (int32_t)((slots >> (2 * l)) & 3u);
-						s.color[(qy + (sl >> 1)) * TW + qx + (sl & 1)] = packed[l];
+						const int32_t sl = e.slot[l];
+						const int32_t p = (qy + (sl >> 1)) * TW + qx + (sl & 1);
+						s.depth[p] = e.l0[l];
+						if (!f.clearInRaster) atomicOr(&s.dirty[p >> 5], 1u << (p & 31));
+						slots |= (uint32_t)sl << (2 * l);
 					}
+					const int32_t id = (s.list[c0 + j] - f.canvasCommandBase[ci]) * 4;
+					shadeDiffLanes(f, s.tail[j], e, mask, slots, id, cv, tileX0 + qx, tileY0 + qy);
 				}
 			}
 			__syncthreads();

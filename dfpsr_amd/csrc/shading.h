@@ -305,66 +305,51 @@ DH void shadeQuad(const ShaderInputs &s, const TextureDesc *textures, const floa
 	}
 }
 
-// shadeQuad for all four lanes followed by packSaturated, two lanes at a time so that fewer
-// values are live at once (the texel loads of a lane pair are issued together).  Each lane's
-// arithmetic is shadeQuad's; the mip level still comes from lanes 0, 1 and 2.
-__device__ __forceinline__ void shadeQuadPacked(const ShaderInputs &s, const TextureDesc *textures, const float *wa, const float *wb,
-                                                const float *wc, uint32_t *out) {
+// The mip level a quad samples its diffuse texture at (0 without one): texture_getMipLevelIndex
+// of lanes 0, 1 and 2's texture coordinates, as shadeQuad takes it.
+__device__ __forceinline__ uint32_t quadMip(const ShaderInputs &s, const TextureDesc *textures, const float *wa, const float *wb,
+                                            const float *wc) {
+	if (s.diffuse < 0) return 0u;
+	float u[3], v[3];
+	for (int l = 0; l < 3; l++) { u[l] = interp(s.u1, wa[l], wb[l], wc[l]); v[l] = interp(s.v1, wa[l], wb[l], wc[l]); }
+	return mipLevelIndex(textures[s.diffuse], u[0], u[1], u[2], v[0], v[1], v[2]);
+}
+
+// One lane of shadeQuad followed by packSaturated, with the quad's mip level given: the same
+// operations in the same order for one pixel.
+__device__ __forceinline__ uint32_t shadePixel(const ShaderInputs &s, const TextureDesc *textures, float wa, float wb, float wc,
+                                               uint32_t mip) {
 	const bool textureOnly = s.colorless && ((s.diffuse >= 0) != (s.light >= 0));
-	uint32_t mip = 0;
+	float r = 0.0f, g = 0.0f, b = 0.0f, a = 0.0f;
+	if (!textureOnly) {
+		if (s.fading) {
+			r = interp(s.red, wa, wb, wc);
+			g = interp(s.green, wa, wb, wc);
+			b = interp(s.blue, wa, wb, wc);
+			a = interp(s.alpha, wa, wb, wc);
+		} else {
+			r = s.red.x; g = s.green.x; b = s.blue.x; a = s.alpha.x;
+		}
+	}
 	if (s.diffuse >= 0) {
-		float u[3], v[3];
-		for (int l = 0; l < 3; l++) { u[l] = interp(s.u1, wa[l], wb[l], wc[l]); v[l] = interp(s.v1, wa[l], wb[l], wc[l]); }
-		mip = mipLevelIndex(textures[s.diffuse], u[0], u[1], u[2], v[0], v[1], v[2]);
+		float sr, sg, sb, sa;
+		unpackColor(sampleBilinear(textures[s.diffuse], interp(s.u1, wa, wb, wc), interp(s.v1, wa, wb, wc), mip, false), sr, sg, sb, sa);
+		if (textureOnly) {
+			r = sr; g = sg; b = sb; a = sa;
+		} else {
+			r = r * sr; g = g * sg; b = b * sb; a = a * sa;
+		}
 	}
-	for (int h = 0; h < 4; h += 2) {
-		float r[2], g[2], b[2], a[2];
-		if (!textureOnly) {
-			for (int l = 0; l < 2; l++) {
-				const float la = wa[h + l], lb = wb[h + l], lc = wc[h + l];
-				if (s.fading) {
-					r[l] = interp(s.red, la, lb, lc);
-					g[l] = interp(s.green, la, lb, lc);
-					b[l] = interp(s.blue, la, lb, lc);
-					a[l] = interp(s.alpha, la, lb, lc);
-				} else {
-					r[l] = s.red.x; g[l] = s.green.x; b[l] = s.blue.x; a[l] = s.alpha.x;
-				}
-			}
+	if (s.light >= 0) {
+		float sr, sg, sb, sa;
+		unpackColor(sampleBilinear(textures[s.light], interp(s.u2, wa, wb, wc), interp(s.v2, wa, wb, wc), 0u, true), sr, sg, sb, sa);
+		if (textureOnly) {
+			r = sr; g = sg; b = sb; a = sa;
+		} else {
+			r = r * sr; g = g * sg; b = b * sb; a = a * sa;
 		}
-		if (s.diffuse >= 0) {
-			const TextureDesc &t = textures[s.diffuse];
-			BilinearSample bs[2];
-			for (int l = 0; l < 2; l++)
-				bs[l] = bilinearSample(t, interp(s.u1, wa[h + l], wb[h + l], wc[h + l]), interp(s.v1, wa[h + l], wb[h + l], wc[h + l]), mip, false);
-			uint32_t px[8];
-			for (int l = 0; l < 2; l++)
-				for (int k = 0; k < 4; k++) px[l * 4 + k] = loadTexel(t.texels, bs[l].offset[k]);
-			for (int l = 0; l < 2; l++) {
-				float sr, sg, sb, sa;
-				unpackColor(mixSample(bs[l], px[l * 4], px[l * 4 + 1], px[l * 4 + 2], px[l * 4 + 3]), sr, sg, sb, sa);
-				if (textureOnly) {
-					r[l] = sr; g[l] = sg; b[l] = sb; a[l] = sa;
-				} else {
-					r[l] = r[l] * sr; g[l] = g[l] * sg; b[l] = b[l] * sb; a[l] = a[l] * sa;
-				}
-			}
-		}
-		if (s.light >= 0) {
-			const TextureDesc &t = textures[s.light];
-			for (int l = 0; l < 2; l++) {
-				float sr, sg, sb, sa;
-				unpackColor(sampleBilinear(t, interp(s.u2, wa[h + l], wb[h + l], wc[h + l]), interp(s.v2, wa[h + l], wb[h + l], wc[h + l]), 0u, true),
-				            sr, sg, sb, sa);
-				if (textureOnly) {
-					r[l] = sr; g[l] = sg; b[l] = sb; a[l] = sa;
-				} else {
-					r[l] = r[l] * sr; g[l] = g[l] * sg; b[l] = b[l] * sb; a[l] = a[l] * sa;
-				}
-			}
-		}
-		for (int l = 0; l < 2; l++) out[h + l] = packSaturated(r[l], g[l], b[l], a[l]);
 	}
+	return packSaturated(r, g, b, a);
 }
 
 // Lane l of shadeQuad: the same operations for one pixel of the quad.  The mip level still
