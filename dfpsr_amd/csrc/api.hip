@@ -37,6 +37,7 @@ hipError_t DevBuf::ensure(size_t need) {
 	size_t n = need < 256 ? 256 : need;
 	hipError_t e = hipMalloc(&ptr, n);
 	if (e == hipSuccess) bytes = n; else ptr = nullptr;
+	generation++;
 	return e;
 }
 

@@ -23,6 +23,7 @@ int fail(int code, const std::string &message);
 struct DevBuf {
 	void *ptr = nullptr;
 	size_t bytes = 0;
+	uint64_t generation = 0;  // counts the allocations: a new one holds nothing of the old
 	DevBuf() = default;
 	DevBuf(const DevBuf &) = delete;
 	DevBuf &operator=(const DevBuf &) = delete;
@@ -154,7 +155,7 @@ private:
 	size_t offTasks_ = 0, offFanEnd_ = 0, offPointEnd_ = 0, offCanvases_ = 0, offTileEnd_ = 0, offTextures_ = 0;
 	bool prepared_ = false;
 	DevBuf dShaders_, dTriangleRows_;
-	DevBuf dCanvasBase_, dProjected_, dFanCount_, dBlockSums_, dCommands_, dCounters_, dTileCount_, dTileOffset_, dTileFill_, dTileList_, dTileQueue_, dPairRecords_;
+	DevBuf dCanvasBase_, dFanCount_, dBlockSums_, dCommands_, dCounters_, dTileCount_, dTileOffset_, dTileFill_, dTileList_, dTileQueue_, dPairRecords_;
 	int32_t *hostCounters_ = nullptr;
 	int32_t *hostCountersDevice_ = nullptr;
 	hipEvent_t events_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -177,7 +178,14 @@ private:
 
 	int32_t textureSlot(const TextureDesc *desc);
 	// parts: 1 = everything before the raster, 2 = the raster, 3 = both.
-	int enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks, bool events, int parts);
+	int enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks, bool events, int parts,
+	            bool skipBegin = false);
+	// The last run left its counts zeroed (a self-cleaning raster that finished), for buffers of
+	// these generations, this many tiles and canvases.
+	bool clean_ = false;
+	int32_t parity_ = 0;  // the set of counters and first-command indices the next frame uses
+	uint64_t cleanGenerations_[3] = {0, 0, 0};
+	int32_t cleanTiles_ = 0, cleanCanvases_ = 0;
 	void dropGraph();
 	// The visibility raster reproduces the in-order result when every command is Solid and
 	// perspective and no canvas has an odd height.

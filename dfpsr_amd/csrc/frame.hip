@@ -139,16 +139,16 @@ int Frame::addTask(const ModelView &m, const float *transform, const CameraParam
 static inline int32_t blocksOf(int64_t n, int32_t size) { return (int32_t)((n + size - 1) / size); }
 static inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks, bool events, int parts) {
+int Frame::enqueue(const FrameDev &f, const RenderOptions &o, int32_t fanBlocks, int32_t tileBlocks, bool events, int parts,
+                   bool skipBegin) {
 	const int32_t tileTotal = f.tileTotal;
 	if (parts & 1) {
 	if (o.clear && !f.clearInRaster) hipLaunchKernelGGL(k_clear_canvases, dim3(tileTotal), dim3(256), 0, stream, f, o.clearColor, o.clearDepth);
 	if (events) HIP_TRY(hipEventRecord(events_[0], stream));
 	int64_t beginThreads = 2 * (int64_t)tileTotal + 1;
-	if (beginThreads < f.pointTotal) beginThreads = f.pointTotal;
 	if (beginThreads < f.canvasCount) beginThreads = f.canvasCount;
 	if (beginThreads < COUNTERS) beginThreads = COUNTERS;
-	hipLaunchKernelGGL(k_frame_begin, dim3(blocksOf(beginThreads, 256)), dim3(256), 0, stream, f);
+	if (!skipBegin) hipLaunchKernelGGL(k_frame_begin, dim3(blocksOf(beginThreads, 256)), dim3(256), 0, stream, f);
 	if (events) HIP_TRY(hipEventRecord(events_[1], stream));
 	if (f.blockCapacity > 0) {
 		hipLaunchKernelGGL(k_setup_single, dim3(fanBlocks), dim3(SCAN_BLOCK), 0, stream, f);
@@ -276,12 +276,11 @@ int Frame::run(const RenderOptions &o) {
 	const int32_t tileBlocks = blocksOf(tileTotal, SCAN_BLOCK);
 	if (commandCapacity_ < fanTotal + 1024) commandCapacity_ = fanTotal + 1024;
 	if (pairCapacity_ < 4 * commandCapacity_) pairCapacity_ = 4 * commandCapacity_;
-	if (dProjected_.ensure((size_t)pointTotal * sizeof(Projected)) != hipSuccess ||
-	    dFanCount_.ensure((size_t)fanTotal * 4) != hipSuccess ||
+	if (dFanCount_.ensure((size_t)fanTotal * 4) != hipSuccess ||
 	    dBlockSums_.ensure((size_t)(fanBlocks > tileBlocks ? fanBlocks : tileBlocks) * 8) != hipSuccess ||
-	    dCounters_.ensure(COUNTERS * 4) != hipSuccess || dTileCount_.ensure((2 * (size_t)tileTotal + 1) * 4) != hipSuccess ||
+	    dCounters_.ensure(2 * COUNTERS * 4) != hipSuccess || dTileCount_.ensure((2 * (size_t)tileTotal + 1) * 4) != hipSuccess ||
 	    dTileOffset_.ensure(((size_t)tileTotal + 1) * 4) != hipSuccess || dTileQueue_.ensure((size_t)tileTotal * 4) != hipSuccess ||
-	    dCanvasBase_.ensure(nC * 4) != hipSuccess)
+	    dCanvasBase_.ensure(2 * nC * 4) != hipSuccess)
 		return fail(-4, "device allocation failed");
 
 	for (int pass = 1; pass <= 3; pass++) {
@@ -310,22 +309,22 @@ int Frame::run(const RenderOptions &o) {
 		f.tileTotal = tileTotal;
 		f.textures = (const TextureDesc *)(base + offTextures_);
 		f.textureCount = (int32_t)textures_.size();
-		f.projected = dProjected_.as<Projected>();
 		f.fanCount = dFanCount_.as<int32_t>();
 		f.blockSums = dBlockSums_.as<int32_t>();
 		f.commands = dCommands_.as<Command>();
 		f.shaders = dShaders_.as<ShaderInputs>();
 		f.triangleRows = dTriangleRows_.as<TriangleRows>();
 		f.commandCapacity = commandCapacity_;
-		f.counters = dCounters_.as<int32_t>();
-		f.queueHead = dCounters_.as<int32_t>() + 4;
+		// Two sets of counters and first-command indices, used by alternate self-cleaning frames.
+		f.counters = dCounters_.as<int32_t>() + parity_ * COUNTERS;
+		f.queueHead = f.counters + 4;
 		f.hostCounters = hostCountersDevice_;
 		f.tileCount = dTileCount_.as<int32_t>();
 		f.tileFill = dTileCount_.as<int32_t>() + tileTotal + 1;
 		f.tileOffset = dTileOffset_.as<int32_t>();
 		f.tileList = dTileList_.as<int32_t>();
 		f.tileQueue = dTileQueue_.as<int32_t>();
-		f.canvasCommandBase = dCanvasBase_.as<int32_t>();
+		f.canvasCommandBase = dCanvasBase_.as<int32_t>() + parity_ * (int32_t)nC;
 		f.pairCapacity = pairCapacity_;
 		f.pairRecords = direct ? nullptr : dPairRecords_.as<PairRecord>();
 		f.binCapacity = direct ? binCapacity_ : 0;
@@ -341,6 +340,19 @@ int Frame::run(const RenderOptions &o) {
 		f.ftexLog2h = ftexLog2h_;
 		f.multipliers = multipliers_;
 		f.blockCapacity = single ? blockCapacity_ : 0;
+		// The raster leaves the counts zeroed for the next frame on the common path: one pass of
+		// setup, direct bins, one workgroup per tile, no graph.  A frame may then skip the begin
+		// kernel when the frame before it cleaned the same buffers and it uses no more tiles or
+		// canvases than that frame did.
+		const bool selfClean = !o.graph && single && direct && o.persistentPerCU <= 0 &&
+		                       (o.mode == 2 || (o.mode == 0 && deferredRaster(o)));
+		const uint64_t gens[3] = {dCounters_.generation, dTileCount_.generation, dCanvasBase_.generation};
+		const bool skipBegin = selfClean && clean_ && std::equal(gens, gens + 3, cleanGenerations_) &&
+		                       tileTotal <= cleanTiles_ && (int32_t)nC == cleanCanvases_;
+		f.selfClean = selfClean ? 1 : 0;
+		f.nextCounters = selfClean ? dCounters_.as<int32_t>() + (1 - parity_) * COUNTERS : nullptr;
+		f.nextCanvasBase = selfClean ? dCanvasBase_.as<int32_t>() + (1 - parity_) * (int32_t)nC : nullptr;
+		clean_ = false;  // this frame uses the counts; they are clean again only if its raster says so
 		f.occlusionGrid = occlusionGrid_;
 		f.gridW = gridW_;
 		f.gridH = gridH_;
@@ -390,7 +402,7 @@ int Frame::run(const RenderOptions &o) {
 				HIP_TRY(hipGraphLaunch(graphExec_, stream));
 			}
 		} else {
-			int rc = enqueue(f, o, fanBlocks, tileBlocks, timing, 3);
+			int rc = enqueue(f, o, fanBlocks, tileBlocks, timing, 3, skipBegin);
 			if (rc) return rc;
 		}
 		HIP_TRY(hipStreamSynchronize(stream));
@@ -413,6 +425,13 @@ int Frame::run(const RenderOptions &o) {
 		int32_t overflow = hostCounters_[2];
 		if (overflow == 0) {
 			lastRunValid_ = true;
+			if (selfClean) {
+				clean_ = true;  // the other set is reset and every tile count zeroed: the next frame uses that set
+				parity_ = 1 - parity_;
+				std::copy(gens, gens + 3, cleanGenerations_);
+				cleanTiles_ = tileTotal;
+				cleanCanvases_ = (int32_t)nC;
+			}
 			return 0;
 		}
 		if (overflow & 1) {
@@ -447,9 +466,6 @@ int Frame::occludeExisting(float *deviceGrid, int32_t gridW, int32_t gridH) {
 	const int32_t tileTotal = canvasTileEnd_.back();
 	if (fanTotal == 0) return 0;
 	const size_t nC = canvases_.size();
-	if (dProjected_.ensure((size_t)pointTotal * sizeof(Projected)) != hipSuccess || dCounters_.ensure(COUNTERS * 4) != hipSuccess ||
-	    dTileCount_.ensure((2 * (size_t)tileTotal + 1) * 4) != hipSuccess || dCanvasBase_.ensure(nC * 4) != hipSuccess)
-		return fail(-4, "device allocation failed");
 	uint8_t *base = (uint8_t *)dTables_.ptr;
 	FrameDev f;
 	std::memset(&f, 0, sizeof(f));
@@ -465,18 +481,11 @@ int Frame::occludeExisting(float *deviceGrid, int32_t gridW, int32_t gridH) {
 	f.tileTotal = tileTotal;
 	f.textures = (const TextureDesc *)(base + offTextures_);
 	f.textureCount = (int32_t)textures_.size();
-	f.projected = dProjected_.as<Projected>();
-	f.counters = dCounters_.as<int32_t>();
-	f.tileCount = dTileCount_.as<int32_t>();
-	f.canvasCommandBase = dCanvasBase_.as<int32_t>();
 	f.occlusionGrid = deviceGrid;
 	f.gridW = gridW;
 	f.gridH = gridH;
-	int64_t beginThreads = 2 * (int64_t)tileTotal + 1;
-	if (beginThreads < pointTotal) beginThreads = pointTotal;
-	if (beginThreads < (int64_t)nC) beginThreads = (int64_t)nC;
-	if (beginThreads < COUNTERS) beginThreads = COUNTERS;
-	hipLaunchKernelGGL(k_frame_begin, dim3(blocksOf(beginThreads, 256)), dim3(256), 0, stream, f);
+	// The setup's occlusion mode projects each fan triangle's corners itself and touches no
+	// counters, so nothing runs before it.
 	hipLaunchKernelGGL(k_occlude_existing, dim3(blocksOf(fanTotal, 256)), dim3(256), 0, stream, f);
 	HIP_TRY(hipGetLastError());
 	HIP_TRY(hipStreamSynchronize(stream));

@@ -13,23 +13,16 @@
 namespace dfpsr {
 
 // ---------------------------------------------------------------------------------------
-// Projection of every model point once per task (modelAPI.cpp:237-242).
+// Projection of a task's point (modelAPI.cpp:237-242): a model point transformed to the world,
+// to the camera and onto the screen; a camera-space corner projected; or the caller's
+// ProjectedPoint as it is.  The setup projects the corners of each fan triangle it reads, so
+// no pass over the points comes before it.
 // ---------------------------------------------------------------------------------------
-__device__ void projectPoint(const FrameDev &f, int32_t i) {
-	int32_t ti = findSegment(f.taskPointEnd, f.taskCount, i);
-	const TaskDev &t = f.tasks[ti];
-	int32_t local = i - t.pointBase;
-	if (t.pointKind == POINTS_PROJECTED) {
-		f.projected[i] = reinterpret_cast<const Projected *>(t.points)[local];
-		return;
-	}
-	f3 p = make3(t.points[local * 3], t.points[local * 3 + 1], t.points[local * 3 + 2]);
-	if (t.pointKind == POINTS_CAMERA_SPACE) {
-		f.projected[i] = cameraToScreen(t.camera, p);
-	} else {
-		Transform m = transformFrom(t.transform);
-		f.projected[i] = cameraToScreen(t.camera, worldToCamera(t.camera.location, transformPoint(m, p)));
-	}
+__device__ __forceinline__ Projected projectedPoint(const TaskDev &t, int32_t local) {
+	if (t.pointKind == POINTS_PROJECTED) return reinterpret_cast<const Projected *>(t.points)[local];
+	const f3 p = make3(t.points[local * 3], t.points[local * 3 + 1], t.points[local * 3 + 2]);
+	if (t.pointKind == POINTS_CAMERA_SPACE) return cameraToScreen(t.camera, p);
+	return cameraToScreen(t.camera, worldToCamera(t.camera.location, transformPoint(transformFrom(t.transform), p)));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -158,7 +151,7 @@ __device__ __forceinline__ TriangleCorners loadCorners(const FrameDev &f, int32_
 	TriangleCorners tc;
 	tc.poly = fan.polygonFan >> 1;
 	tc.k = fan.polygonFan & 1;
-	for (int c = 0; c < 3; c++) tc.P[c] = f.projected[t.pointBase + fan.corner[c]];
+	for (int c = 0; c < 3; c++) tc.P[c] = projectedPoint(t, fan.corner[c]);
 	return tc;
 }
 
@@ -1169,6 +1162,18 @@ __device__ __forceinline__ void reportCounters(const FrameDev &f) {
 	if (blockIdx.x == 0 && threadIdx.x < HOST_COUNTERS) f.hostCounters[threadIdx.x] = f.counters[threadIdx.x];
 }
 
+// A raster that leaves the next frame what k_frame_begin would (selfClean): the counters and the
+// canvases' first-command indices come in two sets used by alternate frames, and workgroup 0
+// resets the set this frame does not use; each tile's workgroup zeroes the tile's count once it
+// has read it (tiles no workgroup visits have a count of zero).  The next frame then needs no
+// begin kernel.  A frame whose raster returns early on an overflow leaves counts behind; the
+// host runs the begin kernel after it.
+__device__ __forceinline__ void resetNextCounters(const FrameDev &f) {
+	if (!f.selfClean || blockIdx.x != 0) return;
+	for (int32_t i = threadIdx.x; i < COUNTERS; i += blockDim.x) f.nextCounters[i] = 0;
+	for (int32_t c = threadIdx.x; c < f.canvasCount; c += blockDim.x) f.nextCanvasBase[c] = 0x7FFFFFFF;
+}
+
 // Runs body(tile) over the tiles with commands -- every tile when the raster fills the
 // canvases -- XCD by XCD.  The slots are the ordered queue of non-empty tiles after the scan,
 // or the tiles themselves with direct bins (empty ones skipped).  Without persistence the grid
@@ -1642,6 +1647,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 		if (colorWrite) cv.color[(size_t)y * cv.colorStride + x] = s.color[p];
 		if (depthRead) cv.depth[(size_t)y * cv.depthStride + x] = s.depth[p];
 	}
+	if (f.selfClean && t == 0) f.tileCount[tile] = 0;  // read by this workgroup alone, long before
 }
 
 // Four waves per SIMD: the raster is bound by latency, and the fourth wave hides more of it than
@@ -1652,6 +1658,7 @@ template <int TW, int TH, int P>
 __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred(FrameDev f) {
 	__shared__ DeferLds<TW, TH> s;
 	reportCounters(f);
+	resetNextCounters(f);
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
 	// Small texture tables are read from LDS by the shading.
 	const bool ldsTextures = f.textureCount <= LDS_TEXTURES;
@@ -1838,12 +1845,14 @@ __device__ void depthTile(const FrameDev &f, int32_t tile, DepthLds<TW, TH> &s) 
 		if (x >= cv.width || y >= cv.height) continue;
 		cv.depth[(size_t)y * cv.depthStride + x] = depthOfKey(key, affine);
 	}
+	if (f.selfClean && t == 0) f.tileCount[tile] = 0;  // read by this workgroup alone, long before
 }
 
 template <int TW, int TH>
 __global__ void __launch_bounds__(TW * TH / 4) k_raster_depth(FrameDev f) {
 	__shared__ DepthLds<TW, TH> s;
 	reportCounters(f);
+	resetNextCounters(f);
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvas untouched
 	forEachQueuedTile(f, 0, &s.slot, [&](int32_t tile) { depthTile<TW, TH>(f, tile, s); });
 }
@@ -2088,12 +2097,10 @@ __global__ void k_frame_begin(FrameDev f) {
 	if (i < COUNTERS) f.counters[i] = 0;
 	if (i < 2 * f.tileTotal + 1) f.tileCount[i] = 0;
 	if (i < f.canvasCount) f.canvasCommandBase[i] = 0x7FFFFFFF;
-	if (i < f.pointTotal) projectPoint(f, i);
 }
 
 // renderer_occludeFromExistingTriangles (rendererAPI.cpp:241-258): the Solid draw commands of
-// every task given so far lower the grid under their hulls.  One thread per fan triangle,
-// after k_frame_begin has projected the points.
+// every task given so far lower the grid under their hulls.  One thread per fan triangle.
 __global__ void __launch_bounds__(256) k_occlude_existing(FrameDev f) {
 	__shared__ ClipPolygon clipPolys[256 / 64];
 	const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;

@@ -37,7 +37,7 @@ struct TaskDev {
 	const int32_t *polygonPart;
 	const int32_t *partTexture;  // frame texture slot per part: diffuse, light (-1 = none)
 	int32_t pointCount, fanCount;
-	int32_t pointBase, fanBase;  // offsets into the frame's projected points / fan numbering
+	int32_t pointBase, fanBase;  // offsets of the task's points and fan triangles in the frame's numbering
 	int32_t filter;              // 0 Solid, 1 Alpha
 	int32_t canvas;              // canvas slot (the batched form renders many canvases)
 	int32_t pointKind;           // POINTS_MODEL, POINTS_CAMERA_SPACE or POINTS_PROJECTED
@@ -85,7 +85,6 @@ struct FrameDev {
 	int32_t tileTotal;
 	const TextureDesc *textures;
 	int32_t textureCount;
-	Projected *projected;         // pointTotal
 	int32_t *fanCount;            // fanTotal: commands made by each fan triangle
 	int32_t *blockSums;           // scan scratch (per block: int32 for triangles, int64 for tiles)
 	Command *commands;
@@ -97,6 +96,9 @@ struct FrameDev {
 	                              // bin overflowed, [7] pair records, [8] most commands of one setup
 	                              // block when a single-pass block overflowed, [16..23] per-XCD raster heads
 	int32_t blockCapacity;        // single-pass setup: command slots per setup block (0: two passes)
+	int32_t selfClean;            // the raster leaves zeroed counts for the next frame (resetNextCounters)
+	int32_t *nextCounters;        // selfClean: the counter set of the next frame
+	int32_t *nextCanvasBase;      // selfClean: the next frame's canvasCount first-command indices
 	int32_t *tileCount;           // tileTotal + 1
 	// Tile lists.  With binCapacity > 0 tile t owns tileList[t * binCapacity, ...) and the setup
 	// kernel appends to it directly; otherwise the lists are packed at tileOffset (an exclusive
