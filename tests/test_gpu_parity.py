@@ -433,3 +433,22 @@ def test_occlusion_calls_need_begin_and_a_depth_canvas(api):
         api.renderer_occludeFromTopRows(r, cam)
     assert e.value.code == -2 and "depth buffer" in str(e.value)
     api.renderer_end(r)
+
+
+def test_consecutive_frames_on_one_renderer(api):
+    """Frames drawn one after another on one renderer (the counts one frame's raster leaves
+    zeroed are the next frame's starting counts) equal frames drawn on fresh renderers: scenes
+    of different sizes, a crowded one whose tile bins overflow and are redrawn, and repeats."""
+    small = S.cube_soup(width=160, height=120, triangles=400, seed=31, spread=0.3, cam_distance=1.5, name="small")
+    wide = S.cube_soup(width=400, height=240, triangles=1500, seed=32, spread=0.4, cam_distance=1.8, name="wide")
+    crowded = S.cube_soup(width=96, height=64, triangles=6000, seed=33, spread=0.6, cam_distance=1.6, name="crowded")
+    textured = fixture_scenes.shader_branch_scenes()[0]
+    order = [small, wide, small, crowded, small, textured, wide, wide, crowded, small]
+    r = api.renderer_create()
+    for i, sc in enumerate(order):
+        print(f"[gpu] frame {i}: {sc.name} {sc.width}x{sc.height}", flush=True)
+        got = api.render_scene(sc, r)
+        want = api.render_scene(sc, api.renderer_create())
+        assert np.array_equal(got[0], want[0]), f"frame {i} ({sc.name}): colour differs from a fresh renderer's"
+        assert np.array_equal(got[1].view(np.uint32), want[1].view(np.uint32)), f"frame {i} ({sc.name}): depth differs"
+    assert_parity(small, api.render_scene(small, r), H.oracle_render(small))
