@@ -149,9 +149,14 @@ int dfpsr_hip_model_render_depth(const dfpsr_hip_model *model, const float trans
                                  const dfpsr_hip_camera *camera);
 /* No counterpart in the reference: dfpsr_hip_model_render_depth for callers that cannot
  * allocate device memory (code under HIP graph capture elsewhere in the process, tight
- * serving loops, harnesses that forbid hipMalloc).  Same kernels and the same canvas as
- * dfpsr_hip_model_render_depth, but it calls no hipMalloc/hipFree: the projected points go to
- * a fixed device array of 2^18 entries (10 MiB per GPU, held until the process ends).
+ * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc/hipFree: the
+ * projected points go to a fixed device array of 2^18 entries (10 MiB per GPU, held until the
+ * process ends), and each fan triangle is drawn by its own group of lanes with a
+ * compare-and-swap per pixel instead of the binned tiles of dfpsr_hip_model_render_depth (whose
+ * bins are working memory).  The row and depth arithmetic is the same, and so is the canvas,
+ * except where two drawn depths of +0 and -0 meet on a pixel: the tiles keep the first, as the
+ * reference does, this entry either (a drawn depth of -0 needs weight-plane products that
+ * underflow to zero).
  *   - The canvas must be device memory (on_device = 1); a host canvas fails with
  *     DFPSR_HIP_ERROR_ARGUMENT.
  *   - The model must already be resident: a model with changes not yet uploaded (never
