@@ -1264,7 +1264,7 @@ __device__ __forceinline__ RowInterval rowInterval(RowSpan r) { return RowInterv
 // Threads of a visibility-raster workgroup: four pixels of the tile per thread.
 template <int TW, int TH> struct DeferShape {
 	static constexpr int32_t NT = TW * TH / 4;
-	static constexpr int32_t CHUNK = 32;                   // commands per chunk
+	static constexpr int32_t CHUNK = 16;                   // commands per chunk
 	static constexpr int32_t LIST = 256;                    // commands sorted per round
 };
 
@@ -1655,7 +1655,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 // two forms are separate kernels so that each gets its own register allocation (the persistent
 // loop keeps more values live).
 template <int TW, int TH, int P>
-__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred(FrameDev f) {
+__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(5, 5))) k_raster_deferred(FrameDev f) {
 	__shared__ DeferLds<TW, TH> s;
 	reportCounters(f);
 	resetNextCounters(f);
