@@ -437,7 +437,7 @@ static int checkCamera(const dfpsr_hip_camera *c) {
 static bool hasImage(const dfpsr_hip_image *im) { return im && im->pixels; }
 
 // model_set_points may replace the points with fewer after polygons were added; drawing such a
-// model would read projected points past the model's own.  Checked before any device work.
+// model would read points past the model's own.  Checked before any device work.
 static int checkModelIndices(const dfpsr_hip_model *m, const char *who) {
 	if (m->maxIndex >= m->pointCount())
 		return fail(DFPSR_HIP_ERROR_ARGUMENT, std::string(who) + ": a polygon names a point the model does not have");

@@ -1,7 +1,9 @@
 // The launch sequence of one frame (renderer_end, or one batch of jobs):
-//   [clear canvases, unless the raster fills them] -> frame begin (counters, projection)
-//   -> triangle setup (count, scan, emit with per-command row setup, shader inputs and tile
-//   counts) -> tile offsets (scan + ordered tile queue) -> list fill -> tile raster.
+//   [clear canvases, unless the raster fills them] -> [frame begin: counters and tile counts
+//   zeroed, unless the previous frame's raster left them so] -> triangle setup (the corners
+//   projected, count, emit with per-command row setup, shader inputs and tile counts, in one
+//   pass or in count / scan / emit) -> [tile offsets (scan + ordered tile queue) -> list fill,
+//   for packed lists] -> tile raster.
 // Buffers that depend on data (draw commands, tile pairs) start from an estimate; a pass that
 // runs out of room raises a flag, leaves the canvases untouched, and the frame is run again
 // with the exact sizes.  With RenderOptions::graph the sequence is captured once into a
@@ -103,7 +105,7 @@ int32_t Frame::textureSlot(const TextureDesc *desc) {
 int Frame::addTask(const ModelView &m, const float *transform, const CameraParams &camera, int32_t canvas, int32_t pointKind) {
 	if (m.fanCount == 0 || m.pointCount == 0) return 0;
 	// The points may have been replaced by fewer after the polygons were added: a polygon
-	// naming a missing point would send the setup kernel past the projected points.
+	// naming a missing point would send the setup kernel past the model's points.
 	if (m.maxIndex >= m.pointCount) return fail(-1, "giveTask: a polygon names a point the model does not have");
 	TaskDev t;
 	std::memset(&t, 0, sizeof(t));
