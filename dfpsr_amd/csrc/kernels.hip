@@ -560,6 +560,7 @@ __device__ __forceinline__ void setupEmit(const FrameDev &f, int32_t sumBlocks) 
 				if (f.binCapacity > 0) binPair(f, tile[u], s, index[u], binned);
 				else recordPair(f, recordBase + round + (int32_t)threadIdx.x + u * SCAN_BLOCK, tile[u], s, index[u]);
 			}
+			if (round + UNROLL * SCAN_BLOCK >= pairs) break;  // the last round: the table is not needed again
 			__syncthreads();
 			for (int32_t e = threadIdx.x; e < TILE_TABLE; e += SCAN_BLOCK) {
 				tableTile[e] = -1;
