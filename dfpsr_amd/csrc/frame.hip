@@ -332,6 +332,8 @@ int Frame::run(const RenderOptions &o) {
 		f.binCapacity = direct ? binCapacity_ : 0;
 		f.tileW = tileConfigW(tileConfig_);
 		f.tileH = tileConfigH(tileConfig_);
+		f.tileShiftW = f.tileW == 32 ? 5 : 4;
+		f.tileShiftH = f.tileH == 32 ? 5 : 4;
 		f.alphaAny = alphaAny_ ? 1 : 0;
 		f.clearInRaster = (o.clear && o.mode == 0 && deferredRaster(o)) ? 1 : 0;
 		f.clearColor = o.clearColor;

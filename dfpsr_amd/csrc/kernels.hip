@@ -32,6 +32,17 @@ __device__ __forceinline__ Projected projectedPoint(const TaskDev &t, int32_t lo
 // ---------------------------------------------------------------------------------------
 DH Rect canvasRect(const CanvasDev &cv) { return rectSides(0, 0, cv.width, cv.height); }
 
+// n / d for 0 <= n < 2^24 and d >= 1 with a quotient below 2^21 (a command's pair index over
+// its tile columns: at most a canvas's tiles, and its tile rows): both operands are exact in
+// float and the float quotient is within one of the integer one, which one step corrects -- no
+// integer division sequence.
+__device__ __forceinline__ int32_t divSmall(int32_t n, int32_t d) {
+	int32_t q = (int32_t)((float)n * __builtin_amdgcn_rcpf((float)d));
+	if (q * d > n) q--;
+	else if ((q + 1) * d <= n) q++;
+	return q;
+}
+
 // (command, tile) pairs and newly used tiles counted by one thread of the setup kernel.
 struct BinTally { int32_t pairs, tiles; };
 
@@ -74,10 +85,11 @@ __device__ void emitCommand(const FrameDev &f, int32_t index, const Command &cmd
 	if (f.mode == 0) f.shaders[index] = si;
 	int32_t tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;  // no tiles when the bound misses the canvas
 	if (tr.clipR > tr.clipL) {
-		tx0 = tr.clipL / f.tileW;
-		tx1 = (tr.clipR - 1) / f.tileW;
-		ty0 = tr.alT / f.tileH;
-		ty1 = tmin((tr.alB - 2) / f.tileH, cv.tilesY - 1);
+		// The columns and rows are at least 0 here, so shifts divide.
+		tx0 = tr.clipL >> f.tileShiftW;
+		tx1 = (tr.clipR - 1) >> f.tileShiftW;
+		ty0 = tr.alT >> f.tileShiftH;
+		ty1 = tmin((tr.alB - 2) >> f.tileShiftH, cv.tilesY - 1);
 	}
 	const int32_t w = tx1 - tx0 + 1, count = tmax(w, 0) * tmax(ty1 - ty0 + 1, 0);
 	if (jobs) {
@@ -524,7 +536,8 @@ __device__ __forceinline__ void setupEmit(const FrameDev &f, int32_t sumBlocks) 
 					}
 					const BinJob &j = jobs[lo];
 					const int32_t i = p - j.pairStart;
-					tile[u] = j.firstTile + (i / j.width) * j.tilesX + i % j.width;
+					const int32_t row = divSmall(i, j.width);
+					tile[u] = j.firstTile + row * j.tilesX + (i - row * j.width);
 					index[u] = j.index;
 					const int32_t h = (int32_t)(((uint32_t)tile[u] * 2654435761u) >> (32 - TILE_TABLE_LOG2));
 					for (int32_t q = 0; q < TILE_TABLE_PROBES; q++) {

@@ -115,6 +115,7 @@ struct FrameDev {
 	int32_t *queueHead;           // next queue slot for persistent raster workgroups
 	int32_t *hostCounters;        // pinned host copy of counters[0..7], written by the raster kernel
 	int32_t tileW, tileH;
+	int32_t tileShiftW, tileShiftH;  // log2 of tileW and tileH (the tile sides are powers of two)
 	int32_t alphaAny;             // any task uses the Alpha filter: tiles read colour first
 	// The visibility raster fills the canvases itself (image_fill) instead of a separate clear:
 	// it then visits every tile, and empty ones only take the fill values.
