@@ -202,11 +202,12 @@ __device__ __forceinline__ bool survivesOcclusion(const FrameDev &f, const Proje
 // sub-vertices indexed at run time) lives in LDS instead of private scratch memory.
 // single: the caller has counted this triangle already and found one draw command, inside the
 // clip frustum and front-facing (the tests are not repeated).  With SETUP_COUNT, *single is set
-// when that is the case.
+// when that is the case.  haveShader: shader holds the triangle's shader inputs already.
 template <int MODE>
-__device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset, BinTally &tally, ClipPolygon *clipPoly,
-                                 BinJob *jobs = nullptr, int32_t jobBase = 0, const TriangleCorners *corners = nullptr,
-                                 bool *single = nullptr) {
+__device__ __forceinline__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32_t offset, BinTally &tally,
+                                                 ClipPolygon *clipPoly, BinJob *jobs = nullptr, int32_t jobBase = 0,
+                                                 const TriangleCorners *corners = nullptr, bool *single = nullptr,
+                                                 bool haveShader = false, ShaderInputs shader = ShaderInputs()) {
 	constexpr bool EMIT = MODE == SETUP_EMIT;
 	const TaskDev &t = f.tasks[ti];
 	const CanvasDev &cv = f.canvases[t.canvas];
@@ -228,7 +229,7 @@ __device__ int32_t setupTriangle(const FrameDev &f, int32_t g, int32_t ti, int32
 	int32_t count = 0;
 	const bool perspective = cam.perspective != 0;
 	ShaderInputs si;
-	if (EMIT && f.mode == 0) si = triangleShader(t, poly, k);
+	if (EMIT && f.mode == 0) si = haveShader ? shader : triangleShader(t, poly, k);
 	if (known || triangleVisibility(P[0].cs, P[1].cs, P[2].cs, cam.clip) == VIS_FULL) {
 		if (known || frontFacing(P[0].flat, P[1].flat, P[2].flat)) {
 			if (MODE == SETUP_OCCLUDE) {
@@ -448,13 +449,19 @@ __device__ __forceinline__ void setupEmit(const FrameDev &f, int32_t sumBlocks) 
 	}
 	const int32_t ti = g < f.fanTotal ? findSegment(f.taskFanEnd, f.taskCount, g) : 0;
 	TriangleCorners corners;
+	ShaderInputs shader;
 	int32_t n;
 	bool single = false;  // counted as one unclipped, front-facing command
 	if (SINGLE) {
+		// The corners and the shader inputs of a triangle that makes commands stay in registers
+		// from the count to the emit; the shader inputs' loads are issued before the block's
+		// prefix sums, so that their latency overlaps them.
 		n = 0;
 		if (g < f.fanTotal) {
 			BinTally unused{0, 0};
-			n = setupTriangle<SETUP_COUNT>(f, g, ti, 0, unused, &clipPolys[threadIdx.x >> 6], nullptr, 0, nullptr, &single);
+			corners = loadCorners(f, g, ti);
+			n = setupTriangle<SETUP_COUNT>(f, g, ti, 0, unused, &clipPolys[threadIdx.x >> 6], nullptr, 0, &corners, &single);
+			if (n > 0 && f.mode == 0) shader = triangleShader(f.tasks[ti], corners.poly, corners.k);
 		}
 	} else {
 		n = (g < f.fanTotal) ? f.fanCount[g] : 0;
@@ -497,10 +504,9 @@ __device__ __forceinline__ void setupEmit(const FrameDev &f, int32_t sumBlocks) 
 		// triangle of each task bounds its canvas's first command: one atomic per task.
 		if (g == f.tasks[ti].fanBase) atomicMin(&f.canvasCommandBase[f.tasks[ti].canvas], offset);
 		if (n > 0) {
-			if (SINGLE) corners = loadCorners(f, g, ti);  // the count above read them; these loads hit the caches
 			if (!SINGLE && offset + n > f.commandCapacity) atomicOr(&f.counters[2], 1);
 			else if (room) setupTriangle<SETUP_EMIT>(f, g, ti, offset, tally, &clipPolys[threadIdx.x >> 6], together ? jobs : nullptr, jobBase, &corners,
-			                                         SINGLE ? &single : nullptr);
+			                                         SINGLE ? &single : nullptr, SINGLE && f.mode == 0, shader);
 		}
 	}
 	if (together) {
