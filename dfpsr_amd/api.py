@@ -346,9 +346,27 @@ class GpuScene:
                           self.textures[p.light] if p.light >= 0 else None)
         self.camera = camera_create(scene.perspective, scene.cam_location, scene.width, scene.height, scene.width_slope,
                                     scene.near_clip, scene.far_clip)
+        self._prepared = None
+
+    def _prepared_arguments(self):
+        """The instances' transforms and the camera as the C ABI takes them, made once and again
+        only when the scene's instances or the camera object are replaced."""
+        key = (id(self.scene.instances), id(self.camera))
+        if self._prepared is None or self._prepared[0] != key:
+            transforms = [np.ascontiguousarray(t, dtype=np.float32) for t in self.scene.instances]
+            self._prepared = (key, transforms, [t.ctypes.data_as(C.c_void_p) for t in transforms], C.byref(self.camera),
+                              (self.scene.instances, self.camera))
+        return self._prepared
 
     def give(self, renderer: Renderer) -> None:
         sc = self.scene
+        if sc.projected is None and not sc.occluders:
+            # The common frame: one renderer_giveTask per instance with the prepared arguments.
+            _, _, pointers, camera, _ = self._prepared_arguments()
+            give_model = lib().dfpsr_hip_renderer_give_model
+            for ptr in pointers:
+                check(give_model(renderer.handle, self.model.handle, ptr, camera))
+            return
         if sc.projected is not None:
             # One renderer_giveTask_triangle batch per part, in polygon order.
             for p in sc.parts:
