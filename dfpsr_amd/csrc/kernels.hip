@@ -1282,9 +1282,17 @@ __device__ __forceinline__ RowSpan rowSpan(RowInterval r) { return RowSpan{(int1
 __device__ __forceinline__ RowInterval rowInterval(RowSpan r) { return RowInterval{r.left, r.right}; }
 
 // Threads of a visibility-raster workgroup: four pixels of the tile per thread.
-template <int TW, int TH> struct DeferShape {
+// CHUNK: commands staged and drawn at once.  Sixteen keep a 32x16 workgroup within the LDS of ten
+// workgroups per CU, the occupancy of five waves per SIMD.  The 32-pixel-wide tiles of kernels
+// that hold fewer workgroups per CU anyway (the persistent grid, the differentiable shader at
+// four waves per SIMD) take 32, so that fewer tiles need a second chunk; 16x16 tiles keep 16,
+// as the LDS of their one-wave workgroups sets their occupancy.
+template <int TW, int TH, bool FEWER_WORKGROUPS>
+constexpr int32_t deferChunk() { return FEWER_WORKGROUPS && TW * TH >= 512 ? 32 : 16; }
+
+template <int TW, int TH, int CH> struct DeferShape {
 	static constexpr int32_t NT = TW * TH / 4;
-	static constexpr int32_t CHUNK = 16;                   // commands per chunk
+	static constexpr int32_t CHUNK = CH;                   // commands per chunk
 	static constexpr int32_t LIST = 256;                    // commands sorted per round
 };
 
@@ -1338,8 +1346,8 @@ __device__ void shadeDiffLanes(const FrameDev &f, const CommandTail &ct, const Q
 }
 
 // LDS of one visibility-raster workgroup.
-template <int TW, int TH> struct DeferLds {
-	using S = DeferShape<TW, TH>;
+template <int TW, int TH, int CH> struct DeferLds {
+	using S = DeferShape<TW, TH, CH>;
 	static constexpr int32_t PAIRS = S::CHUNK * (TH / 2);  // (command, row pair) slots of a chunk
 	// Pass-1 items are found by table up to one per pixel per chunk (by binary search beyond).
 	static constexpr int32_t ITEM_TABLE = TW * TH;
@@ -1417,13 +1425,13 @@ __device__ __forceinline__ int32_t segmentOf(const T *offset, int32_t count, int
 
 // MODE 0: the RGBA8 shaders into the colour canvas.  MODE 1: the differentiable float shader
 // into the float colour canvas plus the visibility id of each pixel's winning lane.
-template <int TW, int TH, int MODE = 0>
-__device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW, TH> &s, const TextureDesc *textures) {
-	using S = DeferShape<TW, TH>;
+template <int TW, int TH, int MODE = 0, int CH>
+__device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW, TH, CH> &s, const TextureDesc *textures) {
+	using S = DeferShape<TW, TH, CH>;
 	constexpr int32_t NT = S::NT, CHUNK = S::CHUNK;
 	constexpr int32_t NP = TW * TH, RP = TH / 2;
-	constexpr int32_t PAIRS = DeferLds<TW, TH>::PAIRS;
-	constexpr int32_t ITEM_TABLE = DeferLds<TW, TH>::ITEM_TABLE;
+	constexpr int32_t PAIRS = DeferLds<TW, TH, CH>::PAIRS;
+	constexpr int32_t ITEM_TABLE = DeferLds<TW, TH, CH>::ITEM_TABLE;
 	constexpr int32_t PAIRS_PER_THREAD = PAIRS / NT > 0 ? PAIRS / NT : 1;
 	static_assert(PAIRS * (TW / 2) <= 32 * NT, "a thread's items fit the 32 four-bit masks of pass 2a");
 	const int32_t n = f.tileCount[tile];
@@ -1524,7 +1532,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				const int32_t pr = t * PAIRS_PER_THREAD + k;
 				if (pr < PAIRS) s.pairOffset[pr] = (int16_t)before;
 				for (int32_t i = before; i < before + pairCount[k] && i < ITEM_TABLE; i++)
-					s.pairOfItem[i] = (typename DeferLds<TW, TH>::PairIndex)pr;
+					s.pairOfItem[i] = (typename DeferLds<TW, TH, CH>::PairIndex)pr;
 				before += pairCount[k];
 			}
 			if (t == 0) s.pairOffset[PAIRS] = (int16_t)items;
@@ -1670,13 +1678,14 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	if (f.selfClean && t == 0) f.tileCount[tile] = 0;  // read by this workgroup alone, long before
 }
 
-// Four waves per SIMD: the raster is bound by latency, and the fourth wave hides more of it than
-// the few registers the compiler then keeps in scratch cost.  P = 1: persistent workgroups.  The
-// two forms are separate kernels so that each gets its own register allocation (the persistent
-// loop keeps more values live).
+// Five waves per SIMD: the raster is bound by latency, and the fifth wave hides more of it than the
+// few registers the compiler then keeps in scratch cost.  P = 1: persistent workgroups, at four
+// waves per SIMD (a persistent grid holds at most eight workgroups per CU).  The two forms are
+// separate kernels so that each gets its own register allocation (the persistent loop keeps
+// more values live).
 template <int TW, int TH, int P>
-__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(5, 5))) k_raster_deferred(FrameDev f) {
-	__shared__ DeferLds<TW, TH> s;
+__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(P ? 4 : 5, P ? 4 : 5))) k_raster_deferred(FrameDev f) {
+	__shared__ DeferLds<TW, TH, deferChunk<TW, TH, P != 0>()> s;
 	reportCounters(f);
 	resetNextCounters(f);
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
@@ -1695,7 +1704,7 @@ __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_e
 // The differentiable forward by the visibility raster.
 template <int TW, int TH, int P>
 __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred_diff(FrameDev f) {
-	__shared__ DeferLds<TW, TH> s;
+	__shared__ DeferLds<TW, TH, deferChunk<TW, TH, true>()> s;
 	reportCounters(f);
 	if (f.counters[2]) return;
 	forEachQueuedTile(f, P, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH, 1>(f, tile, s, nullptr); });
