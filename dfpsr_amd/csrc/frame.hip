@@ -319,7 +319,6 @@ int Frame::run(const RenderOptions &o) {
 		f.commandCapacity = commandCapacity_;
 		// Two sets of counters and first-command indices, used by alternate self-cleaning frames.
 		f.counters = dCounters_.as<int32_t>() + parity_ * COUNTERS;
-		f.queueHead = f.counters + 4;
 		f.hostCounters = hostCountersDevice_;
 		f.tileCount = dTileCount_.as<int32_t>();
 		f.tileFill = dTileCount_.as<int32_t>() + tileTotal + 1;

@@ -1218,7 +1218,7 @@ __device__ void forEachQueuedTile(const FrameDev &f, int32_t persistent, int32_t
 	for (int32_t r = 0; r < XCDS; r++) {
 		const int32_t xr = (x + r) % XCDS;
 		while (true) {
-			if (threadIdx.x == 0) *slotLds = xcdSlot(xr, atomicAdd(&f.counters[16 + xr], 1));
+			if (threadIdx.x == 0) *slotLds = xcdSlot(xr, atomicAdd(&f.counters[QUEUE_HEADS + QUEUE_HEAD_STRIDE * xr], 1));
 			__syncthreads();
 			const int32_t slot = *slotLds;
 			__syncthreads();

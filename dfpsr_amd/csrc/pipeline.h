@@ -92,9 +92,9 @@ struct FrameDev {
 	TriangleRows *triangleRows;   // per command: edges and bounds of its rows
 	int32_t commandCapacity;
 	int32_t *counters;            // [0] commands, [1] pairs, [2] overflow flags, [3] tiles with commands,
-	                              // [4] raster head, [5] backward head, [6] longest tile list when a
-	                              // bin overflowed, [7] pair records, [8] most commands of one setup
-	                              // block when a single-pass block overflowed, [16..23] per-XCD raster heads
+	                              // [5] backward head, [6] longest tile list when a bin overflowed,
+	                              // [7] pair records, [8] most commands of one setup block when a
+	                              // single-pass block overflowed, [QUEUE_HEADS + 32 x] XCD x's raster head
 	int32_t blockCapacity;        // single-pass setup: command slots per setup block (0: two passes)
 	int32_t selfClean;            // the raster leaves zeroed counts for the next frame (resetNextCounters)
 	int32_t *nextCounters;        // selfClean: the counter set of the next frame
@@ -112,7 +112,6 @@ struct FrameDev {
 	PairRecord *pairRecords;      // pairCapacity (packed lists only)
 	int32_t *tileQueue;           // tiles with at least one command, in tile order (counters[3] of them)
 	int32_t *canvasCommandBase;   // per canvas: index of its first draw command (visibility ids count from it)
-	int32_t *queueHead;           // next queue slot for persistent raster workgroups
 	int32_t *hostCounters;        // pinned host copy of counters[0..7], written by the raster kernel
 	int32_t tileW, tileH;
 	int32_t tileShiftW, tileShiftH;  // log2 of tileW and tileH (the tile sides are powers of two)
@@ -187,10 +186,6 @@ __global__ void k_depth_project(const float *points, int32_t n, Transform modelT
 __global__ void k_depth_draw(const Projected *projected, const FanRecord *fans, int32_t fanCount,
                              int32_t groupShift, CameraParams cam, DepthCanvas cv);
 
-constexpr int32_t SCAN_BLOCK = 256;
-constexpr int32_t COUNTERS = 32;
-constexpr int32_t HOST_COUNTERS = 16;  // counters[0..15] are copied to pinned host memory
-
 // Workgroups are dispatched to gfx950's eight XCDs round-robin (workgroup b runs on XCD
 // b % 8, each XCD with its own L2).  The ordered tile queue is cut into segments of
 // XCD_SEGMENT consecutive slots dealt to the XCDs in turn: XCD x takes segments x, x + 8,
@@ -198,6 +193,14 @@ constexpr int32_t HOST_COUNTERS = 16;  // counters[0..15] are copied to pinned h
 // XCD gets a share of every part of the canvas, so a band of costly tiles (a horizon, say)
 // is spread over all eight instead of landing on one.
 constexpr int32_t XCDS = 8;
+
+constexpr int32_t SCAN_BLOCK = 256;
+// The persistent raster's per-XCD queue heads, each on a 128-byte line of its own after the
+// frame's other counters, so that the eight XCDs' tickets do not contend for one line.
+constexpr int32_t QUEUE_HEADS = 32;
+constexpr int32_t QUEUE_HEAD_STRIDE = 32;
+constexpr int32_t COUNTERS = QUEUE_HEADS + QUEUE_HEAD_STRIDE * XCDS;
+constexpr int32_t HOST_COUNTERS = 16;  // counters[0..15] are copied to pinned host memory
 #ifndef DFPSR_XCD_SEGMENT
 #define DFPSR_XCD_SEGMENT 4
 #endif
