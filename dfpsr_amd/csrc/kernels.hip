@@ -1928,7 +1928,10 @@ struct AdjointTerms {
 	float mul[9];
 };
 
-__global__ void __launch_bounds__(256) k_backward(FrameDev f, int32_t *head) {
+// Three waves per SIMD (at most 168 VGPRs): the compiler's own choice of 189 registers left two,
+// and the third wave hides more of the adjoint's gather and atomic latency than the few values it
+// then keeps in scratch (and some private arrays it moves to LDS) cost.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_backward(FrameDev f, int32_t *head) {
 	__shared__ int32_t tileLds;
 	__shared__ int32_t slotCommand[BACKWARD_SLOTS];
 	__shared__ int32_t slotPoint[BACKWARD_SLOTS][3];
