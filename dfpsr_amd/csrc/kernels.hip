@@ -1776,7 +1776,7 @@ __device__ __forceinline__ float depthOfKey(unsigned long long key, bool affine)
 template <int TW, int TH> struct DepthLds {
 	static constexpr int32_t NT = TW * TH / 4;
 	static constexpr int32_t CHUNK = 32;   // commands staged at once
-	static constexpr int32_t LIST = 256;   // commands sorted per round
+	static constexpr int32_t LIST = 128;   // commands sorted per round (longer lists take more rounds)
 	int32_t list[LIST];
 	unsigned long long keys[TW * TH];
 	TriangleRows tr[CHUNK];
