@@ -1282,13 +1282,24 @@ __device__ __forceinline__ RowSpan rowSpan(RowInterval r) { return RowSpan{(int1
 __device__ __forceinline__ RowInterval rowInterval(RowSpan r) { return RowInterval{r.left, r.right}; }
 
 // Threads of a visibility-raster workgroup: four pixels of the tile per thread.
-// CHUNK: commands staged and drawn at once.  Sixteen keep a 32x16 workgroup within the LDS of ten
-// workgroups per CU, the occupancy of five waves per SIMD.  The 32-pixel-wide tiles of kernels
-// that hold fewer workgroups per CU anyway (the persistent grid, the differentiable shader at
-// four waves per SIMD) take 32, so that fewer tiles need a second chunk; 16x16 tiles keep 16,
-// as the LDS of their one-wave workgroups sets their occupancy.
-template <int TW, int TH, bool FEWER_WORKGROUPS>
-constexpr int32_t deferChunk() { return FEWER_WORKGROUPS && TW * TH >= 512 ? 32 : 16; }
+// CHUNK: commands staged and drawn at once.  Sixteen keep a 32x16 workgroup of the RGBA8 shaders
+// within the LDS of ten workgroups per CU, the occupancy of five waves per SIMD.  The persistent
+// grid's 32-pixel-wide tiles take 32 (at most eight workgroups per CU run anyway), and so do all
+// tiles of the differentiable shader, whose scenes are denser, so that fewer tiles need a
+// second chunk.
+template <int TW, int TH, int P>
+constexpr int32_t colorChunk() { return P != 0 && TW * TH >= 512 ? 32 : 16; }
+constexpr int32_t DIFF_CHUNK = 32;
+
+// The thread's index as a value the compiler cannot see through.  Taken at the top of a loop's
+// body, the values derived from it there are computed in each trip instead of being held in
+// registers across the loop (the visibility raster's per-chunk work would otherwise keep a
+// dozen of them live through a whole tile, in scratch memory at five waves per SIMD).
+__device__ __forceinline__ int32_t opaqueThreadIndex() {
+	int32_t t = (int32_t)threadIdx.x;
+	asm volatile("" : "+v"(t));
+	return t;
+}
 
 template <int TW, int TH, int CH> struct DeferShape {
 	static constexpr int32_t NT = TW * TH / 4;
@@ -1461,6 +1472,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	while (done < n) {
 		const int32_t take = nextRound<NT, S::LIST>(src, n, done, last, s.list, s.roundInfo);
 		for (int32_t c0 = 0; c0 < take; c0 += CHUNK) {
+			const int32_t t = opaqueThreadIndex();
 			const int32_t cn = take - c0 < CHUNK ? take - c0 : CHUNK;
 			// The chunk's row setups, shader inputs and weight planes into LDS.
 			stageToLds<NT>(s.tr, (const uint8_t *)f.triangleRows, (int32_t)sizeof(TriangleRows), 0, s.list, c0, cn);
@@ -1678,14 +1690,13 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	if (f.selfClean && t == 0) f.tileCount[tile] = 0;  // read by this workgroup alone, long before
 }
 
-// Five waves per SIMD: the raster is bound by latency, and the fifth wave hides more of it than the
-// few registers the compiler then keeps in scratch cost.  P = 1: persistent workgroups, at four
-// waves per SIMD (a persistent grid holds at most eight workgroups per CU).  The two forms are
-// separate kernels so that each gets its own register allocation (the persistent loop keeps
-// more values live).
+// Five waves per SIMD: the raster is bound by latency, and the fifth wave hides more of it than a
+// larger register budget would save.  P = 1: persistent workgroups, at four waves per SIMD (a
+// persistent grid holds at most eight workgroups per CU).  The two forms are separate kernels so
+// that each gets its own register allocation (the persistent loop keeps more values live).
 template <int TW, int TH, int P>
 __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(P ? 4 : 5, P ? 4 : 5))) k_raster_deferred(FrameDev f) {
-	__shared__ DeferLds<TW, TH, deferChunk<TW, TH, P != 0>()> s;
+	__shared__ DeferLds<TW, TH, colorChunk<TW, TH, P>()> s;
 	reportCounters(f);
 	resetNextCounters(f);
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
@@ -1704,7 +1715,7 @@ __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_e
 // The differentiable forward by the visibility raster.
 template <int TW, int TH, int P>
 __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred_diff(FrameDev f) {
-	__shared__ DeferLds<TW, TH, deferChunk<TW, TH, true>()> s;
+	__shared__ DeferLds<TW, TH, DIFF_CHUNK> s;
 	reportCounters(f);
 	if (f.counters[2]) return;
 	forEachQueuedTile(f, P, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH, 1>(f, tile, s, nullptr); });
