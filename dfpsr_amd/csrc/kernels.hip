@@ -2020,6 +2020,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
 	};
 
 	while (true) {
+		const int32_t t = opaqueThreadIndex();  // values derived from it are computed per tile
 		if (t == 0) tileLds = atomicAdd(head, 1);
 		if (t < BACKWARD_SLOTS) slotCommand[t] = -1;
 		for (int32_t i = t; i < BACKWARD_SLOTS * 9; i += 256) (&slotGrad[0][0])[i] = 0.0f;
