@@ -334,7 +334,10 @@ int Frame::run(const RenderOptions &o) {
 		f.tileShiftW = f.tileW == 32 ? 5 : 4;
 		f.tileShiftH = f.tileH == 32 ? 5 : 4;
 		f.alphaAny = alphaAny_ ? 1 : 0;
-		f.clearInRaster = (o.clear && o.mode == 0 && deferredRaster(o)) ? 1 : 0;
+		// The differentiable frames fill in the raster only with 32-pixel-wide tiles: with 16x16
+		// tiles the separate fill kernel streams the float canvases faster than the one-wave
+		// workgroups' write-back of every pixel (256-view batch forward 1.61 -> 1.98 ms fused).
+		f.clearInRaster = (o.clear && (o.mode == 0 || (o.mode == 1 && f.tileW == 32)) && deferredRaster(o)) ? 1 : 0;
 		f.clearColor = o.clearColor;
 		f.clearDepth = o.clearDepth;
 		f.mode = o.mode;

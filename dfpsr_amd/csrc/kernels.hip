@@ -1465,7 +1465,12 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 			s.depth[p] = (depthRead && x < cv.width && y < cv.height) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
 		}
 	}
-	for (int32_t w = t; w < NP / 32; w += NT) s.dirty[w] = f.clearInRaster ? ~0u : 0u;
+	// With the fill fused in, every pixel is written back.  The differentiable shader writes its
+	// colours and ids itself, so there the dirty bits keep marking the drawn pixels: the others
+	// take the fill values at the write-back.
+	const bool fillAll = f.clearInRaster && MODE == 0;
+	const bool fillUndrawn = f.clearInRaster && MODE == 1;
+	for (int32_t w = t; w < NP / 32; w += NT) s.dirty[w] = fillAll ? ~0u : 0u;
 
 	int32_t last = -1;
 	int32_t done = 0;
@@ -1614,7 +1619,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 						pixel[l] = (qy + (sl >> 1)) * TW + qx + (sl & 1);
 						if (!((mask >> l) & 1u)) continue;
 						s.depth[pixel[l]] = e.l0[l];
-						if (!f.clearInRaster) atomicOr(&s.dirty[pixel[l] >> 5], 1u << (pixel[l] & 31));
+						if (!fillAll) atomicOr(&s.dirty[pixel[l] >> 5], 1u << (pixel[l] & 31));
 					}
 					if (colorWrite) {
 						const bool affine = (s.tail[j].flags & 1) != 0;
@@ -1667,7 +1672,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 						const int32_t sl = e.slot[l];
 						const int32_t p = (qy + (sl >> 1)) * TW + qx + (sl & 1);
 						s.depth[p] = e.l0[l];
-						if (!f.clearInRaster) atomicOr(&s.dirty[p >> 5], 1u << (p & 31));
+						atomicOr(&s.dirty[p >> 5], 1u << (p & 31));
 						slots |= (uint32_t)sl << (2 * l);
 					}
 					const int32_t id = (s.list[c0 + j] - f.canvasCommandBase[ci]) * 4;
@@ -1681,11 +1686,17 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	}
 
 	for (int32_t p = t; p < NP; p += NT) {
-		if (!((s.dirty[p >> 5] >> (p & 31)) & 1u)) continue;
+		const bool drawn = ((s.dirty[p >> 5] >> (p & 31)) & 1u) != 0;
+		if (!drawn && !fillUndrawn) continue;
 		const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
 		if (x >= cv.width || y >= cv.height) continue;
 		if (colorWrite) cv.color[(size_t)y * cv.colorStride + x] = s.color[p];
 		if (depthRead) cv.depth[(size_t)y * cv.depthStride + x] = s.depth[p];
+		if (MODE == 1 && !drawn) {  // the fill values of k_clear_canvases
+			const size_t q = (size_t)y * cv.diffStride + x;
+			*(float4 *)(cv.diffColor + q * 4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			cv.visibility[q] = -1;
+		}
 	}
 	if (f.selfClean && t == 0) f.tileCount[tile] = 0;  // read by this workgroup alone, long before
 }
