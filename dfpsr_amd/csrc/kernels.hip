@@ -1281,7 +1281,6 @@ struct RowSpan { int16_t left, right; };
 __device__ __forceinline__ RowSpan rowSpan(RowInterval r) { return RowSpan{(int16_t)r.left, (int16_t)r.right}; }
 __device__ __forceinline__ RowInterval rowInterval(RowSpan r) { return RowInterval{r.left, r.right}; }
 
-// Threads of a visibility-raster workgroup: four pixels of the tile per thread.
 // CHUNK: commands staged and drawn at once.  Sixteen keep a 32x16 workgroup of the RGBA8 shaders
 // within the LDS of ten workgroups per CU, the occupancy of five waves per SIMD.  The persistent
 // grid's 32-pixel-wide tiles take 32 (at most eight workgroups per CU run anyway), and so do all
@@ -1301,6 +1300,7 @@ __device__ __forceinline__ int32_t opaqueThreadIndex() {
 	return t;
 }
 
+// Threads of a visibility-raster workgroup: four pixels of the tile per thread.
 template <int TW, int TH, int CH> struct DeferShape {
 	static constexpr int32_t NT = TW * TH / 4;
 	static constexpr int32_t CHUNK = CH;                   // commands per chunk
