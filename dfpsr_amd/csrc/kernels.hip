@@ -1356,9 +1356,12 @@ __device__ void shadeDiffLanes(const FrameDev &f, const CommandTail &ct, const Q
 	}
 }
 
-// LDS of one visibility-raster workgroup.
-template <int TW, int TH, int CH> struct DeferLds {
+// LDS of one visibility-raster workgroup.  The differentiable shader (MODE 1) writes its colours
+// straight to the float canvas and reads no shader inputs or texture table, so it keeps none of
+// them here, and more of its workgroups fit a CU.
+template <int TW, int TH, int CH, int MODE> struct DeferLds {
 	using S = DeferShape<TW, TH, CH>;
+	static constexpr bool RGBA = MODE == 0;
 	static constexpr int32_t PAIRS = S::CHUNK * (TH / 2);  // (command, row pair) slots of a chunk
 	// Pass-1 items are found by table up to one per pixel per chunk (by binary search beyond).
 	static constexpr int32_t ITEM_TABLE = TW * TH;
@@ -1368,7 +1371,7 @@ template <int TW, int TH, int CH> struct DeferLds {
 		TriangleRows tr[S::CHUNK];               // the chunk's commands: row setup (until the rows are made),
 		unsigned long long keys[TW * TH];        // then the pixels' (depth, command) keys
 	};
-	ShaderInputs si[S::CHUNK];                   // shader inputs,
+	ShaderInputs si[RGBA ? S::CHUNK : 1];        // shader inputs,
 	CommandTail tail[S::CHUNK];                  // weight planes and flags
 	RowSpan rows[S::CHUNK * TH];                 // chunk command j, tile row r
 	union {
@@ -1382,12 +1385,12 @@ template <int TW, int TH, int CH> struct DeferLds {
 			PairIndex pairOfItem[ITEM_TABLE];    // the pair of each of the first ITEM_TABLE pass-1 items
 		};
 	};
-	uint32_t color[TW * TH];                     // the tile's canvas pixels as they stand
+	uint32_t color[RGBA ? TW * TH : 1];          // the tile's canvas pixels as they stand
 	float depth[TW * TH];
 	uint32_t dirty[TW * TH / 32];                // pixels written, one bit each
-	uint32_t chunkWon[TW * TH / 32];             // pixels the current chunk's commands won
+	uint32_t chunkWon[RGBA ? TW * TH / 32 : 1];  // pixels the current chunk's commands won
 	int32_t scan[S::NT / 64];                    // wavefront totals of the prefix sums
-	TextureDesc textures[LDS_TEXTURES];
+	TextureDesc textures[RGBA ? LDS_TEXTURES : 1];
 	int32_t roundInfo[4];
 	int32_t slot;
 };
@@ -1437,12 +1440,12 @@ __device__ __forceinline__ int32_t segmentOf(const T *offset, int32_t count, int
 // MODE 0: the RGBA8 shaders into the colour canvas.  MODE 1: the differentiable float shader
 // into the float colour canvas plus the visibility id of each pixel's winning lane.
 template <int TW, int TH, int MODE = 0, int CH>
-__device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW, TH, CH> &s, const TextureDesc *textures) {
+__device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW, TH, CH, MODE> &s, const TextureDesc *textures) {
 	using S = DeferShape<TW, TH, CH>;
 	constexpr int32_t NT = S::NT, CHUNK = S::CHUNK;
 	constexpr int32_t NP = TW * TH, RP = TH / 2;
-	constexpr int32_t PAIRS = DeferLds<TW, TH, CH>::PAIRS;
-	constexpr int32_t ITEM_TABLE = DeferLds<TW, TH, CH>::ITEM_TABLE;
+	constexpr int32_t PAIRS = DeferLds<TW, TH, CH, MODE>::PAIRS;
+	constexpr int32_t ITEM_TABLE = DeferLds<TW, TH, CH, MODE>::ITEM_TABLE;
 	constexpr int32_t PAIRS_PER_THREAD = PAIRS / NT > 0 ? PAIRS / NT : 1;
 	static_assert(PAIRS * (TW / 2) <= 32 * NT, "a thread's items fit the 32 four-bit masks of pass 2a");
 	const int32_t n = f.tileCount[tile];
@@ -1459,7 +1462,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 	for (int32_t p = t; p < NP; p += NT) {
 		if (f.clearInRaster) {
 			s.depth[p] = f.clearDepth;
-			s.color[p] = f.clearColor;
+			if (MODE == 0) s.color[p] = f.clearColor;
 		} else {
 			const int32_t x = tileX0 + p % TW, y = tileY0 + p / TW;
 			s.depth[p] = (depthRead && x < cv.width && y < cv.height) ? cv.depth[(size_t)y * cv.depthStride + x] : 0.0f;
@@ -1520,7 +1523,8 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				const float d = s.depth[p];
 				s.keys[p] = !depthRead ? 0ull : (d != d ? ~0ull : (((unsigned long long)depthOrder(d) << 32) | 0xFFFFFFFFull));
 			}
-			for (int32_t w = t; w < NP / 32; w += NT) s.chunkWon[w] = 0u;
+			if (MODE == 0)
+				for (int32_t w = t; w < NP / 32; w += NT) s.chunkWon[w] = 0u;
 			// Quads of each (command, row pair) inside the tile: the outer span of the row pair
 			// (fillShapeSuper), cut to the tile's columns.
 			int32_t pairCount[PAIRS_PER_THREAD];
@@ -1549,7 +1553,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 				const int32_t pr = t * PAIRS_PER_THREAD + k;
 				if (pr < PAIRS) s.pairOffset[pr] = (int16_t)before;
 				for (int32_t i = before; i < before + pairCount[k] && i < ITEM_TABLE; i++)
-					s.pairOfItem[i] = (typename DeferLds<TW, TH, CH>::PairIndex)pr;
+					s.pairOfItem[i] = (typename DeferLds<TW, TH, CH, MODE>::PairIndex)pr;
 				before += pairCount[k];
 			}
 			if (t == 0) s.pairOffset[PAIRS] = (int16_t)items;
@@ -1707,7 +1711,7 @@ __device__ void rasterTileDeferred(const FrameDev &f, int32_t tile, DeferLds<TW,
 // that each gets its own register allocation (the persistent loop keeps more values live).
 template <int TW, int TH, int P>
 __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(P ? 4 : 5, P ? 4 : 5))) k_raster_deferred(FrameDev f) {
-	__shared__ DeferLds<TW, TH, colorChunk<TW, TH, P>()> s;
+	__shared__ DeferLds<TW, TH, colorChunk<TW, TH, P>(), 0> s;
 	reportCounters(f);
 	resetNextCounters(f);
 	if (f.counters[2]) return;  // an earlier stage ran out of room: leave the canvases untouched
@@ -1726,7 +1730,7 @@ __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_e
 // The differentiable forward by the visibility raster.
 template <int TW, int TH, int P>
 __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred_diff(FrameDev f) {
-	__shared__ DeferLds<TW, TH, DIFF_CHUNK> s;
+	__shared__ DeferLds<TW, TH, DIFF_CHUNK, 1> s;
 	reportCounters(f);
 	if (f.counters[2]) return;
 	forEachQueuedTile(f, P, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH, 1>(f, tile, s, nullptr); });
