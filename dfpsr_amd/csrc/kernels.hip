@@ -1727,9 +1727,15 @@ __global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_e
 	forEachQueuedTile(f, P, &s.slot, [&](int32_t tile) { rasterTileDeferred<TW, TH>(f, tile, s, textures); });
 }
 
+// Waves per SIMD of the differentiable visibility raster: five for 32x16 tiles of one workgroup
+// each (20 B of scratch, 6 % faster on the 8-view fit), four for the others, where a fifth costs
+// more in scratch than it gains.
+template <int TW, int TH, int P>
+constexpr int32_t diffWaves() { return P == 0 && TW * TH == 512 ? 5 : 4; }
+
 // The differentiable forward by the visibility raster.
 template <int TW, int TH, int P>
-__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(4, 4))) k_raster_deferred_diff(FrameDev f) {
+__global__ void __launch_bounds__(TW * TH / 4) __attribute__((amdgpu_waves_per_eu(diffWaves<TW, TH, P>(), diffWaves<TW, TH, P>()))) k_raster_deferred_diff(FrameDev f) {
 	__shared__ DeferLds<TW, TH, DIFF_CHUNK, 1> s;
 	reportCounters(f);
 	if (f.counters[2]) return;
