@@ -89,3 +89,29 @@ def test_points_replaced_by_fewer_are_refused_before_drawing(lib):
     assert rc == _lib.ERROR_ARGUMENT and b"does not have" in lib.dfpsr_hip_last_error()
     assert not canvas.any()
     lib.dfpsr_hip_model_destroy(m)
+
+
+def test_scene_give_arguments_prepared_once_and_remade_on_replacement():
+    """GpuScene.give's renderer_giveTask arguments: the instances' float32 transforms and the
+    camera reference are made once, reused while the scene's instance array and the camera object
+    stay the same, and made again when either is replaced (host logic only, no device work)."""
+    from types import SimpleNamespace
+    from dfpsr_amd import api
+    g = api.GpuScene.__new__(api.GpuScene)
+    g.scene = SimpleNamespace(instances=np.stack([np.eye(4, dtype=np.float64)[:3].reshape(-1)] * 2))
+    axes = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+    g.camera = api.camera_create(True, (0.0, 0.0, -5.0) + axes, 64, 48, 1.0)
+    g._prepared = None
+    first = g._prepared_arguments()
+    transforms, pointers = first[1], first[2]
+    assert len(transforms) == 2 and all(t.dtype == np.float32 and t.flags.c_contiguous for t in transforms)
+    assert [p.value for p in pointers] == [t.ctypes.data for t in transforms]
+    np.testing.assert_array_equal(transforms[1], g.scene.instances[1].astype(np.float32))
+    assert g._prepared_arguments() is first
+    g.camera = api.camera_create(True, (0.0, 1.0, -5.0) + axes, 64, 48, 1.0)
+    second = g._prepared_arguments()
+    assert second is not first
+    g.scene.instances = g.scene.instances[:1] * 2.0
+    third = g._prepared_arguments()
+    assert third is not second and len(third[1]) == 1
+    np.testing.assert_array_equal(third[1][0], (g.scene.instances[0]).astype(np.float32))
