@@ -85,7 +85,9 @@ def test_forward_matches_oracle(api):
 def test_forward_visibility_raster_equals_in_order_raster(api):
     """The differentiable forward through the visibility raster (winner per pixel by atomic max,
     shaded once) writes the same float colours, visibility ids and depths, bit for bit, as the
-    in-order tile walk, for every tile shape."""
+    in-order tile walk, for every tile shape.  The canvases hold other values before each
+    forward, so that the fill of the undrawn pixels (a kernel of its own, or the visibility
+    raster's write-back on 32-pixel-wide tiles) is compared too."""
     views = small_views()
     tex = torch.from_numpy(S.float_texture(64, 5)).cuda()
     mul = torch.from_numpy((1.0 + 0.3 * S.Rng(10).uniform(views[0].points.shape[0] * 3, -1, 1)).reshape(-1, 3)
@@ -94,6 +96,10 @@ def test_forward_visibility_raster_equals_in_order_raster(api):
     for tile_config in (0, 1, 2):
         for path in (0, 1):
             gv = api.GpuViews(views)
+            for i in range(len(views)):
+                gv.color[i].fill_(7.0)
+                gv.visibility[i].fill_(12345)
+                gv.depth[i].fill_(3.0)
             r = api.DiffRenderer()
             r.forward(gv.model, gv.views, tex, mul, api.tuning(clear=True, raster_path=path, tile_config=tile_config))
             torch.cuda.synchronize()
